@@ -1,0 +1,288 @@
+"""hdcodex_amd — MI355X-native spectral path-tracing core (Python binding).
+
+The native library ``libhdcx.so`` (built in-tree by ``hdcodex_amd.build``)
+carries the whole hot path; this module binds its C ABI
+(``include/hdcx.h``) with ctypes and mirrors the reference integrator class
+``hdcodex::VulkanPathTracer`` (include/hdcodex/gpu/vulkan_path_tracer.h:26-52)
+as :class:`PathTracer`. There is no Python or CPU fallback: importing works
+without the library, but every call that needs it raises ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_uint8, c_uint32, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libhdcx.so"
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_INVALID_SCENE = 2
+ERR_OUT_OF_MEMORY = 3
+ERR_DEVICE = 4
+ERR_NO_GEOMETRY = 5
+ERR_UNSUPPORTED = 6
+ERR_INTERNAL = 7
+
+SHADING_FUSED = 0
+SHADING_MODULAR = 1
+SHADING_RASTER_PREVIEW = 2
+
+KERNELS = ("raygen", "extend", "shade", "shadow", "film")
+
+# Every symbol include/hdcx.h declares.
+EXPORTED_SYMBOLS = (
+    "hdcx_last_error", "hdcx_create", "hdcx_destroy", "hdcx_set_scene", "hdcx_set_shading_mode",
+    "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_has_geometry", "hdcx_trace_closest",
+    "hdcx_trace_occluded", "hdcx_scene_create_procedural", "hdcx_scene_view_get", "hdcx_scene_camera_get",
+    "hdcx_scene_destroy", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
+)
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("origin", c_float * 3), ("lower_left", c_float * 3), ("horizontal", c_float * 3),
+                ("vertical", c_float * 3)]
+
+    @classmethod
+    def make(cls, origin, lower_left, horizontal, vertical) -> "Camera":
+        c = cls()
+        for name, value in (("origin", origin), ("lower_left", lower_left), ("horizontal", horizontal),
+                            ("vertical", vertical)):
+            getattr(c, name)[:] = [float(x) for x in value]
+        return c
+
+    def as_array(self) -> np.ndarray:
+        return np.array([list(self.origin), list(self.lower_left), list(self.horizontal), list(self.vertical)],
+                        dtype=np.float32)
+
+
+class TextureView(ctypes.Structure):
+    _fields_ = [("width", c_uint32), ("height", c_uint32), ("format", c_uint32), ("reserved", c_uint32),
+                ("texels", c_void_p)]
+
+
+class SceneView(ctypes.Structure):
+    _fields_ = [
+        ("vertex_count", c_uint32), ("triangle_count", c_uint32),
+        ("positions", POINTER(c_float)), ("indices", POINTER(c_uint32)), ("triangle_materials", POINTER(c_uint32)),
+        ("texcoords", POINTER(c_float)), ("normals", POINTER(c_float)), ("emissive", POINTER(c_float)),
+        ("material_count", c_uint32), ("light_count", c_uint32),
+        ("materials", c_void_p), ("lights", c_void_p),
+        ("texture_count", c_uint32), ("opaque", c_uint32),
+        ("textures", POINTER(TextureView)),
+    ]
+
+
+class KernelStats(ctypes.Structure):
+    _fields_ = [
+        ("launches", c_uint64 * 5), ("milliseconds", c_double * 5), ("rays", c_uint64 * 5),
+        ("nodes_visited", c_uint64), ("triangles_tested", c_uint64), ("extend_rays_counted", c_uint64),
+        ("shadow_rays_counted", c_uint64), ("shadow_nodes_visited", c_uint64), ("shadow_triangles_tested", c_uint64),
+        ("node_bytes", c_uint64), ("triangle_bytes", c_uint64), ("bvh_nodes", c_uint64), ("bvh_triangles", c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for i, k in enumerate(KERNELS):
+            out[k] = {"launches": int(self.launches[i]), "ms": float(self.milliseconds[i]), "items": int(self.rays[i])}
+        for name, _ in self._fields_[3:]:
+            out[name] = int(getattr(self, name))
+        return out
+
+
+class HdcxError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"hdcx error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Loads libhdcx.so; raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hdcodex_amd.build` "
+                           "(there is no fallback implementation)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.hdcx_last_error.restype = c_char_p
+    lib.hdcx_create.argtypes = [c_int, POINTER(c_void_p)]
+    lib.hdcx_destroy.argtypes = [c_void_p]
+    lib.hdcx_set_scene.argtypes = [c_void_p, POINTER(SceneView)]
+    lib.hdcx_set_shading_mode.argtypes = [c_void_p, c_int]
+    lib.hdcx_get_shading_mode.argtypes = [c_void_p, POINTER(c_int)]
+    lib.hdcx_render.argtypes = [c_void_p, POINTER(Camera), c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, c_void_p]
+    lib.hdcx_read_film.argtypes = [c_void_p, c_void_p]
+    lib.hdcx_has_geometry.argtypes = [c_void_p]
+    lib.hdcx_trace_closest.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
+    lib.hdcx_trace_occluded.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
+    lib.hdcx_scene_create_procedural.argtypes = [c_char_p, c_uint64, c_uint32, c_uint32, POINTER(c_void_p)]
+    lib.hdcx_scene_view_get.argtypes = [c_void_p, POINTER(SceneView)]
+    lib.hdcx_scene_camera_get.argtypes = [c_void_p, c_uint32, c_uint32, POINTER(Camera)]
+    lib.hdcx_scene_destroy.argtypes = [c_void_p]
+    lib.hdcx_set_instrumentation.argtypes = [c_void_p, c_uint32]
+    lib.hdcx_get_kernel_stats.argtypes = [c_void_p, POINTER(KernelStats)]
+    lib.hdcx_reset_kernel_stats.argtypes = [c_void_p]
+    _lib = lib
+    return lib
+
+
+def _check(code: int) -> None:
+    if code != OK:
+        raise HdcxError(code, library().hdcx_last_error().decode())
+
+
+class ProceduralScene:
+    """A seeded procedural scene packed by the native packer ("cornell",
+    "heightfield", "triangle", "materials")."""
+
+    def __init__(self, name: str, seed: int = 0, param: int = 0, dome_width: int = 0):
+        lib = library()
+        self._h = c_void_p()
+        _check(lib.hdcx_scene_create_procedural(name.encode(), seed, param, dome_width, ctypes.byref(self._h)))
+        self.name = name
+        self.view = SceneView()
+        _check(lib.hdcx_scene_view_get(self._h, ctypes.byref(self.view)))
+
+    def camera(self, width: int, height: int) -> Camera:
+        cam = Camera()
+        _check(library().hdcx_scene_camera_get(self._h, width, height, ctypes.byref(cam)))
+        return cam
+
+    @property
+    def triangle_count(self) -> int:
+        return int(self.view.triangle_count)
+
+    def arrays(self) -> dict:
+        """Copies of the packed buffers as numpy arrays."""
+        v = self.view
+        t, n = v.triangle_count, v.vertex_count
+        return {
+            "positions": np.ctypeslib.as_array(v.positions, shape=(n, 4)).copy(),
+            "indices": np.ctypeslib.as_array(v.indices, shape=(t, 3)).copy(),
+            "triangle_materials": np.ctypeslib.as_array(v.triangle_materials, shape=(t,)).copy(),
+            "texcoords": np.ctypeslib.as_array(v.texcoords, shape=(t, 3, 2)).copy(),
+            "normals": np.ctypeslib.as_array(v.normals, shape=(t, 3, 4)).copy(),
+            "emissive": np.ctypeslib.as_array(v.emissive, shape=(t, 4)).copy(),
+            "materials": np.frombuffer(ctypes.string_at(v.materials, 368 * v.material_count), dtype=np.uint8)
+            .reshape(v.material_count, 368).copy(),
+            "lights": np.frombuffer(ctypes.string_at(v.lights, 224 * max(v.light_count, 1)), dtype=np.uint8)
+            .reshape(max(v.light_count, 1), 224).copy(),
+            "light_count": int(v.light_count),
+            "opaque": bool(v.opaque),
+            "textures": [(v.textures[i].width, v.textures[i].height, v.textures[i].format)
+                         for i in range(v.texture_count)],
+        }
+
+    def close(self) -> None:
+        if self._h:
+            library().hdcx_scene_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rays_array(origins: np.ndarray, directions: np.ndarray, tmin: float = 0.001, tmax: float = 1e4) -> np.ndarray:
+    """Packs rays as the (N, 8) float32 layout of hdcx_ray."""
+    n = len(origins)
+    r = np.empty((n, 8), dtype=np.float32)
+    r[:, 0:3] = origins
+    r[:, 3] = tmin
+    r[:, 4:7] = directions
+    r[:, 7] = tmax
+    return r
+
+
+class PathTracer:
+    """Mirror of hdcodex::VulkanPathTracer over the C ABI.
+
+    ``render`` returns linear RGBA32F of shape (height, width, 4), row 0 at
+    the bottom, or an empty array when there is no geometry or the size is
+    zero (the reference returns an empty vector, :3235).
+    """
+
+    def __init__(self, device: int = 0):
+        lib = library()
+        self._h = c_void_p()
+        _check(lib.hdcx_create(device, ctypes.byref(self._h)))
+
+    def set_scene(self, scene: "ProceduralScene | SceneView | None") -> None:
+        view = scene.view if isinstance(scene, ProceduralScene) else scene
+        _check(library().hdcx_set_scene(self._h, ctypes.byref(view) if view is not None else None))
+
+    def set_shading_mode(self, mode: int) -> None:
+        code = library().hdcx_set_shading_mode(self._h, mode)
+        if code == ERR_INVALID_ARGUMENT:
+            raise ValueError(library().hdcx_last_error().decode())
+        _check(code)
+
+    def get_shading_mode(self) -> int:
+        m = c_int()
+        _check(library().hdcx_get_shading_mode(self._h, ctypes.byref(m)))
+        return int(m.value)
+
+    def has_geometry(self) -> bool:
+        return bool(library().hdcx_has_geometry(self._h))
+
+    def render(self, camera: Camera, width: int, height: int, sample_index: int, max_bounces: int = 8,
+               sample_count: int = 1, readback: bool = True) -> np.ndarray:
+        out = np.empty((height, width, 4), dtype=np.float32) if readback else None
+        code = library().hdcx_render(self._h, ctypes.byref(camera), width, height, sample_index, max_bounces,
+                                     sample_count, out.ctypes.data if out is not None else None)
+        if code == ERR_NO_GEOMETRY:
+            return np.empty((0,), dtype=np.float32)
+        _check(code)
+        return out if out is not None else np.empty((0,), dtype=np.float32)
+
+    def read_film(self, width: int, height: int) -> np.ndarray:
+        out = np.empty((height, width, 4), dtype=np.float32)
+        _check(library().hdcx_read_film(self._h, out.ctypes.data))
+        return out
+
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        """rays: (N, 8) float32 -> structured hits (t, primitive, u, v)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        hits = np.empty(len(rays), dtype=HIT_DTYPE)
+        _check(library().hdcx_trace_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data))
+        return hits
+
+    def trace_occluded(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        out = np.empty(len(rays), dtype=np.uint8)
+        _check(library().hdcx_trace_occluded(self._h, rays.ctypes.data, len(rays), out.ctypes.data))
+        return out
+
+    def set_instrumentation(self, flags: int) -> None:
+        _check(library().hdcx_set_instrumentation(self._h, flags))
+
+    def kernel_stats(self) -> dict:
+        s = KernelStats()
+        _check(library().hdcx_get_kernel_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_kernel_stats(self) -> None:
+        _check(library().hdcx_reset_kernel_stats(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            library().hdcx_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+HIT_DTYPE = np.dtype([("t", np.float32), ("primitive", np.uint32), ("u", np.float32), ("v", np.float32)])

@@ -1,0 +1,97 @@
+"""Builds the native parts in-tree.
+
+* ``hdcodex_amd/libhdcx.so`` — the product: C++ host code and HIP kernels for
+  gfx950 behind the C ABI declared in ``include/hdcx.h``.
+* ``oracle/liboracle.so`` — the CPU oracle (test infrastructure only), built
+  with ``-ffp-contract=off`` so that its FP32 arithmetic is the IEEE sequence
+  the restatement spells out.
+
+Usage: ``python -m hdcodex_amd.build [--lib] [--oracle]`` (default: both).
+"""
+
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as futures
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "hdcodex_amd"
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libhdcx.so"
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+
+ARCH = os.environ.get("HDCX_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+
+# Packed FP32 instructions are kept out of the generated code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", str(ROOT / "include"),
+]
+
+HOST_SOURCES = ["packer.cpp", "procedural.cpp", "procedural_showcase.cpp", "bvh_host.cpp"]
+HIP_SOURCES = ["kernels.hip", "tracer.cpp"]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
+
+
+def build_library() -> Path:
+    BUILD.mkdir(parents=True, exist_ok=True)
+    jobs = []
+    objects = []
+    for src in HOST_SOURCES:
+        obj = BUILD / (src + ".o")
+        objects.append(obj)
+        jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)])
+    for src in HIP_SOURCES:
+        obj = BUILD / (src + ".o")
+        objects.append(obj)
+        jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)])
+    with futures.ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as pool:
+        list(pool.map(_run, jobs))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    tmp = ORACLE_LIB.with_suffix(".so.tmp")
+    _run([
+        "g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
+        "-I", str(ROOT / "include"), str(ORACLE_DIR / "oracle.cpp"), "-o", str(tmp),
+    ])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--lib", action="store_true")
+    ap.add_argument("--oracle", action="store_true")
+    args = ap.parse_args(argv)
+    both = not (args.lib or args.oracle)
+    if both or args.oracle:
+        print("built", build_oracle())
+    if both or args.lib:
+        print("built", build_library())
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,152 @@
+// Device-side types and helpers of the wavefront integrator (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/hdcx.h"
+#include "bvh.hpp"
+#include "records.h"
+
+namespace hdcx {
+
+// ---------------------------------------------------------------------------
+// Scene as seen by the kernels. All pointers are device memory owned by the
+// tracer; the struct itself is passed by value as a kernel argument.
+// ---------------------------------------------------------------------------
+struct TexDesc {
+    std::uint64_t offset;  // bytes into the texel pool
+    std::uint32_t width, height, format, pad;
+};
+
+// Dome importance tables in a compact layout: the marginal CDF of rows
+// (height floats), and per texel the conditional CDF and the solid-angle
+// pdf. Values are those of the packed importance texture (r, g, b).
+struct ImportanceDesc {
+    std::uint64_t marginal, conditional, pdf;  // float offsets into the pool
+    std::uint32_t width, height;
+};
+
+struct DeviceScene {
+    const Bvh4Node* nodes;
+    const float4* tris;              // 3 per leaf slot
+    const float4* positions;
+    const std::uint32_t* indices;
+    const std::uint32_t* triMaterial;
+    const float2* texcoords;         // 3 per triangle
+    const float4* normals;           // 3 per triangle
+    const float4* emissive;          // 1 per triangle
+    const GpuMaterialRecord* materials;
+    const GpuLightRecord* lights;
+    const TexDesc* textures;
+    const unsigned char* texels;
+    const ImportanceDesc* importance;  // indexed by light
+    const float* importancePool;
+    std::uint32_t primitiveCount;
+    std::uint32_t lightCount;
+    std::uint32_t opaque;
+    std::uint32_t zUp;
+    std::uint32_t hasEmissive;
+    std::uint32_t textureCount;
+    std::uint32_t nodeCount;
+    std::uint32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Path state of one chunk, structure of arrays indexed by path id.
+// ---------------------------------------------------------------------------
+enum PathFlags : std::uint32_t {
+    kEnvOnMiss = 1U << 0,
+    kPrevDelta = 1U << 1,
+    kInMedium = 1U << 2,
+};
+
+constexpr int kShadowSlots = 3;
+constexpr std::uint32_t kShadowMeshLight = 0xfffffffeU;  // boolean visibility, no shadow controls
+
+struct PathBuffers {
+    float4* rayO;            // origin xyz
+    float4* rayD;            // direction xyz
+    float4* hit;             // t, primitive bits, u, v
+    std::uint32_t* rng;
+    float* wavelength;
+    float* throughput;
+    float* radiance;
+    std::uint32_t* flags;
+    float2* mis;             // previousBsdfPdf, previousEnvironmentLightPdf
+    float4* medium;          // absorption, scattering, anisotropy
+    float4* shadowO;         // [slot * capacity + path]: origin xyz, tmax
+    float4* shadowD;         // direction xyz, unoccluded contribution
+    std::uint32_t* shadowLight;  // light index, kShadowMeshLight, or ~0 = empty
+    std::uint32_t* queue[2];
+    std::uint32_t* shadowQueue;
+    std::uint32_t* counters; // [0],[1] extend queue sizes; [2] shadow queue size
+    unsigned long long* totals;  // [0] extend rays, [1] shadow-queued paths, [2] shadow rays traced
+    std::uint32_t capacity;
+};
+
+// Pixel-sample chunk: samples [s0, s1) x pixels [p0, p1); path id =
+// ((s - s0) * (p1 - p0) + (p - p0)) * 3 + lane.
+struct ChunkDesc {
+    std::uint32_t width, height;
+    std::uint32_t sampleBase;        // sample number of s = 0
+    std::uint32_t s0, s1, p0, p1;
+    std::uint32_t maxBounces;
+    float cam[12];                   // origin, lowerLeft, horizontal, vertical
+};
+
+struct TraversalCounters {
+    unsigned long long nodes;
+    unsigned long long triangles;
+    unsigned long long rays;
+};
+
+// ---------------------------------------------------------------------------
+// Small float3 algebra (explicit, so the contraction rules are visible).
+// ---------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 v3(float s) { return V3{s, s, s}; }
+__device__ __forceinline__ V3 v3(const float4& a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 v3(const F4& a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V3 normalize(V3 a) { return a / length(a); }
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ float sqf(float x) { return x * x; }
+__device__ __forceinline__ float fractf(float x) { return x - floorf(x); }
+__device__ __forceinline__ V3 reflect3(V3 i, V3 n) { return i - n * (2.0f * dot(n, i)); }
+__device__ __forceinline__ V3 vmax(V3 a, float s) { return {fmaxf(a.x, s), fmaxf(a.y, s), fmaxf(a.z, s)}; }
+
+// lowbias32 integer hash and its FP32 uniform (reference
+// src/gpu/vulkan_path_tracer.cpp:100-113); bit-exact with any IEEE host.
+__device__ __forceinline__ std::uint32_t hashState(std::uint32_t v)
+{
+    v ^= v >> 16;
+    v *= 0x7feb352dU;
+    v ^= v >> 15;
+    v *= 0x846ca68bU;
+    return v ^ (v >> 16);
+}
+
+__device__ __forceinline__ float randomFloat(std::uint32_t& state)
+{
+    state = hashState(state);
+    return __uint2float_rn(state) * (1.0f / 4294967296.0f);
+}
+
+} // namespace hdcx

@@ -1,0 +1,265 @@
+// Wide-BVH traversal for gfx950: one ray per lane, a four-child node per
+// 128-byte fetch, near-to-far ordered stack whose first 16 entries live in
+// LDS (lane-interleaved, conflict-free) and the rest in private memory.
+//
+// Hit semantics (what the reference leaves to the Vulkan intersector,
+// restated as a definition): a triangle is hit when the watertight test of
+// Woop, Benthin and Wald (JCGT 2013) passes with tmin <= t <= tmax; the
+// closest hit is the smallest (t, primitive id) pair. The triangle test runs
+// with FP contraction off and IEEE division, so t, u, v are bit-identical to
+// any IEEE FP32 host evaluating the same operations.
+#pragma once
+
+#include "dev_common.h"
+
+namespace hdcx {
+
+constexpr int kLdsStack = 16;
+constexpr int kSpillStack = 48;
+constexpr int kMaxStack = kLdsStack + kSpillStack;
+
+struct RayPre {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float ix, iy, iz;  // reciprocal direction for the slab test
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+
+struct HitRec {
+    float t;
+    std::uint32_t prim;
+    float u, v;
+};
+
+__device__ __forceinline__ float pick(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+
+__device__ __forceinline__ float safeRcp(float d)
+{
+    // Axis-parallel rays: a huge finite reciprocal keeps the slab test
+    // free of 0 * inf.
+    const float tiny = 1e-30f;
+    return 1.0f / (fabsf(d) > tiny ? d : copysignf(tiny, d));
+}
+
+__device__ __forceinline__ RayPre prepareRay(float ox, float oy, float oz, float dx, float dy, float dz)
+{
+    RayPre r;
+    r.ox = ox; r.oy = oy; r.oz = oz;
+    r.dx = dx; r.dy = dy; r.dz = dz;
+    r.ix = safeRcp(dx); r.iy = safeRcp(dy); r.iz = safeRcp(dz);
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = 0;
+    float m = ax;
+    if (ay > m) { kz = 1; m = ay; }
+    if (az > m) { kz = 2; }
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    const float dkz = pick(dx, dy, dz, kz);
+    if (dkz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.sx = pick(dx, dy, dz, kx) / dkz;
+    r.sy = pick(dx, dy, dz, ky) / dkz;
+    r.sz = 1.0f / dkz;
+    return r;
+}
+
+// Watertight ray/triangle test; returns true with t in [tmin, tmax].
+__device__ __forceinline__ bool intersectTriangle(const RayPre& r, const float4& p0, const float4& p1,
+                                                  const float4& p2, float tmin, float tmax, float& t, float& u,
+                                                  float& v)
+{
+#pragma clang fp contract(off)
+    const float ax0 = p0.x - r.ox, ay0 = p0.y - r.oy, az0 = p0.z - r.oz;
+    const float bx0 = p1.x - r.ox, by0 = p1.y - r.oy, bz0 = p1.z - r.oz;
+    const float cx0 = p2.x - r.ox, cy0 = p2.y - r.oy, cz0 = p2.z - r.oz;
+    const float aK = pick(ax0, ay0, az0, r.kz), bK = pick(bx0, by0, bz0, r.kz), cK = pick(cx0, cy0, cz0, r.kz);
+    const float ax = pick(ax0, ay0, az0, r.kx) - r.sx * aK, ay = pick(ax0, ay0, az0, r.ky) - r.sy * aK;
+    const float bx = pick(bx0, by0, bz0, r.kx) - r.sx * bK, by = pick(bx0, by0, bz0, r.ky) - r.sy * bK;
+    const float cx = pick(cx0, cy0, cz0, r.kx) - r.sx * cK, cy = pick(cx0, cy0, cz0, r.ky) - r.sy * cK;
+    float U = cx * by - cy * bx;
+    float V = ax * cy - ay * cx;
+    float W = bx * ay - by * ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        const double cxd = cx, cyd = cy, bxd = bx, byd = by, axd = ax, ayd = ay;
+        U = static_cast<float>(cxd * byd - cyd * bxd);
+        V = static_cast<float>(axd * cyd - ayd * cxd);
+        W = static_cast<float>(bxd * ayd - byd * axd);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float az = r.sz * aK, bz = r.sz * bK, cz = r.sz * cK;
+    const float T = U * az + V * bz + W * cz;
+    t = T / det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    u = V / det;
+    v = W / det;
+    return true;
+}
+
+// Slab test of the four children of a node; returns near distances
+// (+inf when missed). tfar is inflated by 4 ulp so that the two roundings
+// of each plane distance never cull a box the ray touches.
+__device__ __forceinline__ void testNode(const Bvh4Node* __restrict__ node, const RayPre& r, float tmin, float tmax,
+                                         float tn[4], std::uint32_t ref[4])
+{
+    const float4* n4 = reinterpret_cast<const float4*>(node);
+    const float4 lox = n4[0], hix = n4[1], loy = n4[2], hiy = n4[3], loz = n4[4], hiz = n4[5];
+    const uint4 ch = reinterpret_cast<const uint4*>(node)[6];
+    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    const std::uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float tx0 = (lx[i] - r.ox) * r.ix, tx1 = (hx[i] - r.ox) * r.ix;
+        const float ty0 = (ly[i] - r.oy) * r.iy, ty1 = (hy[i] - r.oy) * r.iy;
+        const float tz0 = (lz[i] - r.oz) * r.iz, tz1 = (hz[i] - r.oz) * r.iz;
+        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const bool hit = tnear <= tfar * 1.00000048f;
+        tn[i] = hit ? tnear : INFINITY;
+        ref[i] = c[i];
+    }
+}
+
+struct TraversalStack {
+    std::uint32_t* lds;      // this lane's column; stride = block size
+    std::uint32_t stride;
+    std::uint32_t spill[kSpillStack];
+    int sp;
+    __device__ __forceinline__ void push(std::uint32_t v)
+    {
+        if (sp < kLdsStack) lds[sp * stride] = v;
+        else spill[sp - kLdsStack] = v;
+        ++sp;
+    }
+    __device__ __forceinline__ std::uint32_t pop()
+    {
+        --sp;
+        return sp < kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+    }
+};
+
+#define HDCX_SWAP_IF(a, b)                                        \
+    if (tn[b] < tn[a]) {                                          \
+        const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;        \
+        const std::uint32_t rr = ref[a]; ref[a] = ref[b]; ref[b] = rr; \
+    }
+
+// Closest hit with a lexicographic lower bound: only triangles with
+// (t, prim) > (loT, loPrim) are accepted (loPrim = 0xffffffff with
+// loT = tmin accepts everything from tmin on). Used directly for opaque
+// scenes and iterated for candidate-ordered any-hit in non-opaque ones.
+template <bool kCount>
+__device__ bool traceClosest(const DeviceScene& s, const RayPre& r, float tmin, float tmax, float loT,
+                             std::uint32_t loPrim, HitRec& best, TraversalStack& st, TraversalCounters* ctr)
+{
+    best.t = tmax;
+    best.prim = 0xffffffffU;
+    best.u = best.v = 0.0f;
+    if (s.nodeCount == 0) return false;
+    bool found = false;
+    st.sp = 0;
+    std::uint32_t cur = 0;  // root is an inner node
+    unsigned long long nodes = 0, tris = 0;
+    for (;;) {
+        if (cur & kLeafFlag) {
+            const std::uint32_t first = (cur >> 3) & 0x0fffffffU;
+            const std::uint32_t count = (cur & 7U) + 1U;
+            for (std::uint32_t j = 0; j < count; ++j) {
+                const float4* tp = s.tris + static_cast<std::size_t>(first + j) * 3;
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                const std::uint32_t prim = __float_as_uint(a.w);
+                if (kCount) ++tris;
+                float t, u, v;
+                if (intersectTriangle(r, a, b, c, tmin, best.t, t, u, v)) {
+                    const bool aboveLo = loPrim == 0xffffffffU || t > loT || (t == loT && prim > loPrim);
+                    const bool better = !found || t < best.t || (t == best.t && prim < best.prim);
+                    if (aboveLo && better) {
+                        found = true;
+                        best.t = t;
+                        best.prim = prim;
+                        best.u = u;
+                        best.v = v;
+                    }
+                }
+            }
+        } else {
+            if (kCount) ++nodes;
+            float tn[4];
+            std::uint32_t ref[4];
+            testNode(s.nodes + cur, r, tmin, best.t, tn, ref);
+            HDCX_SWAP_IF(0, 1) HDCX_SWAP_IF(2, 3) HDCX_SWAP_IF(0, 2) HDCX_SWAP_IF(1, 3) HDCX_SWAP_IF(1, 2)
+            if (tn[0] != INFINITY) {
+                if (tn[3] != INFINITY) st.push(ref[3]);
+                if (tn[2] != INFINITY) st.push(ref[2]);
+                if (tn[1] != INFINITY) st.push(ref[1]);
+                cur = ref[0];
+                continue;
+            }
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    if (kCount && ctr) {
+        atomicAdd(&ctr->nodes, nodes);
+        atomicAdd(&ctr->triangles, tris);
+    }
+    return found;
+}
+
+// Any hit in [tmin, tmax] (terminate on first hit).
+template <bool kCount>
+__device__ bool traceAny(const DeviceScene& s, const RayPre& r, float tmin, float tmax, TraversalStack& st,
+                         TraversalCounters* ctr)
+{
+    if (s.nodeCount == 0) return false;
+    st.sp = 0;
+    std::uint32_t cur = 0;
+    bool hitAny = false;
+    unsigned long long nodes = 0, tris = 0;
+    for (;;) {
+        if (cur & kLeafFlag) {
+            const std::uint32_t first = (cur >> 3) & 0x0fffffffU;
+            const std::uint32_t count = (cur & 7U) + 1U;
+            for (std::uint32_t j = 0; j < count; ++j) {
+                const float4* tp = s.tris + static_cast<std::size_t>(first + j) * 3;
+                if (kCount) ++tris;
+                float t, u, v;
+                if (intersectTriangle(r, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) {
+                    hitAny = true;
+                    break;
+                }
+            }
+            if (hitAny) break;
+        } else {
+            if (kCount) ++nodes;
+            float tn[4];
+            std::uint32_t ref[4];
+            testNode(s.nodes + cur, r, tmin, tmax, tn, ref);
+            std::uint32_t next = 0xffffffffU;
+            bool have = false;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (tn[i] != INFINITY) {
+                    if (!have) { next = ref[i]; have = true; }
+                    else st.push(ref[i]);
+                }
+            }
+            if (have) { cur = next; continue; }
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    if (kCount && ctr) {
+        atomicAdd(&ctr->nodes, nodes);
+        atomicAdd(&ctr->triangles, tris);
+    }
+    return hitAny;
+}
+
+#undef HDCX_SWAP_IF
+
+} // namespace hdcx

@@ -1,0 +1,24 @@
+// Host-side launchers of the integrator kernels (defined in kernels.hip).
+#pragma once
+
+#include "dev_common.h"
+
+namespace hdcx {
+
+hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t paths, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+                        unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
+hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
+                       std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                        TraversalCounters* ctr, hipStream_t st);
+hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchResolve(double4* mean, const double4* batch, float4* out, std::uint32_t pixels, std::uint32_t sampleIndex,
+                         std::uint32_t sampleCount, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchZero(double4* p, std::uint32_t n, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchQueryClosest(const DeviceScene& s, const float4* rays, std::uint32_t n, float4* hits, unsigned maxBlocks,
+                              hipStream_t st);
+hipError_t LaunchQueryAny(const DeviceScene& s, const float4* rays, std::uint32_t n, unsigned char* out, unsigned maxBlocks,
+                          hipStream_t st);
+
+} // namespace hdcx

@@ -1,0 +1,53 @@
+// Host-side scene packer: SceneSnapshot -> the flattened buffers the device
+// consumes (hdcx_scene_view).
+#pragma once
+
+#include "../../include/hdcx.h"
+#include "../../include/hdcx_scene.hpp"
+#include "records.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace hdcx {
+
+struct PackedTexture {
+    std::uint32_t width{0};
+    std::uint32_t height{0};
+    std::uint32_t format{HDCX_TEXTURE_RGBA8_UNORM};
+    std::vector<std::uint8_t> rgba8;
+    std::vector<float> rgba32f;
+};
+
+struct PackedScene {
+    std::vector<float> positions;            // 4 per vertex
+    std::vector<std::uint32_t> indices;      // 3 per triangle
+    std::vector<std::uint32_t> triangleMaterials;
+    std::vector<float> texcoords;            // 6 per triangle
+    std::vector<float> normals;              // 12 per triangle
+    std::vector<float> emissive;             // 4 per triangle
+    std::vector<GpuMaterialRecord> materials;
+    std::vector<GpuLightRecord> lights;      // >= 1 (dummy when none authored)
+    std::uint32_t lightCount{0};             // authored, visible, <= 64
+    std::vector<PackedTexture> textures;
+    std::vector<hdcx_texture_view> textureViews;
+    bool opaque{true};
+
+    [[nodiscard]] bool HasGeometry() const noexcept
+    {
+        return !positions.empty() && !indices.empty();
+    }
+    // The view points into this object; refreshes textureViews.
+    hdcx_scene_view View();
+};
+
+// Restates the reference's BuildScene host flattening and packing
+// (src/gpu/vulkan_path_tracer.cpp:2604-2993).
+PackedScene PackScene(const SceneSnapshot& snapshot);
+
+// src/gpu/vulkan_path_tracer.cpp:1968-1996
+float TextureAverageLuminance(const SceneTexture& source);
+// src/gpu/vulkan_path_tracer.cpp:1998-2066; empty result when unusable.
+SceneTexture BuildLatLongImportanceTexture(const SceneTexture& source);
+
+} // namespace hdcx

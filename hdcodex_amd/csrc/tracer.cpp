@@ -1,0 +1,687 @@
+// C ABI of the hdcx path-tracing core: tracer lifetime, scene upload, the
+// progressive render loop, ray queries, procedural scenes and measurement.
+// Errors never cross the ABI as exceptions: every entry point maps them to
+// a status code and a thread-local message.
+#include "../../include/hdcx.h"
+#include "kernel_api.h"
+#include "packer.hpp"
+#include "procedural.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace hdcx {
+namespace {
+
+thread_local std::string gLastError;
+
+struct Failure : std::runtime_error {
+    int code;
+    Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void Check(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        throw Failure(HDCX_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+    throw Failure(HDCX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int Guard(F&& f)
+{
+    try {
+        f();
+        gLastError.clear();
+        return HDCX_OK;
+    } catch (const Failure& e) {
+        gLastError = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        gLastError = "host allocation failed";
+        return HDCX_ERR_OUT_OF_MEMORY;
+    } catch (const std::invalid_argument& e) {
+        gLastError = e.what();
+        return HDCX_ERR_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        gLastError = e.what();
+        return HDCX_ERR_INTERNAL;
+    }
+}
+
+// Owning device allocation.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { Reset(); }
+    void Reset()
+    {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    void Allocate(std::size_t bytes)
+    {
+        if (bytes == bytes_ && ptr_) return;
+        Reset();
+        if (bytes == 0) return;
+        Check(hipMalloc(&ptr_, bytes), "hipMalloc");
+        bytes_ = bytes;
+    }
+    void Upload(const void* host, std::size_t bytes, hipStream_t st)
+    {
+        Allocate(bytes);
+        if (bytes) Check(hipMemcpyAsync(ptr_, host, bytes, hipMemcpyHostToDevice, st), "upload");
+    }
+    template <class T> T* as() const { return static_cast<T*>(ptr_); }
+    [[nodiscard]] std::size_t bytes() const { return bytes_; }
+
+private:
+    void* ptr_ = nullptr;
+    std::size_t bytes_ = 0;
+};
+
+constexpr std::uint32_t kChunkPixelSamples = 1U << 21;  // 6.3M lane paths per chunk
+
+} // namespace
+} // namespace hdcx
+
+using namespace hdcx;
+
+struct hdcx_scene {
+    ProceduralScene procedural;
+    PackedScene packed;
+};
+
+struct hdcx_tracer {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    unsigned maxBlocks = 1024;
+    int shadingMode = HDCX_SHADING_FUSED;
+    bool geometry = false;
+    bool fallbackInit = false;
+    bool fallbackZUp = false;
+    DeviceScene scene{};
+    // Scene storage.
+    DeviceBuffer nodes, tris, positions, indices, triMaterial, texcoords, normals, emissive, materials, lights,
+        textures, texels, importance, importancePool;
+    Bvh4 hostBvhInfo;  // depth / stack statistics of the uploaded BVH
+    // Path state.
+    DeviceBuffer rayO, rayD, hit, rng, wavelength, throughput, radiance, flags, mis, medium, shadowO, shadowD,
+        shadowLight, queue0, queue1, shadowQueue, counters, totals, traversalCounters;
+    std::uint32_t capacity = 0;
+    // Film.
+    DeviceBuffer filmMean, filmBatch, filmOut;
+    std::uint32_t filmWidth = 0, filmHeight = 0;
+    // Ray queries.
+    DeviceBuffer queryRays, queryHits;
+    // Instrumentation.
+    std::uint32_t instrumentation = 0;
+    hdcx_kernel_stats stats{};
+    std::vector<hipEvent_t> eventPool;
+    struct Timed { int kernel; hipEvent_t a, b; };
+    std::vector<Timed> pending;
+    std::size_t eventsUsed = 0;
+
+    ~hdcx_tracer()
+    {
+        for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void Activate() const { Check(hipSetDevice(device), "hipSetDevice"); }
+
+    hipEvent_t NextEvent()
+    {
+        if (eventsUsed == eventPool.size()) {
+            hipEvent_t e;
+            Check(hipEventCreate(&e), "hipEventCreate");
+            eventPool.push_back(e);
+        }
+        return eventPool[eventsUsed++];
+    }
+
+    template <class Launch>
+    void Timed(int kernel, Launch&& launch)
+    {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (instrumentation & 1U) {
+            a = NextEvent();
+            b = NextEvent();
+            Check(hipEventRecord(a, stream), "hipEventRecord");
+        }
+        Check(launch(), "kernel launch");
+        if (instrumentation & 1U) {
+            Check(hipEventRecord(b, stream), "hipEventRecord");
+            pending.push_back({kernel, a, b});
+        }
+        ++stats.launches[kernel];
+    }
+
+    void CollectTimings()
+    {
+        for (const auto& t : pending) {
+            float ms = 0.0F;
+            Check(hipEventElapsedTime(&ms, t.a, t.b), "hipEventElapsedTime");
+            stats.milliseconds[t.kernel] += ms;
+        }
+        pending.clear();
+        eventsUsed = 0;
+    }
+
+    PathBuffers Paths()
+    {
+        PathBuffers pb{};
+        pb.rayO = rayO.as<float4>();
+        pb.rayD = rayD.as<float4>();
+        pb.hit = hit.as<float4>();
+        pb.rng = rng.as<std::uint32_t>();
+        pb.wavelength = wavelength.as<float>();
+        pb.throughput = throughput.as<float>();
+        pb.radiance = radiance.as<float>();
+        pb.flags = flags.as<std::uint32_t>();
+        pb.mis = mis.as<float2>();
+        pb.medium = medium.as<float4>();
+        pb.shadowO = shadowO.as<float4>();
+        pb.shadowD = shadowD.as<float4>();
+        pb.shadowLight = shadowLight.as<std::uint32_t>();
+        pb.queue[0] = queue0.as<std::uint32_t>();
+        pb.queue[1] = queue1.as<std::uint32_t>();
+        pb.shadowQueue = shadowQueue.as<std::uint32_t>();
+        pb.counters = counters.as<std::uint32_t>();
+        pb.totals = totals.as<unsigned long long>();
+        pb.capacity = capacity;
+        return pb;
+    }
+
+    void EnsurePaths(std::uint32_t paths)
+    {
+        if (paths <= capacity) return;
+        const std::size_t n = paths;
+        rayO.Allocate(n * 16);
+        rayD.Allocate(n * 16);
+        hit.Allocate(n * 16);
+        rng.Allocate(n * 4);
+        wavelength.Allocate(n * 4);
+        throughput.Allocate(n * 4);
+        radiance.Allocate(n * 4);
+        flags.Allocate(n * 4);
+        mis.Allocate(n * 8);
+        medium.Allocate(n * 16);
+        shadowO.Allocate(n * 16 * kShadowSlots);
+        shadowD.Allocate(n * 16 * kShadowSlots);
+        shadowLight.Allocate(n * 4 * kShadowSlots);
+        queue0.Allocate(n * 4);
+        queue1.Allocate(n * 4);
+        shadowQueue.Allocate(n * 4);
+        capacity = paths;
+    }
+
+    void EnsureFilm(std::uint32_t w, std::uint32_t h)
+    {
+        if (w == filmWidth && h == filmHeight && filmMean.bytes()) return;
+        const std::size_t n = static_cast<std::size_t>(w) * h;
+        filmMean.Allocate(n * sizeof(double4));
+        filmBatch.Allocate(n * sizeof(double4));
+        filmOut.Allocate(n * sizeof(float4));
+        filmWidth = w;
+        filmHeight = h;
+        // A resized film starts from zero (reference EnsureOutput :3130-3160).
+        Check(LaunchZero(filmMean.as<double4>(), static_cast<std::uint32_t>(n), maxBlocks, stream), "zero film");
+    }
+};
+
+namespace hdcx {
+namespace {
+
+void ValidateTextureRef(std::uint32_t ti, std::uint32_t textureCount, const char* what)
+{
+    if (ti == kMissingTexture) return;
+    if (ti & kUdimTextureFlag) {
+        const std::uint32_t base = ti & 0xffU;
+        const std::uint32_t tiles = static_cast<std::uint32_t>(__builtin_popcount((ti >> 8U) & kUdimTileMask));
+        if (base + tiles > textureCount) throw Failure(HDCX_ERR_INVALID_SCENE, std::string(what) + ": UDIM set out of range");
+        return;
+    }
+    if (ti >= textureCount) throw Failure(HDCX_ERR_INVALID_SCENE, std::string(what) + ": texture index out of range");
+}
+
+void ValidateScene(const hdcx_scene_view& v)
+{
+    if (v.triangle_count && (!v.indices || !v.triangle_materials || !v.texcoords || !v.normals || !v.emissive))
+        throw Failure(HDCX_ERR_INVALID_SCENE, "triangle arrays missing");
+    if (v.vertex_count && !v.positions) throw Failure(HDCX_ERR_INVALID_SCENE, "positions missing");
+    if (v.material_count == 0 || !v.materials) throw Failure(HDCX_ERR_INVALID_SCENE, "no materials");
+    if (v.light_count > HDCX_MAX_LIGHTS) throw Failure(HDCX_ERR_INVALID_SCENE, "more than 64 lights");
+    if (!v.lights) throw Failure(HDCX_ERR_INVALID_SCENE, "light table missing");
+    if (v.texture_count > HDCX_MAX_TEXTURES) throw Failure(HDCX_ERR_INVALID_SCENE, "more than 256 textures");
+    if (v.texture_count && !v.textures) throw Failure(HDCX_ERR_INVALID_SCENE, "texture views missing");
+    for (std::uint64_t i = 0; i < static_cast<std::uint64_t>(v.vertex_count) * 4; ++i)
+        if (!std::isfinite(v.positions[i])) throw Failure(HDCX_ERR_INVALID_SCENE, "non-finite vertex position");
+    for (std::uint64_t i = 0; i < static_cast<std::uint64_t>(v.triangle_count) * 3; ++i)
+        if (v.indices[i] >= v.vertex_count) throw Failure(HDCX_ERR_INVALID_SCENE, "vertex index out of range");
+    for (std::uint32_t t = 0; t < v.triangle_count; ++t)
+        if (v.triangle_materials[t] >= v.material_count) throw Failure(HDCX_ERR_INVALID_SCENE, "material index out of range");
+    for (std::uint32_t i = 0; i < v.texture_count; ++i) {
+        const hdcx_texture_view& t = v.textures[i];
+        if (t.width == 0 || t.height == 0 || !t.texels || t.format > HDCX_TEXTURE_RGBA32F)
+            throw Failure(HDCX_ERR_INVALID_SCENE, "invalid texture");
+    }
+    const auto* mats = static_cast<const GpuMaterialRecord*>(v.materials);
+    for (std::uint32_t m = 0; m < v.material_count; ++m) {
+        const U4* slots[6] = {&mats[m].textureIndices0, &mats[m].textureIndices1, &mats[m].textureIndices2,
+                              &mats[m].textureIndices3, &mats[m].textureIndices4, &mats[m].textureIndices5};
+        for (const U4* u : slots) {
+            ValidateTextureRef(u->x, v.texture_count, "material");
+            ValidateTextureRef(u->y, v.texture_count, "material");
+            ValidateTextureRef(u->z, v.texture_count, "material");
+            ValidateTextureRef(u->w, v.texture_count, "material");
+        }
+    }
+    const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
+    for (std::uint32_t l = 0; l < v.light_count; ++l) {
+        ValidateTextureRef(ls[l].textureInfo.x, v.texture_count, "light");
+        if (ls[l].textureInfo.w != kMissingTexture) {
+            if (ls[l].textureInfo.w >= v.texture_count ||
+                v.textures[ls[l].textureInfo.w].format != HDCX_TEXTURE_RGBA32F)
+                throw Failure(HDCX_ERR_INVALID_SCENE, "light importance texture invalid");
+        }
+    }
+}
+
+void SetScene(hdcx_tracer* t, const hdcx_scene_view* view)
+{
+    t->Activate();
+    t->geometry = false;
+    if (!view || view->vertex_count == 0 || view->triangle_count == 0) return;
+    ValidateScene(*view);
+    const hdcx_scene_view& v = *view;
+    hipStream_t st = t->stream;
+
+    Bvh4 bvh = BuildBvh4Host(v.positions, v.indices, v.triangle_count);
+    if (bvh.maxStack + 1 > static_cast<std::uint32_t>(64))
+        throw Failure(HDCX_ERR_INVALID_SCENE, "BVH deeper than the traversal stack");
+    t->nodes.Upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), st);
+    t->tris.Upload(bvh.triangles.data(), bvh.triangles.size() * sizeof(float), st);
+    t->positions.Upload(v.positions, static_cast<std::size_t>(v.vertex_count) * 16, st);
+    t->indices.Upload(v.indices, static_cast<std::size_t>(v.triangle_count) * 12, st);
+    t->triMaterial.Upload(v.triangle_materials, static_cast<std::size_t>(v.triangle_count) * 4, st);
+    t->texcoords.Upload(v.texcoords, static_cast<std::size_t>(v.triangle_count) * 24, st);
+    t->normals.Upload(v.normals, static_cast<std::size_t>(v.triangle_count) * 48, st);
+    t->emissive.Upload(v.emissive, static_cast<std::size_t>(v.triangle_count) * 16, st);
+    t->materials.Upload(v.materials, static_cast<std::size_t>(v.material_count) * HDCX_MATERIAL_BYTES, st);
+    const std::uint32_t lightRecords = std::max<std::uint32_t>(v.light_count, 1);
+    t->lights.Upload(v.lights, static_cast<std::size_t>(lightRecords) * HDCX_LIGHT_BYTES, st);
+
+    // Texel pool (16-byte aligned entries).
+    std::vector<TexDesc> descs(std::max<std::uint32_t>(v.texture_count, 1));
+    std::size_t poolBytes = 0;
+    for (std::uint32_t i = 0; i < v.texture_count; ++i) {
+        const hdcx_texture_view& tv = v.textures[i];
+        const std::size_t bytes = static_cast<std::size_t>(tv.width) * tv.height * (tv.format == HDCX_TEXTURE_RGBA32F ? 16 : 4);
+        descs[i] = TexDesc{poolBytes, tv.width, tv.height, tv.format, 0};
+        poolBytes += (bytes + 15) & ~static_cast<std::size_t>(15);
+    }
+    std::vector<unsigned char> pool(std::max<std::size_t>(poolBytes, 16));
+    for (std::uint32_t i = 0; i < v.texture_count; ++i) {
+        const hdcx_texture_view& tv = v.textures[i];
+        const std::size_t bytes = static_cast<std::size_t>(tv.width) * tv.height * (tv.format == HDCX_TEXTURE_RGBA32F ? 16 : 4);
+        std::memcpy(pool.data() + descs[i].offset, tv.texels, bytes);
+    }
+    t->textures.Upload(descs.data(), descs.size() * sizeof(TexDesc), st);
+    t->texels.Upload(pool.data(), pool.size(), st);
+
+    // Dome importance tables re-laid out per light: marginal CDF column,
+    // conditional CDF and pdf planes of the packed importance texture.
+    const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
+    std::vector<ImportanceDesc> imp(lightRecords, ImportanceDesc{0, 0, 0, 0, 0});
+    std::vector<float> impPool;
+    for (std::uint32_t l = 0; l < v.light_count; ++l) {
+        const std::uint32_t ti = ls[l].textureInfo.w;
+        if (ls[l].textureInfo.z != 0U || ti == kMissingTexture) continue;
+        const hdcx_texture_view& tv = v.textures[ti];
+        const float* px = static_cast<const float*>(tv.texels);
+        ImportanceDesc d{};
+        d.width = tv.width;
+        d.height = tv.height;
+        d.marginal = impPool.size();
+        for (std::uint32_t y = 0; y < tv.height; ++y) impPool.push_back(px[(static_cast<std::size_t>(y) * tv.width) * 4 + 1]);
+        d.conditional = impPool.size();
+        for (std::size_t p = 0; p < static_cast<std::size_t>(tv.width) * tv.height; ++p) impPool.push_back(px[p * 4 + 0]);
+        d.pdf = impPool.size();
+        for (std::size_t p = 0; p < static_cast<std::size_t>(tv.width) * tv.height; ++p) impPool.push_back(px[p * 4 + 2]);
+        imp[l] = d;
+    }
+    if (impPool.empty()) impPool.push_back(0.0F);
+    t->importance.Upload(imp.data(), imp.size() * sizeof(ImportanceDesc), st);
+    t->importancePool.Upload(impPool.data(), impPool.size() * sizeof(float), st);
+    Check(hipStreamSynchronize(st), "scene upload");
+
+    DeviceScene& s = t->scene;
+    s.nodes = t->nodes.as<Bvh4Node>();
+    s.tris = t->tris.as<float4>();
+    s.positions = t->positions.as<float4>();
+    s.indices = t->indices.as<std::uint32_t>();
+    s.triMaterial = t->triMaterial.as<std::uint32_t>();
+    s.texcoords = t->texcoords.as<float2>();
+    s.normals = t->normals.as<float4>();
+    s.emissive = t->emissive.as<float4>();
+    s.materials = t->materials.as<GpuMaterialRecord>();
+    s.lights = t->lights.as<GpuLightRecord>();
+    s.textures = t->textures.as<TexDesc>();
+    s.texels = t->texels.as<unsigned char>();
+    s.importance = t->importance.as<ImportanceDesc>();
+    s.importancePool = t->importancePool.as<float>();
+    s.primitiveCount = v.triangle_count;
+    s.lightCount = v.light_count;
+    s.opaque = v.opaque ? 1U : 0U;
+    s.hasEmissive = v.emissive[(static_cast<std::size_t>(v.triangle_count) - 1) * 4] > 0.0F ? 1U : 0U;
+    s.textureCount = v.texture_count;
+    s.nodeCount = static_cast<std::uint32_t>(bvh.nodes.size());
+    t->hostBvhInfo.maxStack = bvh.maxStack;
+    t->hostBvhInfo.depth = bvh.depth;
+    t->stats.node_bytes = sizeof(Bvh4Node);
+    t->stats.triangle_bytes = 48;
+    t->stats.bvh_nodes = bvh.nodes.size();
+    t->stats.bvh_triangles = bvh.triangles.size() / 12;
+    t->geometry = true;
+}
+
+void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32_t h, std::uint32_t sampleIndex,
+            std::uint32_t maxBounces, std::uint32_t sampleCount, float* out)
+{
+    t->Activate();
+    if (!t->fallbackInit) {
+        // Fallback sky axis from the first camera (:3237-3241).
+        t->fallbackZUp = std::fabs(cam.vertical[2]) > std::fabs(cam.vertical[1]);
+        t->fallbackInit = true;
+    }
+    t->scene.zUp = t->fallbackZUp ? 1U : 0U;
+    const std::uint32_t samples = std::max(sampleCount, 1U);
+    const std::uint32_t bounces = std::clamp(maxBounces, 1U, kMaxBounces);
+    const std::uint32_t pixels = w * h;
+    t->EnsureFilm(w, h);
+    hipStream_t st = t->stream;
+    Check(LaunchZero(t->filmBatch.as<double4>(), pixels, t->maxBlocks, st), "zero batch");
+    if (!t->counters.bytes()) {
+        t->counters.Allocate(64);
+        t->totals.Allocate(64);
+        Check(hipMemsetAsync(t->totals.as<void>(), 0, 64, st), "memset totals");
+        t->traversalCounters.Allocate(sizeof(TraversalCounters) * 2);
+        Check(hipMemsetAsync(t->traversalCounters.as<void>(), 0, sizeof(TraversalCounters) * 2, st), "memset counters");
+    }
+    // Chunking: whole frames of k samples when a frame is small, else pixel
+    // ranges of one sample; no pixel appears twice within a chunk.
+    std::uint32_t pixelsPerChunk = std::min(pixels, kChunkPixelSamples);
+    std::uint32_t samplesPerChunk = pixels <= kChunkPixelSamples ? std::max(1U, kChunkPixelSamples / pixels) : 1U;
+    samplesPerChunk = std::min(samplesPerChunk, samples);
+    t->EnsurePaths(pixelsPerChunk * samplesPerChunk * 3U);
+    PathBuffers pb = t->Paths();
+    TraversalCounters* ctr = (t->instrumentation & 2U) ? t->traversalCounters.as<TraversalCounters>() : nullptr;
+    TraversalCounters* shadowCtr = ctr ? ctr + 1 : nullptr;
+    ChunkDesc c{};
+    c.width = w;
+    c.height = h;
+    c.sampleBase = sampleIndex;
+    c.maxBounces = bounces;
+    std::memcpy(c.cam + 0, cam.origin, 12);
+    std::memcpy(c.cam + 3, cam.lower_left, 12);
+    std::memcpy(c.cam + 6, cam.horizontal, 12);
+    std::memcpy(c.cam + 9, cam.vertical, 12);
+    for (std::uint32_t s0 = 0; s0 < samples; s0 += samplesPerChunk) {
+        const std::uint32_t s1 = std::min(samples, s0 + samplesPerChunk);
+        for (std::uint32_t p0 = 0; p0 < pixels; p0 += pixelsPerChunk) {
+            const std::uint32_t p1 = std::min(pixels, p0 + pixelsPerChunk);
+            c.s0 = s0;
+            c.s1 = s1;
+            c.p0 = p0;
+            c.p1 = p1;
+            const std::uint32_t paths = (s1 - s0) * (p1 - p0) * 3U;
+            t->Timed(HDCX_KERNEL_RAYGEN, [&] { return LaunchRaygen(pb, c, paths, t->maxBlocks, st); });
+            t->stats.rays[HDCX_KERNEL_RAYGEN] += paths;
+            for (std::uint32_t b = 0; b < bounces; ++b) {
+                const std::uint32_t in = b & 1U;
+                t->Timed(HDCX_KERNEL_EXTEND, [&] { return LaunchExtend(t->scene, pb, in, paths, t->maxBlocks, ctr, st); });
+                t->Timed(HDCX_KERNEL_SHADE, [&] {
+                    return LaunchShade(t->scene, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, paths, t->maxBlocks, st);
+                });
+                t->Timed(HDCX_KERNEL_SHADOW, [&] { return LaunchShadow(t->scene, pb, paths, t->maxBlocks, shadowCtr, st); });
+            }
+            t->Timed(HDCX_KERNEL_FILM, [&] { return LaunchFilm(pb, c, t->filmBatch.as<double4>(), t->maxBlocks, st); });
+            t->stats.rays[HDCX_KERNEL_FILM] += (p1 - p0);
+        }
+    }
+    Check(LaunchResolve(t->filmMean.as<double4>(), t->filmBatch.as<double4>(), t->filmOut.as<float4>(), pixels,
+                        sampleIndex, samples, t->maxBlocks, st),
+          "resolve");
+    if (out) Check(hipMemcpyAsync(out, t->filmOut.as<void>(), static_cast<std::size_t>(pixels) * 16, hipMemcpyDeviceToHost, st),
+                   "film readback");
+    Check(hipStreamSynchronize(st), "render");
+    t->CollectTimings();
+}
+
+void Query(hdcx_tracer* t, const hdcx_ray* rays, std::uint32_t count, void* out, bool closest)
+{
+    t->Activate();
+    if (count == 0) return;
+    hipStream_t st = t->stream;
+    t->queryRays.Upload(rays, static_cast<std::size_t>(count) * sizeof(hdcx_ray), st);
+    t->queryHits.Allocate(static_cast<std::size_t>(count) * (closest ? sizeof(hdcx_hit) : 1));
+    if (closest)
+        Check(LaunchQueryClosest(t->scene, t->queryRays.as<float4>(), count, t->queryHits.as<float4>(), t->maxBlocks, st), "query");
+    else
+        Check(LaunchQueryAny(t->scene, t->queryRays.as<float4>(), count, t->queryHits.as<unsigned char>(), t->maxBlocks, st), "query");
+    Check(hipMemcpyAsync(out, t->queryHits.as<void>(), static_cast<std::size_t>(count) * (closest ? sizeof(hdcx_hit) : 1),
+                         hipMemcpyDeviceToHost, st),
+          "query readback");
+    Check(hipStreamSynchronize(st), "query");
+}
+
+} // namespace
+} // namespace hdcx
+
+extern "C" {
+
+const char* hdcx_last_error(void) { return hdcx::gLastError.c_str(); }
+
+int hdcx_create(int device_ordinal, hdcx_tracer** out_tracer)
+{
+    return Guard([&] {
+        if (!out_tracer) throw std::invalid_argument("out_tracer is null");
+        *out_tracer = nullptr;
+        int devices = 0;
+        Check(hipGetDeviceCount(&devices), "hipGetDeviceCount");
+        if (device_ordinal < 0 || device_ordinal >= devices) throw Failure(HDCX_ERR_DEVICE, "no such HIP device");
+        auto t = std::make_unique<hdcx_tracer>();
+        t->device = device_ordinal;
+        t->Activate();
+        Check(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking), "hipStreamCreate");
+        int cus = 0;
+        Check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_ordinal), "device attribute");
+        t->maxBlocks = static_cast<unsigned>(std::max(cus, 1)) * 16U;
+        *out_tracer = t.release();
+    });
+}
+
+int hdcx_destroy(hdcx_tracer* tracer)
+{
+    return Guard([&] {
+        if (!tracer) return;
+        (void)hipSetDevice(tracer->device);
+        (void)hipStreamSynchronize(tracer->stream);
+        delete tracer;
+    });
+}
+
+int hdcx_set_scene(hdcx_tracer* tracer, const hdcx_scene_view* scene)
+{
+    return Guard([&] {
+        if (!tracer) throw std::invalid_argument("tracer is null");
+        SetScene(tracer, scene);
+    });
+}
+
+int hdcx_set_shading_mode(hdcx_tracer* tracer, int mode)
+{
+    return Guard([&] {
+        if (!tracer) throw std::invalid_argument("tracer is null");
+        if (mode == HDCX_SHADING_RASTER_PREVIEW)
+            throw std::invalid_argument("raster preview is not a path-tracer shading mode");
+        if (mode != HDCX_SHADING_FUSED && mode != HDCX_SHADING_MODULAR) throw std::invalid_argument("unknown shading mode");
+        tracer->shadingMode = mode;
+    });
+}
+
+int hdcx_get_shading_mode(const hdcx_tracer* tracer, int* out_mode)
+{
+    return Guard([&] {
+        if (!tracer || !out_mode) throw std::invalid_argument("null argument");
+        *out_mode = tracer->shadingMode;
+    });
+}
+
+int hdcx_render(hdcx_tracer* tracer, const hdcx_camera* camera, uint32_t width, uint32_t height, uint32_t sample_index,
+                uint32_t max_bounces, uint32_t sample_count, float* out_rgba)
+{
+    return Guard([&] {
+        if (!tracer || !camera) throw std::invalid_argument("null argument");
+        if (!tracer->geometry || width == 0 || height == 0) throw Failure(HDCX_ERR_NO_GEOMETRY, "no geometry or empty image");
+        if (static_cast<std::uint64_t>(width) * height > 0x7fffffffULL) throw std::invalid_argument("image too large");
+        Render(tracer, *camera, width, height, sample_index, max_bounces, sample_count, out_rgba);
+    });
+}
+
+int hdcx_read_film(hdcx_tracer* tracer, float* out_rgba)
+{
+    return Guard([&] {
+        if (!tracer || !out_rgba) throw std::invalid_argument("null argument");
+        if (!tracer->filmOut.bytes()) throw Failure(HDCX_ERR_NO_GEOMETRY, "nothing rendered yet");
+        tracer->Activate();
+        Check(hipMemcpy(out_rgba, tracer->filmOut.as<void>(), tracer->filmOut.bytes(), hipMemcpyDeviceToHost), "readback");
+    });
+}
+
+int hdcx_has_geometry(const hdcx_tracer* tracer) { return tracer && tracer->geometry ? 1 : 0; }
+
+int hdcx_trace_closest(hdcx_tracer* tracer, const hdcx_ray* rays, uint32_t count, hdcx_hit* out_hits)
+{
+    return Guard([&] {
+        if (!tracer || (count && (!rays || !out_hits))) throw std::invalid_argument("null argument");
+        if (!tracer->geometry) throw Failure(HDCX_ERR_NO_GEOMETRY, "no geometry");
+        Query(tracer, rays, count, out_hits, true);
+    });
+}
+
+int hdcx_trace_occluded(hdcx_tracer* tracer, const hdcx_ray* rays, uint32_t count, uint8_t* out_occluded)
+{
+    return Guard([&] {
+        if (!tracer || (count && (!rays || !out_occluded))) throw std::invalid_argument("null argument");
+        if (!tracer->geometry) throw Failure(HDCX_ERR_NO_GEOMETRY, "no geometry");
+        Query(tracer, rays, count, out_occluded, false);
+    });
+}
+
+int hdcx_scene_create_procedural(const char* name, uint64_t seed, uint32_t param, uint32_t param_dome_width,
+                                 hdcx_scene** out_scene)
+{
+    return Guard([&] {
+        if (!name || !out_scene) throw std::invalid_argument("null argument");
+        auto s = std::make_unique<hdcx_scene>();
+        s->procedural = MakeProceduralScene(name, seed, param, param_dome_width);
+        s->packed = PackScene(s->procedural.snapshot);
+        *out_scene = s.release();
+    });
+}
+
+int hdcx_scene_view_get(const hdcx_scene* scene, hdcx_scene_view* out_view)
+{
+    return Guard([&] {
+        if (!scene || !out_view) throw std::invalid_argument("null argument");
+        *out_view = const_cast<hdcx_scene*>(scene)->packed.View();
+    });
+}
+
+int hdcx_scene_camera_get(const hdcx_scene* scene, uint32_t width, uint32_t height, hdcx_camera* out_camera)
+{
+    return Guard([&] {
+        if (!scene || !out_camera) throw std::invalid_argument("null argument");
+        const PathTracerCamera c = scene->procedural.Camera(width, height);
+        for (int k = 0; k < 3; ++k) {
+            out_camera->origin[k] = c.origin[k];
+            out_camera->lower_left[k] = c.lowerLeft[k];
+            out_camera->horizontal[k] = c.horizontal[k];
+            out_camera->vertical[k] = c.vertical[k];
+        }
+    });
+}
+
+int hdcx_scene_destroy(hdcx_scene* scene)
+{
+    delete scene;
+    return HDCX_OK;
+}
+
+int hdcx_set_instrumentation(hdcx_tracer* tracer, uint32_t flags)
+{
+    return Guard([&] {
+        if (!tracer) throw std::invalid_argument("tracer is null");
+        tracer->instrumentation = flags;
+    });
+}
+
+int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats)
+{
+    return Guard([&] {
+        if (!tracer || !out_stats) throw std::invalid_argument("null argument");
+        tracer->Activate();
+        hdcx_kernel_stats s = tracer->stats;
+        if (tracer->totals.bytes()) {
+            unsigned long long totals[3] = {0, 0, 0};
+            TraversalCounters ctr[2] = {};
+            Check(hipMemcpy(totals, tracer->totals.as<void>(), sizeof(totals), hipMemcpyDeviceToHost), "stats");
+            Check(hipMemcpy(ctr, tracer->traversalCounters.as<void>(), sizeof(ctr), hipMemcpyDeviceToHost), "stats");
+            s.rays[HDCX_KERNEL_EXTEND] = totals[0];
+            s.rays[HDCX_KERNEL_SHADE] = totals[0];
+            s.rays[HDCX_KERNEL_SHADOW] = totals[2];
+            s.nodes_visited = ctr[0].nodes;
+            s.triangles_tested = ctr[0].triangles;
+            s.shadow_nodes_visited = ctr[1].nodes;
+            s.shadow_triangles_tested = ctr[1].triangles;
+            s.extend_rays_counted = (tracer->instrumentation & 2U) ? totals[0] : 0;
+            s.shadow_rays_counted = (tracer->instrumentation & 2U) ? totals[2] : 0;
+        }
+        *out_stats = s;
+    });
+}
+
+int hdcx_reset_kernel_stats(hdcx_tracer* tracer)
+{
+    return Guard([&] {
+        if (!tracer) throw std::invalid_argument("tracer is null");
+        tracer->Activate();
+        const std::uint64_t nodeBytes = tracer->stats.node_bytes, triBytes = tracer->stats.triangle_bytes;
+        const std::uint64_t nodes = tracer->stats.bvh_nodes, tris = tracer->stats.bvh_triangles;
+        tracer->stats = hdcx_kernel_stats{};
+        tracer->stats.node_bytes = nodeBytes;
+        tracer->stats.triangle_bytes = triBytes;
+        tracer->stats.bvh_nodes = nodes;
+        tracer->stats.bvh_triangles = tris;
+        if (tracer->totals.bytes()) {
+            Check(hipMemset(tracer->totals.as<void>(), 0, tracer->totals.bytes()), "reset");
+            Check(hipMemset(tracer->traversalCounters.as<void>(), 0, tracer->traversalCounters.bytes()), "reset");
+        }
+    });
+}
+
+} // extern "C"

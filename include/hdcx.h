@@ -1,0 +1,221 @@
+/*
+ * hdcx — C ABI of the MI355X spectral path-tracing core.
+ *
+ * This is the drop-in boundary that replaces the reference integrator class
+ * hdcodex::VulkanPathTracer (include/hdcodex/gpu/vulkan_path_tracer.h:26-52).
+ * Every entry point returns an int status (HDCX_OK = 0) and never throws;
+ * the message of the last failure on the calling thread is returned by
+ * hdcx_last_error(). The C++ wrapper in hdcx_path_tracer.hpp restores the
+ * reference's exact method surface on top of this ABI.
+ *
+ * Threading: one tracer is driven by one thread at a time, like the
+ * reference (one Hydra render thread). Distinct tracers are independent.
+ */
+#ifndef HDCX_H
+#define HDCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDCX_OK 0
+#define HDCX_ERR_INVALID_ARGUMENT 1
+#define HDCX_ERR_INVALID_SCENE 2
+#define HDCX_ERR_OUT_OF_MEMORY 3
+#define HDCX_ERR_DEVICE 4
+#define HDCX_ERR_NO_GEOMETRY 5
+#define HDCX_ERR_UNSUPPORTED 6
+#define HDCX_ERR_INTERNAL 7
+
+/* Byte sizes of the packed records; identical to the reference's std430
+ * structs GpuMaterial (src/gpu/vulkan_path_tracer.cpp:1907-1933) and
+ * GpuLight (:1935-1952). */
+#define HDCX_MATERIAL_BYTES 368
+#define HDCX_LIGHT_BYTES 224
+#define HDCX_MAX_LIGHTS 64
+#define HDCX_MAX_TEXTURES 256
+#define HDCX_MAX_BOUNCES 12
+#define HDCX_MISSING_TEXTURE 0xffffffffu
+
+/* Texel formats of hdcx_texture_view (reference CreateTexture :2215-2221). */
+#define HDCX_TEXTURE_RGBA8_UNORM 0
+#define HDCX_TEXTURE_RGBA8_SRGB 1
+#define HDCX_TEXTURE_RGBA32F 2
+
+/* Shading modes (include/hdcodex/core/shading_mode.h). */
+#define HDCX_SHADING_FUSED 0
+#define HDCX_SHADING_MODULAR 1
+#define HDCX_SHADING_RASTER_PREVIEW 2
+
+typedef struct hdcx_tracer hdcx_tracer;
+typedef struct hdcx_scene hdcx_scene;
+
+/* Pinhole camera, the reference's PathTracerCamera
+ * (include/hdcodex/gpu/vulkan_path_tracer.h:16-23). Pixel row 0 is the
+ * bottom row (v = 0 at lower_left). */
+typedef struct hdcx_camera {
+    float origin[3];
+    float lower_left[3];
+    float horizontal[3];
+    float vertical[3];
+} hdcx_camera;
+
+typedef struct hdcx_texture_view {
+    uint32_t width;
+    uint32_t height;
+    uint32_t format;     /* HDCX_TEXTURE_* */
+    uint32_t reserved;
+    const void* texels;  /* width*height*4 bytes (RGBA8) or floats (RGBA32F) */
+} hdcx_texture_view;
+
+/* A flattened, packed scene: exactly the buffers the reference uploads in
+ * BuildScene (src/gpu/vulkan_path_tracer.cpp:3004-3030), in its order and
+ * byte layout. All pointers are host memory owned by the caller for the
+ * duration of hdcx_set_scene. */
+typedef struct hdcx_scene_view {
+    uint32_t vertex_count;
+    uint32_t triangle_count;
+    const float* positions;            /* float4 per vertex, w = 1 */
+    const uint32_t* indices;           /* 3 per triangle, world-flattened */
+    const uint32_t* triangle_materials;/* 1 per triangle */
+    const float* texcoords;            /* 6 per triangle (uv per corner) */
+    const float* normals;              /* 12 per triangle (float4 per corner) */
+    const float* emissive;             /* 4 per triangle: cdf, pdf/area, area, 0 */
+    uint32_t material_count;
+    uint32_t light_count;              /* authored lights, <= HDCX_MAX_LIGHTS */
+    const void* materials;             /* material_count * HDCX_MATERIAL_BYTES */
+    const void* lights;                /* max(light_count,1) * HDCX_LIGHT_BYTES */
+    uint32_t texture_count;
+    uint32_t opaque;                   /* 1 iff every material is opaque */
+    const hdcx_texture_view* textures;
+} hdcx_scene_view;
+
+/* One query ray: origin, tmin, direction, tmax (32 bytes). */
+typedef struct hdcx_ray {
+    float origin[3];
+    float tmin;
+    float direction[3];
+    float tmax;
+} hdcx_ray;
+
+/* Closest-hit record (16 bytes). primitive = flattened triangle index, or
+ * 0xffffffff on a miss; instance is always 0 for a flattened scene;
+ * (u, v) are the barycentrics of vertices 1 and 2. */
+typedef struct hdcx_hit {
+    float t;
+    uint32_t primitive;
+    float u;
+    float v;
+} hdcx_hit;
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char* hdcx_last_error(void);
+
+/* Tracer lifetime. Replaces VulkanPathTracer(VulkanContext&, ShaderCache&)
+ * (vulkan_path_tracer.h:28) and ~VulkanPathTracer. */
+int hdcx_create(int device_ordinal, hdcx_tracer** out_tracer);
+int hdcx_destroy(hdcx_tracer* tracer);
+
+/* Replaces VulkanPathTracer::SetScene (vulkan_path_tracer.h:34; body
+ * src/gpu/vulkan_path_tracer.cpp:2604-3128): uploads the packed scene and
+ * builds the BVH on the device. A NULL view, or a view without vertices or
+ * triangles, clears the scene (HasGeometry() becomes false). */
+int hdcx_set_scene(hdcx_tracer* tracer, const hdcx_scene_view* scene);
+
+/* Replaces SetShadingMode / GetShadingMode (vulkan_path_tracer.h:35-36).
+ * HDCX_SHADING_RASTER_PREVIEW is rejected with HDCX_ERR_INVALID_ARGUMENT as
+ * the reference rejects it with std::invalid_argument (:2495-2498). */
+int hdcx_set_shading_mode(hdcx_tracer* tracer, int mode);
+int hdcx_get_shading_mode(const hdcx_tracer* tracer, int* out_mode);
+
+/* Replaces VulkanPathTracer::Render (vulkan_path_tracer.h:39-45; body
+ * :3230-3306). Traces sample_count progressive samples per pixel starting at
+ * sample number sample_index and writes the running mean of samples
+ * [0, sample_index + sample_count) as linear RGBA32F, alpha 1, row 0 at the
+ * bottom, into out_rgba (width*height*4 floats, caller-owned). out_rgba may
+ * be NULL: the film then stays on the device (hdcx_read_film reads it).
+ * Returns HDCX_ERR_NO_GEOMETRY (writing nothing) when there is no geometry
+ * or the size is zero, where the reference returns an empty vector. */
+int hdcx_render(hdcx_tracer* tracer, const hdcx_camera* camera,
+                uint32_t width, uint32_t height, uint32_t sample_index,
+                uint32_t max_bounces, uint32_t sample_count, float* out_rgba);
+
+/* Copies the current device film (RGBA32F, width*height*4) to out_rgba. */
+int hdcx_read_film(hdcx_tracer* tracer, float* out_rgba);
+
+/* Replaces VulkanPathTracer::HasGeometry (vulkan_path_tracer.h:47).
+ * Returns 1 or 0; 0 for a NULL tracer. */
+int hdcx_has_geometry(const hdcx_tracer* tracer);
+
+/* Closest-hit queries against the current scene: the reference's committed
+ * ray query (rayQueryInitializeEXT ... rayQueryProceedEXT,
+ * src/gpu/vulkan_path_tracer.cpp:1228-1268) exposed for batches of rays in
+ * host memory. Opaque semantics (no any-hit confirmation). */
+int hdcx_trace_closest(hdcx_tracer* tracer, const hdcx_ray* rays,
+                       uint32_t count, hdcx_hit* out_hits);
+
+/* Any-hit (occlusion) queries with terminate-on-first-hit semantics
+ * (shadowVisibility / meshLightVisible, :855-912) in an opaque scene:
+ * out_occluded[i] = 1 when any triangle lies in [tmin, tmax]. */
+int hdcx_trace_occluded(hdcx_tracer* tracer, const hdcx_ray* rays,
+                        uint32_t count, uint8_t* out_occluded);
+
+/* ---- Procedural scenes (no counterpart in the reference: its scenes come
+ * from Hydra). They produce the packed view of a SceneSnapshot through the
+ * same packer the C++ wrapper uses. ---- */
+
+/* name: "cornell" (34 triangles, one rect light, Lambert walls),
+ * "heightfield" (param = grid vertices per side, e.g. 1024 -> 2,093,058
+ * triangles; GGX dielectric over Burley; HDR lat-long dome of
+ * param_dome_width x param_dome_width/2 texels), "triangle" (the
+ * reference's one-triangle test scene). */
+int hdcx_scene_create_procedural(const char* name, uint64_t seed,
+                                 uint32_t param, uint32_t param_dome_width,
+                                 hdcx_scene** out_scene);
+int hdcx_scene_view_get(const hdcx_scene* scene, hdcx_scene_view* out_view);
+int hdcx_scene_camera_get(const hdcx_scene* scene, uint32_t width,
+                          uint32_t height, hdcx_camera* out_camera);
+int hdcx_scene_destroy(hdcx_scene* scene);
+
+/* ---- Measurement (no counterpart in the reference, which has no GPU
+ * timing: docs/architecture.md:255-257). ---- */
+
+#define HDCX_KERNEL_RAYGEN 0
+#define HDCX_KERNEL_EXTEND 1
+#define HDCX_KERNEL_SHADE 2
+#define HDCX_KERNEL_SHADOW 3
+#define HDCX_KERNEL_FILM 4
+#define HDCX_KERNEL_COUNT 5
+
+typedef struct hdcx_kernel_stats {
+    uint64_t launches[HDCX_KERNEL_COUNT];
+    double milliseconds[HDCX_KERNEL_COUNT]; /* HIP-event time summed over launches */
+    uint64_t rays[HDCX_KERNEL_COUNT];       /* work items processed */
+    uint64_t nodes_visited;                 /* counted only when counting is on */
+    uint64_t triangles_tested;
+    uint64_t extend_rays_counted;
+    uint64_t shadow_rays_counted;
+    uint64_t shadow_nodes_visited;
+    uint64_t shadow_triangles_tested;
+    uint64_t node_bytes;                    /* bytes per BVH node record */
+    uint64_t triangle_bytes;                /* bytes per leaf triangle record */
+    uint64_t bvh_nodes;
+    uint64_t bvh_triangles;
+} hdcx_kernel_stats;
+
+/* flags bit 0: time every kernel launch with HIP events on the launch
+ * stream; bit 1: run the traversal kernels compiled with node/triangle
+ * counters (a separate instantiation; the counting build is slower and is
+ * used only to measure algorithmic bytes). */
+int hdcx_set_instrumentation(hdcx_tracer* tracer, uint32_t flags);
+int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats);
+int hdcx_reset_kernel_stats(hdcx_tracer* tracer);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDCX_H */

@@ -1,0 +1,238 @@
+"""ORACLE — CPU restatement of the reference integrator (test infrastructure).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this package, and only as the checker (or, for the baseline, as
+the timed CPU port). The product library never links or calls it.
+
+``liboracle.so`` is built from ``oracle/oracle.cpp`` by
+``python -m hdcodex_amd.build --oracle``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_float, c_int, c_uint32, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} missing: run `python -m hdcodex_amd.build --oracle`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    lib.oracle_scene_create.argtypes = [c_void_p, POINTER(c_void_p)]
+    lib.oracle_scene_destroy.argtypes = [c_void_p]
+    lib.oracle_render.argtypes = [c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, c_int,
+                                  c_void_p, c_uint32, c_uint32, c_int, c_void_p]
+    lib.oracle_trace_closest.argtypes = [c_void_p, c_void_p, c_uint32, c_int, c_void_p, c_void_p]
+    lib.oracle_trace_occluded.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
+    lib.oracle_hash.argtypes = [c_uint32]
+    lib.oracle_hash.restype = c_uint32
+    lib.oracle_random_float.argtypes = [POINTER(c_uint32)]
+    lib.oracle_random_float.restype = c_float
+    for name in ("oracle_cie_xyz", "oracle_sensor"):
+        getattr(lib, name).argtypes = [c_double, c_void_p]
+    lib.oracle_reconstruct_rgb.argtypes = [c_void_p, c_double]
+    lib.oracle_reconstruct_rgb.restype = c_double
+    lib.oracle_illuminant.argtypes = [c_void_p, c_double]
+    lib.oracle_illuminant.restype = c_double
+    lib.oracle_dispersed_ior.argtypes = [c_double, c_double, c_double, c_double]
+    lib.oracle_dispersed_ior.restype = c_double
+    lib.oracle_eval_diffuse.argtypes = [c_void_p, c_void_p, c_double, c_double, c_uint32]
+    lib.oracle_eval_diffuse.restype = c_double
+    lib.oracle_ggx_pdf.argtypes = [c_void_p, c_void_p, c_double, c_double]
+    lib.oracle_ggx_pdf.restype = c_double
+    lib.oracle_ggx_eval.argtypes = [c_void_p, c_void_p, c_double, c_double, c_double]
+    lib.oracle_ggx_eval.restype = c_double
+    lib.oracle_ggx_sample.argtypes = [c_void_p, c_double, c_double, c_double, POINTER(c_uint32), c_void_p]
+    lib.oracle_ggx_sample.restype = c_double
+    lib.oracle_sheen_albedo.argtypes = [c_double, c_double, c_int]
+    lib.oracle_sheen_albedo.restype = c_double
+    lib.oracle_sheen_response.argtypes = [c_void_p, c_void_p, c_double, c_int]
+    lib.oracle_sheen_response.restype = c_double
+    lib.oracle_eon_albedo.argtypes = [c_double, c_double]
+    lib.oracle_eon_albedo.restype = c_double
+    lib.oracle_sample_hg.argtypes = [c_void_p, c_double, POINTER(c_uint32), c_void_p]
+    lib.oracle_cosine_hemisphere.argtypes = [c_void_p, POINTER(c_uint32), c_void_p]
+    lib.oracle_dome_uv.argtypes = [c_void_p, c_uint32, c_void_p]
+    lib.oracle_lat_long_direction.argtypes = [c_double, c_double, c_void_p]
+    lib.oracle_sample_dome.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), c_void_p, c_void_p]
+    lib.oracle_sample_dome.restype = c_double
+    _lib = lib
+    return lib
+
+
+def _vec(v) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+
+
+class OracleScene:
+    """The oracle's copy of a packed scene (takes an hdcx SceneView)."""
+
+    def __init__(self, view):
+        self._h = c_void_p()
+        if library().oracle_scene_create(ctypes.byref(view), ctypes.byref(self._h)) != 0:
+            raise RuntimeError(library().oracle_last_error().decode())
+
+    def render(self, camera, width: int, height: int, sample_index: int, max_bounces: int, sample_count: int,
+               precision: int = 64, rows=None, col_step: int = 1, threads: int = 8) -> np.ndarray:
+        """Per-pixel SUM over the batch's samples of the sample RGB, for the
+        given rows (default: all) and every col_step-th column; shape
+        (len(rows), ceil(width/col_step), 3), float64."""
+        rows = np.arange(height, dtype=np.uint32) if rows is None else np.ascontiguousarray(rows, dtype=np.uint32)
+        cols = (width + col_step - 1) // col_step
+        out = np.zeros((len(rows), cols, 3), dtype=np.float64)
+        code = library().oracle_render(self._h, ctypes.byref(camera), width, height, sample_index, max_bounces,
+                                       sample_count, precision, rows.ctypes.data, len(rows), col_step, threads,
+                                       out.ctypes.data)
+        if code != 0:
+            raise RuntimeError(library().oracle_last_error().decode())
+        return out
+
+    def trace_closest(self, rays: np.ndarray, brute_force: bool = False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        hits = np.empty((len(rays), 4), dtype=np.uint32)
+        t64 = np.empty(len(rays), dtype=np.float64)
+        library().oracle_trace_closest(self._h, rays.ctypes.data, len(rays), int(brute_force), hits.ctypes.data,
+                                       t64.ctypes.data)
+        out = np.empty(len(rays), dtype=[("t", np.float32), ("primitive", np.uint32), ("u", np.float32),
+                                         ("v", np.float32)])
+        out["t"] = hits[:, 0].view(np.float32)
+        out["primitive"] = hits[:, 1]
+        out["u"] = hits[:, 2].view(np.float32)
+        out["v"] = hits[:, 3].view(np.float32)
+        return out, t64
+
+    def trace_occluded(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        out = np.empty(len(rays), dtype=np.uint8)
+        library().oracle_trace_occluded(self._h, rays.ctypes.data, len(rays), out.ctypes.data)
+        return out
+
+    def sample_dome(self, light: int, state: int):
+        st = c_uint32(state)
+        d = np.zeros(3)
+        pdf_eval = np.zeros(1)
+        pdf = library().oracle_sample_dome(self._h, light, ctypes.byref(st), d.ctypes.data, pdf_eval.ctypes.data)
+        return d, pdf, float(pdf_eval[0]), st.value
+
+    def close(self):
+        if self._h:
+            library().oracle_scene_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- pure functions (FP64 unless noted) ----
+def hash_state(v: int) -> int:
+    return int(library().oracle_hash(v & 0xFFFFFFFF))
+
+
+def random_float(state: int):
+    st = c_uint32(state)
+    f = library().oracle_random_float(ctypes.byref(st))
+    return float(f), st.value
+
+
+def cie_xyz(w: float) -> np.ndarray:
+    out = np.zeros(3)
+    library().oracle_cie_xyz(w, out.ctypes.data)
+    return out
+
+
+def sensor(w: float) -> np.ndarray:
+    out = np.zeros(3)
+    library().oracle_sensor(w, out.ctypes.data)
+    return out
+
+
+def reconstruct_rgb(rgb, w: float) -> float:
+    v = _vec(rgb)
+    return float(library().oracle_reconstruct_rgb(v.ctypes.data, w))
+
+
+def illuminant(rgb, w: float) -> float:
+    v = _vec(rgb)
+    return float(library().oracle_illuminant(v.ctypes.data, w))
+
+
+def dispersed_ior(ior: float, scale: float, abbe: float, w: float) -> float:
+    return float(library().oracle_dispersed_ior(ior, scale, abbe, w))
+
+
+def eval_diffuse(v, l, color: float, rough: float, model: int) -> float:
+    a, b = _vec(v), _vec(l)
+    return float(library().oracle_eval_diffuse(a.ctypes.data, b.ctypes.data, color, rough, model))
+
+
+def ggx_pdf(v, l, rx: float, ry: float) -> float:
+    a, b = _vec(v), _vec(l)
+    return float(library().oracle_ggx_pdf(a.ctypes.data, b.ctypes.data, rx, ry))
+
+
+def ggx_eval(v, l, rx: float, ry: float, f0: float) -> float:
+    a, b = _vec(v), _vec(l)
+    return float(library().oracle_ggx_eval(a.ctypes.data, b.ctypes.data, rx, ry, f0))
+
+
+def ggx_sample(inc, rx: float, ry: float, f0: float, state: int):
+    a = _vec(inc)
+    st = c_uint32(state)
+    out = np.zeros(3)
+    w = library().oracle_ggx_sample(a.ctypes.data, rx, ry, f0, ctypes.byref(st), out.ctypes.data)
+    return out, float(w), st.value
+
+
+def sheen_albedo(nv: float, rough: float, zeltner: bool) -> float:
+    return float(library().oracle_sheen_albedo(nv, rough, int(zeltner)))
+
+
+def sheen_response(v, l, rough: float, zeltner: bool) -> float:
+    a, b = _vec(v), _vec(l)
+    return float(library().oracle_sheen_response(a.ctypes.data, b.ctypes.data, rough, int(zeltner)))
+
+
+def eon_albedo(nd: float, rough: float) -> float:
+    return float(library().oracle_eon_albedo(nd, rough))
+
+
+def sample_hg(fwd, g: float, state: int):
+    a = _vec(fwd)
+    st = c_uint32(state)
+    out = np.zeros(3)
+    library().oracle_sample_hg(a.ctypes.data, g, ctypes.byref(st), out.ctypes.data)
+    return out, st.value
+
+
+def cosine_hemisphere(n, state: int):
+    a = _vec(n)
+    st = c_uint32(state)
+    out = np.zeros(3)
+    library().oracle_cosine_hemisphere(a.ctypes.data, ctypes.byref(st), out.ctypes.data)
+    return out, st.value
+
+
+def dome_uv(d, fmt: int):
+    a = _vec(d)
+    uv = np.zeros(2)
+    library().oracle_dome_uv(a.ctypes.data, fmt, uv.ctypes.data)
+    return uv
+
+
+def lat_long_direction(u: float, v: float) -> np.ndarray:
+    out = np.zeros(3)
+    library().oracle_lat_long_direction(u, v, out.ctypes.data)
+    return out
