@@ -38,7 +38,7 @@ EXPORTED_SYMBOLS = (
     "hdcx_last_error", "hdcx_create", "hdcx_destroy", "hdcx_set_scene", "hdcx_set_shading_mode",
     "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_has_geometry", "hdcx_trace_closest",
     "hdcx_trace_occluded", "hdcx_scene_create_procedural", "hdcx_scene_view_get", "hdcx_scene_camera_get",
-    "hdcx_scene_destroy", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
+    "hdcx_scene_destroy", "hdcx_bvh_build_host", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
 )
 
 
@@ -93,6 +93,10 @@ class KernelStats(ctypes.Structure):
         return out
 
 
+class BvhInfo(ctypes.Structure):
+    _fields_ = [("node_count", c_uint32), ("slot_count", c_uint32), ("max_stack", c_uint32), ("depth", c_uint32)]
+
+
 class HdcxError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"hdcx error {code}: {message}")
@@ -126,6 +130,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_scene_view_get.argtypes = [c_void_p, POINTER(SceneView)]
     lib.hdcx_scene_camera_get.argtypes = [c_void_p, c_uint32, c_uint32, POINTER(Camera)]
     lib.hdcx_scene_destroy.argtypes = [c_void_p]
+    lib.hdcx_bvh_build_host.argtypes = [POINTER(SceneView), POINTER(BvhInfo), c_void_p, c_void_p]
     lib.hdcx_set_instrumentation.argtypes = [c_void_p, c_uint32]
     lib.hdcx_get_kernel_stats.argtypes = [c_void_p, POINTER(KernelStats)]
     lib.hdcx_reset_kernel_stats.argtypes = [c_void_p]
@@ -190,6 +195,23 @@ class ProceduralScene:
             self.close()
         except Exception:
             pass
+
+
+BVH_NODE_DTYPE = np.dtype([("lox", np.float32, 4), ("hix", np.float32, 4), ("loy", np.float32, 4),
+                           ("hiy", np.float32, 4), ("loz", np.float32, 4), ("hiz", np.float32, 4),
+                           ("child", np.uint32, 4), ("pad", np.uint32, 4)])
+
+
+def build_bvh_host(scene: "ProceduralScene"):
+    """Host-built BVH4 of a scene: (info dict, nodes, slots[(n, 3, 4) float32])."""
+    lib = library()
+    info = BvhInfo()
+    _check(lib.hdcx_bvh_build_host(ctypes.byref(scene.view), ctypes.byref(info), None, None))
+    nodes = np.zeros(info.node_count, dtype=BVH_NODE_DTYPE)
+    slots = np.zeros((info.slot_count, 3, 4), dtype=np.float32)
+    _check(lib.hdcx_bvh_build_host(ctypes.byref(scene.view), ctypes.byref(info), nodes.ctypes.data, slots.ctypes.data))
+    return ({"node_count": info.node_count, "slot_count": info.slot_count, "max_stack": info.max_stack,
+             "depth": info.depth}, nodes, slots)
 
 
 def rays_array(origins: np.ndarray, directions: np.ndarray, tmin: float = 0.001, tmax: float = 1e4) -> np.ndarray:

@@ -48,4 +48,11 @@ Bvh4 BuildBvh4Host(const float* positions, const std::uint32_t* indices, std::ui
 // Worst-case stack occupancy and depth of a BVH4 (root at node 0).
 void MeasureBvh4(Bvh4& bvh);
 
+// Structural check of a BVH4 against the triangle count: every inner
+// reference in range, every leaf inside the slot array with 1..8 triangles,
+// every slot's primitive id < triangleCount, every triangle referenced
+// exactly once, empty slots only with inverted boxes. Throws
+// std::runtime_error with the first violation.
+void ValidateBvh4(const Bvh4& bvh, std::uint32_t triangleCount);
+
 } // namespace hdcx

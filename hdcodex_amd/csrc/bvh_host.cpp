@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
+#include <string>
 
 namespace hdcx {
 namespace {
@@ -273,6 +275,49 @@ void MeasureBvh4(Bvh4& bvh)
         for (std::uint32_t c : n.child)
             if (c != kEmptyChild && (c & kLeafFlag) == 0) work.push_back({c, stack, it.depth + 1});
     }
+}
+
+void ValidateBvh4(const Bvh4& bvh, std::uint32_t triangleCount)
+{
+    auto fail = [](const std::string& m) { throw std::runtime_error("invalid BVH: " + m); };
+    if (triangleCount == 0) {
+        if (!bvh.nodes.empty()) fail("nodes without triangles");
+        return;
+    }
+    if (bvh.nodes.empty()) fail("no root");
+    const std::size_t slots = bvh.triangles.size() / 12;
+    if (bvh.triangles.size() % 12 != 0) fail("slot array not a multiple of 3 float4");
+    std::vector<std::uint8_t> seen(triangleCount, 0);
+    std::vector<std::uint8_t> visited(bvh.nodes.size(), 0);
+    std::vector<std::uint32_t> work{0};
+    visited[0] = 1;
+    while (!work.empty()) {
+        const Bvh4Node& n = bvh.nodes[work.back()];
+        work.pop_back();
+        for (int i = 0; i < 4; ++i) {
+            const std::uint32_t c = n.child[i];
+            if (c == kEmptyChild) {
+                if (!(n.lox[i] > n.hix[i])) fail("empty slot with a non-inverted box");
+                continue;
+            }
+            if (c & kLeafFlag) {
+                const std::uint32_t first = (c >> 3) & 0x0fffffffU, count = (c & 7U) + 1U;
+                if (static_cast<std::size_t>(first) + count > slots) fail("leaf beyond the slot array");
+                for (std::uint32_t j = 0; j < count; ++j) {
+                    std::uint32_t prim;
+                    std::memcpy(&prim, &bvh.triangles[(static_cast<std::size_t>(first) + j) * 12 + 3], 4);
+                    if (prim >= triangleCount) fail("slot primitive id out of range");
+                    if (seen[prim]++) fail("triangle referenced twice");
+                }
+            } else {
+                if (c >= bvh.nodes.size()) fail("inner reference out of range");
+                if (visited[c]++) fail("node reached twice");
+                work.push_back(c);
+            }
+        }
+    }
+    for (std::uint32_t t = 0; t < triangleCount; ++t)
+        if (!seen[t]) fail("triangle " + std::to_string(t) + " not in any leaf");
 }
 
 } // namespace hdcx

@@ -118,7 +118,9 @@ __device__ __forceinline__ void testNode(const Bvh4Node* __restrict__ node, cons
         const float tz0 = (lz[i] - r.oz) * r.iz, tz1 = (hz[i] - r.oz) * r.iz;
         const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
         const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-        const bool hit = tnear <= tfar * 1.00000048f;
+        // Empty slots must be rejected explicitly: min/max of the two plane
+        // distances would turn an inverted box into an infinite one.
+        const bool hit = c[i] != kEmptyChild && tnear <= tfar * 1.00000048f;
         tn[i] = hit ? tnear : INFINITY;
         ref[i] = c[i];
     }
@@ -129,10 +131,13 @@ struct TraversalStack {
     std::uint32_t stride;
     std::uint32_t spill[kSpillStack];
     int sp;
+    // The host rejects BVHs whose worst-case occupancy exceeds kMaxStack;
+    // the bound check only keeps a corrupt tree from writing out of range.
     __device__ __forceinline__ void push(std::uint32_t v)
     {
         if (sp < kLdsStack) lds[sp * stride] = v;
-        else spill[sp - kLdsStack] = v;
+        else if (sp < kMaxStack) spill[sp - kLdsStack] = v;
+        else return;
         ++sp;
     }
     __device__ __forceinline__ std::uint32_t pop()

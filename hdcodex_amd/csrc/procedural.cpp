@@ -335,8 +335,8 @@ ProceduralScene MakeHeightfield(std::uint64_t seed, std::uint32_t n, std::uint32
     dome.intensity = 1.0F;
     s.lights.push_back(dome);
     SortSnapshot(s);
-    out.eye = {0.0, 32.0, 68.0};
-    out.target = {0.0, 0.0, 5.0};
+    out.eye = {0.0, 25.0, 75.0};
+    out.target = {0.0, 8.0, 0.0};
     out.up = {0.0, 1.0, 0.0};
     out.verticalFovDegrees = 42.0;
     return out;

@@ -3,6 +3,7 @@
 // Errors never cross the ABI as exceptions: every entry point maps them to
 // a status code and a thread-local message.
 #include "../../include/hdcx.h"
+#include "dev_traverse.h"
 #include "kernel_api.h"
 #include "packer.hpp"
 #include "procedural.hpp"
@@ -310,7 +311,8 @@ void SetScene(hdcx_tracer* t, const hdcx_scene_view* view)
     hipStream_t st = t->stream;
 
     Bvh4 bvh = BuildBvh4Host(v.positions, v.indices, v.triangle_count);
-    if (bvh.maxStack + 1 > static_cast<std::uint32_t>(64))
+    ValidateBvh4(bvh, v.triangle_count);
+    if (bvh.maxStack > static_cast<std::uint32_t>(kMaxStack))
         throw Failure(HDCX_ERR_INVALID_SCENE, "BVH deeper than the traversal stack");
     t->nodes.Upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), st);
     t->tris.Upload(bvh.triangles.data(), bvh.triangles.size() * sizeof(float), st);
@@ -630,6 +632,22 @@ int hdcx_scene_destroy(hdcx_scene* scene)
 {
     delete scene;
     return HDCX_OK;
+}
+
+int hdcx_bvh_build_host(const hdcx_scene_view* scene, hdcx_bvh_info* out_info, void* out_nodes, float* out_slots)
+{
+    return Guard([&] {
+        if (!scene || !out_info) throw std::invalid_argument("null argument");
+        if (scene->triangle_count && (!scene->positions || !scene->indices)) throw std::invalid_argument("no geometry");
+        Bvh4 bvh = BuildBvh4Host(scene->positions, scene->indices, scene->triangle_count);
+        ValidateBvh4(bvh, scene->triangle_count);
+        out_info->node_count = static_cast<std::uint32_t>(bvh.nodes.size());
+        out_info->slot_count = static_cast<std::uint32_t>(bvh.triangles.size() / 12);
+        out_info->max_stack = bvh.maxStack;
+        out_info->depth = bvh.depth;
+        if (out_nodes) std::memcpy(out_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node));
+        if (out_slots) std::memcpy(out_slots, bvh.triangles.data(), bvh.triangles.size() * sizeof(float));
+    });
 }
 
 int hdcx_set_instrumentation(hdcx_tracer* tracer, uint32_t flags)

@@ -180,6 +180,24 @@ int hdcx_scene_camera_get(const hdcx_scene* scene, uint32_t width,
                           uint32_t height, hdcx_camera* out_camera);
 int hdcx_scene_destroy(hdcx_scene* scene);
 
+/* ---- BVH (no counterpart in the reference: its acceleration structure is
+ * built inside the Vulkan driver, :3032-3125). ---- */
+
+/* Layout summary of a four-wide BVH: 128-byte nodes, 48-byte triangle
+ * slots (three float4 vertices, primitive id in the first w). */
+typedef struct hdcx_bvh_info {
+    uint32_t node_count;
+    uint32_t slot_count;
+    uint32_t max_stack;  /* worst-case traversal stack entries */
+    uint32_t depth;
+} hdcx_bvh_info;
+
+/* Builds the host (binned-SAH) BVH of a scene view and validates it. Call
+ * with out_nodes = out_slots = NULL to get the sizes, then again with
+ * buffers of node_count*128 bytes and slot_count*12 floats. */
+int hdcx_bvh_build_host(const hdcx_scene_view* scene, hdcx_bvh_info* out_info,
+                        void* out_nodes, float* out_slots);
+
 /* ---- Measurement (no counterpart in the reference, which has no GPU
  * timing: docs/architecture.md:255-257). ---- */
 
