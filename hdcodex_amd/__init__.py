@@ -35,7 +35,7 @@ KERNELS = ("raygen", "extend", "shade", "shadow", "film")
 
 # Every symbol include/hdcx.h declares.
 EXPORTED_SYMBOLS = (
-    "hdcx_last_error", "hdcx_create", "hdcx_destroy", "hdcx_set_scene", "hdcx_set_shading_mode",
+    "hdcx_last_error", "hdcx_create", "hdcx_destroy", "hdcx_set_scene", "hdcx_validate_scene", "hdcx_set_shading_mode",
     "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_has_geometry", "hdcx_trace_closest",
     "hdcx_trace_occluded", "hdcx_scene_create_procedural", "hdcx_scene_view_get", "hdcx_scene_camera_get",
     "hdcx_scene_destroy", "hdcx_bvh_build_host", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
@@ -119,6 +119,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_create.argtypes = [c_int, POINTER(c_void_p)]
     lib.hdcx_destroy.argtypes = [c_void_p]
     lib.hdcx_set_scene.argtypes = [c_void_p, POINTER(SceneView)]
+    lib.hdcx_validate_scene.argtypes = [POINTER(SceneView)]
     lib.hdcx_set_shading_mode.argtypes = [c_void_p, c_int]
     lib.hdcx_get_shading_mode.argtypes = [c_void_p, POINTER(c_int)]
     lib.hdcx_render.argtypes = [c_void_p, POINTER(Camera), c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, c_void_p]
@@ -212,6 +213,11 @@ def build_bvh_host(scene: "ProceduralScene"):
     _check(lib.hdcx_bvh_build_host(ctypes.byref(scene.view), ctypes.byref(info), nodes.ctypes.data, slots.ctypes.data))
     return ({"node_count": info.node_count, "slot_count": info.slot_count, "max_stack": info.max_stack,
              "depth": info.depth}, nodes, slots)
+
+
+def validate_scene(view: SceneView) -> None:
+    """Runs the scene checks of hdcx_set_scene without a device."""
+    _check(library().hdcx_validate_scene(ctypes.byref(view)))
 
 
 def rays_array(origins: np.ndarray, directions: np.ndarray, tmin: float = 0.001, tmax: float = 1e4) -> np.ndarray:

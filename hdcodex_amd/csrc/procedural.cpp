@@ -431,6 +431,39 @@ ProceduralScene MakeTriangle()
     return out;
 }
 
+// ---- Furnace: a 2000 x 2000 Lambert floor (flat albedo 0.5, no specular)
+// under a uniform untextured white dome, camera 1 unit above looking down.
+// Every camera ray sees the floor; every reflected ray escapes to the dome,
+// so the expected pixel value is 0.5 x (dome radiance integrated against
+// the sensor) in closed form. ----
+ProceduralScene MakeFurnace()
+{
+    ProceduralScene out;
+    SceneSnapshot& s = out.snapshot;
+    s.revision = 1;
+    SceneMesh floor;
+    floor.id = "/furnace/floor";
+    const double q[4][3] = {{-1000, 0, 1000}, {1000, 0, 1000}, {1000, 0, -1000}, {-1000, 0, -1000}};
+    AddQuad(floor, q);
+    floor.materialId = "/furnace/grey";
+    s.meshes.push_back(floor);
+    SceneMaterial grey;
+    grey.id = "/furnace/grey";
+    grey.baseColor = {0.5F, 0.5F, 0.5F};
+    grey.specularWeight = 0.0F;
+    s.materials.push_back(grey);
+    SceneLight dome;
+    dome.id = "/furnace/dome";
+    dome.type = SceneLightType::Dome;
+    dome.intensity = 1.0F;
+    s.lights.push_back(dome);
+    out.eye = {0.0, 1.0, 0.0};
+    out.target = {0.0, 0.0, 0.0};
+    out.up = {0.0, 0.0, -1.0};
+    out.verticalFovDegrees = 60.0;
+    return out;
+}
+
 } // namespace
 
 PathTracerCamera ProceduralScene::Camera(std::uint32_t width, std::uint32_t height) const
@@ -464,6 +497,7 @@ ProceduralScene MakeProceduralScene(const std::string& name, std::uint64_t seed,
                                                       domeWidth == 0 ? 2048 : domeWidth);
     if (name == "triangle") return MakeTriangle();
     if (name == "materials") return MakeMaterialShowcase(seed);
+    if (name == "furnace") return MakeFurnace();
     throw std::invalid_argument("unknown procedural scene '" + name + "'");
 }
 

@@ -634,6 +634,15 @@ int hdcx_scene_destroy(hdcx_scene* scene)
     return HDCX_OK;
 }
 
+int hdcx_validate_scene(const hdcx_scene_view* scene)
+{
+    return Guard([&] {
+        if (!scene) throw std::invalid_argument("scene is null");
+        if (scene->vertex_count == 0 || scene->triangle_count == 0) return;
+        ValidateScene(*scene);
+    });
+}
+
 int hdcx_bvh_build_host(const hdcx_scene_view* scene, hdcx_bvh_info* out_info, void* out_nodes, float* out_slots)
 {
     return Guard([&] {

@@ -125,6 +125,13 @@ int hdcx_destroy(hdcx_tracer* tracer);
  * triangles, clears the scene (HasGeometry() becomes false). */
 int hdcx_set_scene(hdcx_tracer* tracer, const hdcx_scene_view* scene);
 
+/* The checks hdcx_set_scene runs before touching the device: indices and
+ * material ids in range, finite positions, texture references (plain and
+ * UDIM) in range, <= 64 lights, dome importance tables RGBA32F. Returns
+ * HDCX_ERR_INVALID_SCENE with the first violation. Needs no device; the
+ * reference has no such check (the driver consumes the buffers as given). */
+int hdcx_validate_scene(const hdcx_scene_view* scene);
+
 /* Replaces SetShadingMode / GetShadingMode (vulkan_path_tracer.h:35-36).
  * HDCX_SHADING_RASTER_PREVIEW is rejected with HDCX_ERR_INVALID_ARGUMENT as
  * the reference rejects it with std::invalid_argument (:2495-2498). */
