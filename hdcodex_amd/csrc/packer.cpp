@@ -476,3 +476,17 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
 }
 
 } // namespace hdcx
+
+// ---- C++ wrapper support (include/hdcx_path_tracer.hpp) ----
+#include "../../include/hdcx_path_tracer.hpp"
+
+namespace hdcx {
+
+PackedSceneHandle::PackedSceneHandle(const SceneSnapshot& snapshot) : packed_(new PackedScene(PackScene(snapshot)))
+{
+    if (packed_->HasGeometry()) view_ = packed_->View();
+}
+
+PackedSceneHandle::~PackedSceneHandle() { delete packed_; }
+
+} // namespace hdcx
