@@ -135,6 +135,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_set_instrumentation.argtypes = [c_void_p, c_uint32]
     lib.hdcx_get_kernel_stats.argtypes = [c_void_p, POINTER(KernelStats)]
     lib.hdcx_reset_kernel_stats.argtypes = [c_void_p]
+    _bind_cache(lib)
     _lib = lib
     return lib
 
@@ -314,3 +315,90 @@ class PathTracer:
 
 
 HIT_DTYPE = np.dtype([("t", np.float32), ("primitive", np.uint32), ("u", np.float32), ("v", np.float32)])
+
+
+# ---------------------------------------------------------------- cache ----
+class RadianceCacheConfig(ctypes.Structure):
+    _fields_ = [("enabled", c_uint32), ("train_samples", c_uint32), ("terminate_bounce", c_uint32),
+                ("train_steps", c_uint32), ("learning_rate", c_float), ("beta1", c_float), ("beta2", c_float),
+                ("epsilon", c_float), ("seed", c_uint64)]
+
+
+class RcStats(ctypes.Structure):
+    _fields_ = [("frames", c_uint64), ("queries", c_uint64), ("train_samples", c_uint64), ("adam_steps", c_uint64),
+                ("last_loss", c_float), ("reserved", c_float), ("infer_ms", c_double), ("train_ms", c_double)]
+
+
+RC_PARAMS = 24768
+RC_QUERY_DTYPE = np.dtype([("position", np.float32, 3), ("direction", np.float32, 3), ("lambda0", np.float32),
+                           ("roughness", np.float32), ("albedo", np.float32), ("lane", np.uint32),
+                           ("path", np.uint32), ("target", np.float32)])
+
+
+def _bind_cache(lib: ctypes.CDLL) -> None:
+    lib.hdcx_set_radiance_cache.argtypes = [c_void_p, POINTER(RadianceCacheConfig)]
+    lib.hdcx_rc_get_params.argtypes = [c_void_p, c_void_p]
+    lib.hdcx_rc_set_params.argtypes = [c_void_p, c_void_p]
+    lib.hdcx_rc_infer.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
+    lib.hdcx_rc_gradient.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p, POINTER(c_float)]
+    lib.hdcx_rc_train.argtypes = [c_void_p, c_void_p, c_uint32, c_uint32, POINTER(c_float)]
+    lib.hdcx_rc_get_stats.argtypes = [c_void_p, POINTER(RcStats)]
+
+
+def _tracer_set_radiance_cache(self, enabled: bool = True, train_samples: int = 65536, terminate_bounce: int = 2,
+                               train_steps: int = 4, learning_rate: float = 1e-2, seed: int = 0) -> None:
+    c = RadianceCacheConfig(int(enabled), train_samples, terminate_bounce, train_steps, learning_rate, 0.9, 0.99,
+                            1e-8, seed)
+    _check(library().hdcx_set_radiance_cache(self._h, ctypes.byref(c)))
+
+
+def _tracer_rc_params(self) -> np.ndarray:
+    out = np.empty(RC_PARAMS, dtype=np.float32)
+    _check(library().hdcx_rc_get_params(self._h, out.ctypes.data))
+    return out
+
+
+def _tracer_rc_set_params(self, params: np.ndarray) -> None:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    assert p.size == RC_PARAMS
+    _check(library().hdcx_rc_set_params(self._h, p.ctypes.data))
+
+
+def _tracer_rc_infer(self, queries: np.ndarray) -> np.ndarray:
+    q = np.ascontiguousarray(queries, dtype=RC_QUERY_DTYPE)
+    out = np.empty((len(q), 3), dtype=np.float32)
+    _check(library().hdcx_rc_infer(self._h, q.ctypes.data, len(q), out.ctypes.data))
+    return out
+
+
+def _tracer_rc_gradient(self, samples: np.ndarray):
+    q = np.ascontiguousarray(samples, dtype=RC_QUERY_DTYPE)
+    g = np.empty(RC_PARAMS, dtype=np.float32)
+    loss = c_float()
+    _check(library().hdcx_rc_gradient(self._h, q.ctypes.data, len(q), g.ctypes.data, ctypes.byref(loss)))
+    return float(loss.value), g
+
+
+def _tracer_rc_train(self, samples: np.ndarray, steps: int = 1) -> float:
+    q = np.ascontiguousarray(samples, dtype=RC_QUERY_DTYPE)
+    loss = c_float()
+    _check(library().hdcx_rc_train(self._h, q.ctypes.data, len(q), steps, ctypes.byref(loss)))
+    return float(loss.value)
+
+
+def _tracer_rc_stats(self) -> dict:
+    s = RcStats()
+    _check(library().hdcx_rc_get_stats(self._h, ctypes.byref(s)))
+    return {name: getattr(s, name) for name, _ in s._fields_}
+
+
+PathTracer.set_radiance_cache = _tracer_set_radiance_cache
+PathTracer.rc_params = _tracer_rc_params
+PathTracer.rc_set_params = _tracer_rc_set_params
+PathTracer.rc_infer = _tracer_rc_infer
+PathTracer.rc_gradient = _tracer_rc_gradient
+PathTracer.rc_train = _tracer_rc_train
+PathTracer.rc_stats = _tracer_rc_stats
+EXPORTED_SYMBOLS = EXPORTED_SYMBOLS + (
+    "hdcx_set_radiance_cache", "hdcx_rc_get_params", "hdcx_rc_set_params", "hdcx_rc_infer", "hdcx_rc_gradient",
+    "hdcx_rc_train", "hdcx_rc_get_stats")

@@ -41,7 +41,7 @@ HIP_FLAGS = [
 ]
 
 HOST_SOURCES = ["packer.cpp", "procedural.cpp", "procedural_showcase.cpp", "bvh_host.cpp"]
-HIP_SOURCES = ["kernels.hip", "tracer.cpp"]
+HIP_SOURCES = ["kernels.hip", "rcache.hip", "tracer.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

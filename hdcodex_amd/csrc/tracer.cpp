@@ -7,6 +7,7 @@
 #include "kernel_api.h"
 #include "packer.hpp"
 #include "procedural.hpp"
+#include "rcache.h"
 
 #include <hip/hip_runtime.h>
 
@@ -127,6 +128,15 @@ struct hdcx_tracer {
     std::uint32_t filmWidth = 0, filmHeight = 0;
     // Ray queries.
     DeviceBuffer queryRays, queryHits;
+    // Radiance cache.
+    static constexpr unsigned kRcGradBlocks = 256;
+    hdcx_radiance_cache_config rcConfig{};
+    bool rcEnabled = false;
+    bool rcInitialised = false;
+    std::uint64_t rcSeed = 0;
+    int rcStep = 0;
+    hdcx_rc_stats rcStats{};
+    DeviceBuffer rcMaster, rcM, rcV, rcGrad, rcW, rcWt, rcLoss, rcQueries, rcOut;
     // Instrumentation.
     std::uint32_t instrumentation = 0;
     hdcx_kernel_stats stats{};
@@ -179,6 +189,86 @@ struct hdcx_tracer {
         }
         pending.clear();
         eventsUsed = 0;
+    }
+
+    RcState Rc()
+    {
+        RcState r{};
+        r.master = rcMaster.as<float>();
+        r.m = rcM.as<float>();
+        r.v = rcV.as<float>();
+        r.grad = rcGrad.as<float>();
+        r.w = rcW.as<__bf16>();
+        r.wt = rcWt.as<__bf16>();
+        r.loss = rcLoss.as<float>();
+        return r;
+    }
+
+    RcNetwork RcNet() { return RcNetwork{rcW.as<__bf16>(), rcWt.as<__bf16>()}; }
+
+    // Allocates the cache state and (re)initialises the weights for `seed`:
+    // He-uniform bounds sqrt(6 / fan_in), the output layer scaled by 0.5;
+    // Adam moments and gradient partials zeroed.
+    void EnsureRc(std::uint64_t seed)
+    {
+        if (rcInitialised && seed == rcSeed) return;
+        Activate();
+        rcMaster.Allocate(kRcParams * sizeof(float));
+        rcM.Allocate(kRcParams * sizeof(float));
+        rcV.Allocate(kRcParams * sizeof(float));
+        rcGrad.Allocate(static_cast<std::size_t>(kRcGradBlocks) * kRcParams * sizeof(float));
+        rcW.Allocate(kRcParams * sizeof(std::uint16_t));
+        rcWt.Allocate(4 * 4096 * sizeof(std::uint16_t));
+        rcLoss.Allocate(2 * sizeof(float));
+        std::vector<float> w(kRcParams);
+        std::uint64_t state = seed ^ 0x5deece66dULL;
+        auto next = [&]() {
+            std::uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+            return static_cast<double>((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+        };
+        for (int i = 0; i < kRcParams; ++i) {
+            const int fanIn = i < kRcW1 ? kRcIn : kRcHidden;
+            const double bound = std::sqrt(6.0 / fanIn) * (i >= kRcW5 ? 0.5 : 1.0);
+            w[i] = static_cast<float>((2.0 * next() - 1.0) * bound);
+        }
+        Check(hipMemcpyAsync(rcMaster.as<void>(), w.data(), kRcParams * sizeof(float), hipMemcpyHostToDevice, stream), "rc init");
+        Check(hipMemsetAsync(rcM.as<void>(), 0, rcM.bytes(), stream), "rc init");
+        Check(hipMemsetAsync(rcV.as<void>(), 0, rcV.bytes(), stream), "rc init");
+        Check(hipMemsetAsync(rcGrad.as<void>(), 0, rcGrad.bytes(), stream), "rc init");
+        Check(LaunchRcPublish(Rc(), stream), "rc publish");
+        Check(hipStreamSynchronize(stream), "rc init");
+        rcSeed = seed;
+        rcStep = 0;
+        rcInitialised = true;
+    }
+
+    // One Adam step per mini-batch over `count` samples already on the device.
+    float RcTrain(const RcQuery* samples, const std::uint32_t* deviceCount, std::uint32_t count, std::uint32_t steps)
+    {
+        const hdcx_radiance_cache_config& c = rcConfig;
+        steps = std::max(steps, 1U);
+        float lossHost[2] = {0.0F, 0.0F};
+        for (std::uint32_t i = 0; i < steps; ++i) {
+            const std::uint32_t b0 = static_cast<std::uint32_t>(static_cast<std::uint64_t>(count) * i / steps);
+            const std::uint32_t b1 = static_cast<std::uint32_t>(static_cast<std::uint64_t>(count) * (i + 1) / steps);
+            if (b1 <= b0) continue;
+            (void)deviceCount;
+            Check(hipMemsetAsync(rcLoss.as<void>(), 0, 2 * sizeof(float), stream), "rc loss");
+            Check(LaunchRcGradient(RcNet(), samples + b0, nullptr, b1 - b0, rcGrad.as<float>(), rcLoss.as<float>(),
+                                   kRcGradBlocks, stream),
+                  "rc gradient");
+            Check(LaunchRcAdamBlocks(Rc(), static_cast<int>(kRcGradBlocks), ++rcStep, c.learning_rate, c.beta1, c.beta2,
+                                     c.epsilon, stream),
+                  "rc adam");
+            ++rcStats.adam_steps;
+            rcStats.train_samples += b1 - b0;
+        }
+        Check(hipMemcpyAsync(lossHost, rcLoss.as<void>(), sizeof(lossHost), hipMemcpyDeviceToHost, stream), "rc loss");
+        Check(hipStreamSynchronize(stream), "rc train");
+        rcStats.last_loss = lossHost[1] > 0.0F ? lossHost[0] / lossHost[1] : 0.0F;
+        return rcStats.last_loss;
     }
 
     PathBuffers Paths()
@@ -656,6 +746,116 @@ int hdcx_bvh_build_host(const hdcx_scene_view* scene, hdcx_bvh_info* out_info, v
         out_info->depth = bvh.depth;
         if (out_nodes) std::memcpy(out_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node));
         if (out_slots) std::memcpy(out_slots, bvh.triangles.data(), bvh.triangles.size() * sizeof(float));
+    });
+}
+
+int hdcx_set_radiance_cache(hdcx_tracer* tracer, const hdcx_radiance_cache_config* config)
+{
+    return Guard([&] {
+        if (!tracer) throw std::invalid_argument("tracer is null");
+        if (!config || !config->enabled) {
+            tracer->rcEnabled = false;
+            return;
+        }
+        hdcx_radiance_cache_config c = *config;
+        if (c.train_samples == 0) c.train_samples = 65536;
+        if (c.train_steps == 0) c.train_steps = 4;
+        if (!(c.learning_rate > 0.0F)) c.learning_rate = 1e-2F;
+        if (!(c.beta1 > 0.0F)) c.beta1 = 0.9F;
+        if (!(c.beta2 > 0.0F)) c.beta2 = 0.99F;
+        if (!(c.epsilon > 0.0F)) c.epsilon = 1e-8F;
+        if (c.terminate_bounce >= HDCX_MAX_BOUNCES) throw std::invalid_argument("terminate_bounce out of range");
+        tracer->rcConfig = c;
+        tracer->EnsureRc(c.seed);
+        tracer->rcEnabled = true;
+    });
+}
+
+int hdcx_rc_get_params(hdcx_tracer* tracer, float* out_params)
+{
+    return Guard([&] {
+        if (!tracer || !out_params) throw std::invalid_argument("null argument");
+        tracer->EnsureRc(tracer->rcInitialised ? tracer->rcSeed : 0);
+        Check(hipMemcpy(out_params, tracer->rcMaster.as<void>(), kRcParams * sizeof(float), hipMemcpyDeviceToHost), "rc params");
+    });
+}
+
+int hdcx_rc_set_params(hdcx_tracer* tracer, const float* params)
+{
+    return Guard([&] {
+        if (!tracer || !params) throw std::invalid_argument("null argument");
+        tracer->EnsureRc(tracer->rcInitialised ? tracer->rcSeed : 0);
+        Check(hipMemcpy(tracer->rcMaster.as<void>(), params, kRcParams * sizeof(float), hipMemcpyHostToDevice), "rc params");
+        Check(hipMemset(tracer->rcM.as<void>(), 0, tracer->rcM.bytes()), "rc params");
+        Check(hipMemset(tracer->rcV.as<void>(), 0, tracer->rcV.bytes()), "rc params");
+        tracer->rcStep = 0;
+        Check(LaunchRcPublish(tracer->Rc(), tracer->stream), "rc publish");
+        Check(hipStreamSynchronize(tracer->stream), "rc params");
+    });
+}
+
+int hdcx_rc_infer(hdcx_tracer* tracer, const hdcx_rc_query* queries, uint32_t count, float* out)
+{
+    return Guard([&] {
+        if (!tracer || (count && (!queries || !out))) throw std::invalid_argument("null argument");
+        if (count == 0) return;
+        tracer->EnsureRc(tracer->rcInitialised ? tracer->rcSeed : 0);
+        tracer->rcQueries.Upload(queries, static_cast<std::size_t>(count) * sizeof(RcQuery), tracer->stream);
+        tracer->rcOut.Allocate(static_cast<std::size_t>(count) * 3 * sizeof(float));
+        Check(LaunchRcInfer(tracer->RcNet(), tracer->rcQueries.as<RcQuery>(), nullptr, count, tracer->rcOut.as<float>(),
+                            nullptr, nullptr, tracer->maxBlocks, tracer->stream),
+              "rc infer");
+        Check(hipMemcpyAsync(out, tracer->rcOut.as<void>(), static_cast<std::size_t>(count) * 3 * sizeof(float),
+                             hipMemcpyDeviceToHost, tracer->stream),
+              "rc infer");
+        Check(hipStreamSynchronize(tracer->stream), "rc infer");
+    });
+}
+
+int hdcx_rc_gradient(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, float* out_grad, float* out_loss)
+{
+    return Guard([&] {
+        if (!tracer || !samples || !out_grad || count == 0) throw std::invalid_argument("bad argument");
+        tracer->EnsureRc(tracer->rcInitialised ? tracer->rcSeed : 0);
+        hipStream_t st = tracer->stream;
+        tracer->rcQueries.Upload(samples, static_cast<std::size_t>(count) * sizeof(RcQuery), st);
+        tracer->rcOut.Allocate(std::max<std::size_t>(static_cast<std::size_t>(count) * 3, kRcParams) * sizeof(float));
+        Check(hipMemsetAsync(tracer->rcLoss.as<void>(), 0, 2 * sizeof(float), st), "rc loss");
+        Check(LaunchRcGradient(tracer->RcNet(), tracer->rcQueries.as<RcQuery>(), nullptr, count, tracer->rcGrad.as<float>(),
+                               tracer->rcLoss.as<float>(), hdcx_tracer::kRcGradBlocks, st),
+              "rc gradient");
+        Check(LaunchRcReduce(tracer->rcGrad.as<float>(), static_cast<int>(hdcx_tracer::kRcGradBlocks), tracer->rcOut.as<float>(), st),
+              "rc reduce");
+        float loss[2];
+        Check(hipMemcpyAsync(out_grad, tracer->rcOut.as<void>(), kRcParams * sizeof(float), hipMemcpyDeviceToHost, st), "rc grad");
+        Check(hipMemcpyAsync(loss, tracer->rcLoss.as<void>(), sizeof(loss), hipMemcpyDeviceToHost, st), "rc grad");
+        Check(hipStreamSynchronize(st), "rc gradient");
+        if (out_loss) *out_loss = loss[1] > 0.0F ? loss[0] / loss[1] : 0.0F;
+    });
+}
+
+int hdcx_rc_train(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, uint32_t steps, float* out_loss)
+{
+    return Guard([&] {
+        if (!tracer || !samples || count == 0) throw std::invalid_argument("bad argument");
+        tracer->EnsureRc(tracer->rcInitialised ? tracer->rcSeed : 0);
+        if (tracer->rcConfig.learning_rate <= 0.0F) {
+            tracer->rcConfig.learning_rate = 1e-2F;
+            tracer->rcConfig.beta1 = 0.9F;
+            tracer->rcConfig.beta2 = 0.99F;
+            tracer->rcConfig.epsilon = 1e-8F;
+        }
+        tracer->rcQueries.Upload(samples, static_cast<std::size_t>(count) * sizeof(RcQuery), tracer->stream);
+        const float loss = tracer->RcTrain(tracer->rcQueries.as<RcQuery>(), nullptr, count, steps);
+        if (out_loss) *out_loss = loss;
+    });
+}
+
+int hdcx_rc_get_stats(hdcx_tracer* tracer, hdcx_rc_stats* out_stats)
+{
+    return Guard([&] {
+        if (!tracer || !out_stats) throw std::invalid_argument("null argument");
+        *out_stats = tracer->rcStats;
     });
 }
 

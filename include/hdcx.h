@@ -205,6 +205,67 @@ typedef struct hdcx_bvh_info {
 int hdcx_bvh_build_host(const hdcx_scene_view* scene, hdcx_bvh_info* out_info,
                         void* out_nodes, float* out_slots);
 
+/* ---- Radiance cache (no counterpart in the reference: its neural
+ * rendering plan is unimplemented, docs/neural-rendering-plan.md:32-48).
+ * An MLP (128 encoded inputs -> 5 x 64 ReLU -> 3) trained online on the
+ * path tracer's own radiance estimates predicts the radiance scattered from
+ * a surface point at the three wavelengths of a hero triple; paths
+ * terminate into it at a fixed vertex. BF16 on MFMA, FP32 accumulation. ---- */
+
+#define HDCX_RC_PARAMS 24768
+
+/* One cache query or training sample (48 bytes). */
+typedef struct hdcx_rc_query {
+    float position[3];  /* normalised to the scene bounds, [0,1]^3 */
+    float direction[3]; /* unit outgoing direction */
+    float lambda0;      /* base wavelength of the hero triple (lane 0), nm */
+    float roughness;
+    float albedo;
+    uint32_t lane;      /* 0..2: which output the sample trains / the path uses */
+    uint32_t path;      /* path id (ignored by hdcx_rc_infer) */
+    float target;       /* training target radiance of that lane */
+} hdcx_rc_query;
+
+typedef struct hdcx_radiance_cache_config {
+    uint32_t enabled;
+    uint32_t train_samples;    /* training samples gathered per frame (default 65536) */
+    uint32_t terminate_bounce; /* 0-based path vertex answered by the cache (default 2) */
+    uint32_t train_steps;      /* Adam steps (mini-batches) per frame (default 4) */
+    float learning_rate;       /* default 1e-2 */
+    float beta1;               /* default 0.9 */
+    float beta2;               /* default 0.99 */
+    float epsilon;             /* default 1e-8 */
+    uint64_t seed;             /* weight initialisation */
+} hdcx_radiance_cache_config;
+
+typedef struct hdcx_rc_stats {
+    uint64_t frames;
+    uint64_t queries;          /* cache queries answered */
+    uint64_t train_samples;    /* samples trained on */
+    uint64_t adam_steps;
+    float last_loss;           /* mean relative L2 loss of the last step */
+    float reserved;
+    double infer_ms;           /* HIP-event time of inference launches (when timing is on) */
+    double train_ms;
+} hdcx_rc_stats;
+
+/* Enables/disables the cache for hdcx_render and (re)initialises the
+ * network when the config's seed changes. NULL disables. */
+int hdcx_set_radiance_cache(hdcx_tracer* tracer, const hdcx_radiance_cache_config* config);
+/* FP32 master weights, row-major W0 [64][128], W1..W4 [64][64], W5 [3][64]. */
+int hdcx_rc_get_params(hdcx_tracer* tracer, float* out_params);
+int hdcx_rc_set_params(hdcx_tracer* tracer, const float* params);
+/* Network outputs (3 per query) for host-side queries. */
+int hdcx_rc_infer(hdcx_tracer* tracer, const hdcx_rc_query* queries, uint32_t count, float* out);
+/* Gradient of the mean relative L2 loss over the samples, without an
+ * update; out_loss receives the mean loss. */
+int hdcx_rc_gradient(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, float* out_grad,
+                     float* out_loss);
+/* `steps` Adam steps over consecutive mini-batches of the samples. */
+int hdcx_rc_train(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, uint32_t steps,
+                  float* out_loss);
+int hdcx_rc_get_stats(hdcx_tracer* tracer, hdcx_rc_stats* out_stats);
+
 /* ---- Measurement (no counterpart in the reference, which has no GPU
  * timing: docs/architecture.md:255-257). ---- */
 
