@@ -82,7 +82,33 @@ struct PathBuffers {
     std::uint32_t* shadowQueue;
     std::uint32_t* counters; // [0],[1] extend queue sizes; [2] shadow queue size
     unsigned long long* totals;  // [0] extend rays, [1] shadow-queued paths, [2] shadow rays traced
+    float* lambda0;          // wavelength of lane 0 of the path's hero triple
     std::uint32_t capacity;
+};
+
+struct RcQuery;
+
+// Radiance-cache hooks of the integrator. Training pixel samples (all three
+// lanes) are those with (ps + trainOffset) % trainStride == 0; they run the
+// plain estimator and record their first recordVertices vertices. Other
+// paths stop at vertex terminateBounce and queue a cache query.
+struct CacheParams {
+    std::uint32_t enabled;
+    std::uint32_t terminateBounce;
+    std::uint32_t trainStride;
+    std::uint32_t trainOffset;
+    std::uint32_t recordVertices;
+    std::uint32_t queryCapacity;
+    std::uint32_t recordSlots;     // training pixel-sample slots in the record buffer
+    std::uint32_t sampleCapacity;  // training samples kept per frame
+    float lo[3];                   // scene box used to normalise positions
+    float invExtent[3];
+    RcQuery* queries;              // per chunk
+    std::uint32_t* queryCount;
+    RcQuery* records;              // [slot][lane][vertex]; target holds the radiance after emission
+    float* recordThroughput;
+    RcQuery* samples;              // per frame
+    std::uint32_t* sampleCount;
 };
 
 // Pixel-sample chunk: samples [s0, s1) x pixels [p0, p1); path id =

@@ -9,6 +9,7 @@
 #include "kernel_api.h"
 #include "dev_shading.h"
 #include "dev_traverse.h"
+#include "rcache.h"
 
 namespace hdcx {
 
@@ -35,7 +36,8 @@ __device__ __forceinline__ void waveAppend(bool keep, std::uint32_t item, std::u
 // ---------------------------------------------------------------------------
 // raygen: seed, hero wavelength, jittered pinhole ray (:1198-1224).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc c, std::uint32_t pathCount)
+__global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc c, std::uint32_t pathCount,
+                                                      CacheParams cp)
 {
     const std::uint32_t pixelsInChunk = c.p1 - c.p0;
     for (std::uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x; pid < pathCount; pid += gridDim.x * blockDim.x) {
@@ -45,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc
         const std::uint32_t sampleNumber = c.sampleBase + c.s0 + sOff;
         const std::uint32_t px = pixel % c.width, py = pixel / c.width;
         std::uint32_t rng = hashState(pixel ^ (sampleNumber * 0x9e3779b9U) ^ 0xa511e9b3U);
-        float wl;
+        float wl, wl0;
         {
 #pragma clang fp contract(off)
             const float shift = __uint2float_rn(hashState(pixel ^ 0x68bc21ebU)) * (1.0f / 4294967296.0f);
@@ -53,6 +55,15 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc
             const float step = __uint2float_rn(lane) / 3.0f;
             const float f = shift + phase + step;
             wl = 380.0f + 400.0f * (f - floorf(f));
+            const float f0 = shift + phase;
+            wl0 = 380.0f + 400.0f * (f0 - floorf(f0));
+        }
+        pb.lambda0[pid] = wl0;
+        if (cp.enabled && (ps + cp.trainOffset) % cp.trainStride == 0U) {
+            const std::uint32_t slot = (ps + cp.trainOffset) / cp.trainStride;
+            if (slot < cp.recordSlots)
+                for (std::uint32_t j = 0; j < cp.recordVertices; ++j)
+                    cp.records[(slot * 3U + lane) * cp.recordVertices + j].lane = 0xffffffffU;
         }
         const float jx = randomFloat(rng), jy = randomFloat(rng);
         const float u = (static_cast<float>(px) + jx) / static_cast<float>(c.width);
@@ -72,7 +83,25 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc
         for (int k = 0; k < kShadowSlots; ++k) pb.shadowLight[k * pb.capacity + pid] = 0xffffffffU;
         pb.queue[0][pid] = pid;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) pb.counters[0] = pathCount;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.counters[0] = pathCount;
+        if (cp.enabled) *cp.queryCount = 0U;
+    }
+}
+
+// Index of this lane's entry in a compacted array (one atomic per wave), or
+// 0xffffffff when it does not want one.
+__device__ __forceinline__ std::uint32_t waveSlot(bool want, std::uint32_t* counter)
+{
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ULL) return 0xffffffffU;
+    const unsigned long long active = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(active)) - 1;
+    const std::uint32_t lane = laneId();
+    std::uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(counter, static_cast<std::uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    return want ? base + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL))) : 0xffffffffU;
 }
 
 __device__ __forceinline__ float materialOpacity(const DeviceScene& s, std::uint32_t prim, float bu, float bv)
@@ -201,7 +230,7 @@ __device__ __forceinline__ void queueShadow(const PathBuffers& pb, std::uint32_t
 // additions (emission, analytic NEE, mesh-light NEE).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers pb, std::uint32_t in, int bounce,
-                                                     int lastBounce)
+                                                     int lastBounce, CacheParams cp)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
     TraversalStack st;
@@ -218,6 +247,8 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
         bool keep = false;
         int slot = 0;
         std::uint32_t pid = 0;
+        bool wantQuery = false;
+        RcQuery query{};
         if (i < count) {
             pid = queue[i];
             const float4 hit4 = pb.hit[pid];
@@ -425,7 +456,42 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
                     c.pTranslucent = baseE > 1e-5f ? c.translucentE / baseE : 0.0f;
                     const float dSpecE = c.specE + lumGrey(c.coatContribution) + c.pSheen;
 
-                    if (s.lightCount > 0U) {
+                    // Radiance cache: training paths record this vertex (the
+                    // radiance after its emission and the throughput reaching
+                    // it); other paths stop at terminateBounce and queue a
+                    // query for the radiance scattered toward the viewer.
+                    bool cacheStop = false;
+                    if (cp.enabled) {
+                        const std::uint32_t ps = pid / 3U, lane = pid - ps * 3U;
+                        RcQuery rq;
+                        rq.px = (hp.x - cp.lo[0]) * cp.invExtent[0];
+                        rq.py = (hp.y - cp.lo[1]) * cp.invExtent[1];
+                        rq.pz = (hp.z - cp.lo[2]) * cp.invExtent[2];
+                        rq.dx = c.view.x;
+                        rq.dy = c.view.y;
+                        rq.dz = c.view.z;
+                        rq.lambda0 = pb.lambda0[pid];
+                        rq.roughness = 0.5f * (c.rx + c.ry);
+                        rq.albedo = clampf(0.2126f * baseRgb.x + 0.7152f * baseRgb.y + 0.0722f * baseRgb.z, 0.0f, 1.0f);
+                        rq.lane = lane;
+                        rq.path = pid;
+                        rq.target = radiance;
+                        if ((ps + cp.trainOffset) % cp.trainStride == 0U) {
+                            const std::uint32_t tslot = (ps + cp.trainOffset) / cp.trainStride;
+                            if (tslot < cp.recordSlots && static_cast<std::uint32_t>(bounce) < cp.recordVertices) {
+                                const std::uint32_t r = (tslot * 3U + lane) * cp.recordVertices + static_cast<std::uint32_t>(bounce);
+                                cp.records[r] = rq;
+                                cp.recordThroughput[r] = throughput;
+                            }
+                        } else if (bounce == static_cast<int>(cp.terminateBounce)) {
+                            cacheStop = true;
+                            wantQuery = true;
+                            query = rq;
+                        }
+                    }
+
+                    if (cacheStop) {
+                    } else if (s.lightCount > 0U) {
                         // Analytic NEE (:1469-1592).
                         std::uint32_t li;
                         float selPdf;
@@ -530,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
                             }
                         }
                     }
-                    if (s.hasEmissive) {
+                    if (s.hasEmissive && !cacheStop) {
                         // Emissive-mesh NEE (:1670-1731).
                         const float target = randomFloat(rng);
                         std::uint32_t lo = 0, hi = s.primitiveCount - 1U;
@@ -577,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
                         }
                     }
 
-                    if (!lastBounce) {
+                    if (!lastBounce && !cacheStop) {
                         bool walked = false;
                         if (randomFloat(rng) < c.transmission) {
                             // Transmission (:1735-1777).
@@ -708,6 +774,10 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
         }
         waveAppend(keep, pid, outQueue, outCounter);
         waveAppend(slot > 0, pid, pb.shadowQueue, pb.counters + 2);
+        if (cp.enabled) {
+            const std::uint32_t qslot = waveSlot(wantQuery, cp.queryCount);
+            if (wantQuery && qslot < cp.queryCapacity) cp.queries[qslot] = query;
+        }
     }
 }
 
@@ -811,6 +881,29 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(DeviceScene s, PathBuffer
     if (threadIdx.x == 0 && tracedInBlock) atomicAdd(&pb.totals[2], static_cast<unsigned long long>(tracedInBlock));
 }
 
+// Training targets of a chunk: for each recorded vertex j of a training
+// path, the radiance the path gathered after vertex j's emission divided by
+// the throughput that reached j (an unbiased estimate of the radiance
+// scattered at j toward the previous vertex).
+__global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32_t records)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *reinterpret_cast<unsigned long long*>(cp.queryCount + 2) += min(*cp.queryCount, cp.queryCapacity);
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < records; i += gridDim.x * blockDim.x) {
+        RcQuery r = cp.records[i];
+        if (r.lane > 2U) continue;
+        const float t = cp.recordThroughput[i];
+        if (!(t > 0.0f)) continue;
+        const float target = (pb.radiance[r.path] - r.target) / t;
+        if (!isfinite(target)) continue;
+        const std::uint32_t k = atomicAdd(cp.sampleCount, 1U);
+        if (k < cp.sampleCapacity) {
+            r.target = fmaxf(target, 0.0f);
+            cp.samples[k] = r;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // film: per pixel of the chunk, sum over its samples of
 // (sum over 3 lanes of sensor(lambda) x radiance) / 3 in FP64 (:1882-1884).
@@ -908,9 +1001,16 @@ static unsigned GridFor(std::uint32_t work, unsigned maxBlocks)
     return blocks == 0 ? 1U : (blocks < maxBlocks ? blocks : maxBlocks);
 }
 
-hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t paths, unsigned maxBlocks, hipStream_t st)
+hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t paths, const CacheParams& cp,
+                        unsigned maxBlocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(raygenKernel, dim3(GridFor(paths, maxBlocks)), dim3(kBlock), 0, st, pb, c, paths);
+    hipLaunchKernelGGL(raygenKernel, dim3(GridFor(paths, maxBlocks)), dim3(kBlock), 0, st, pb, c, paths, cp);
+    return hipGetLastError();
+}
+hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, std::uint32_t records, unsigned maxBlocks,
+                                hipStream_t st)
+{
+    hipLaunchKernelGGL(gatherTrainingKernel, dim3(GridFor(records, maxBlocks)), dim3(kBlock), 0, st, pb, cp, records);
     return hipGetLastError();
 }
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
@@ -921,9 +1021,9 @@ hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32
     return hipGetLastError();
 }
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                       std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+                       const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(shadeKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb, in, bounce, last);
+    hipLaunchKernelGGL(shadeKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb, in, bounce, last, cp);
     return hipGetLastError();
 }
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,

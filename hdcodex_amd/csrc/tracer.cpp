@@ -121,7 +121,7 @@ struct hdcx_tracer {
     Bvh4 hostBvhInfo;  // depth / stack statistics of the uploaded BVH
     // Path state.
     DeviceBuffer rayO, rayD, hit, rng, wavelength, throughput, radiance, flags, mis, medium, shadowO, shadowD,
-        shadowLight, queue0, queue1, shadowQueue, counters, totals, traversalCounters;
+        shadowLight, queue0, queue1, shadowQueue, counters, totals, traversalCounters, lambda0;
     std::uint32_t capacity = 0;
     // Film.
     DeviceBuffer filmMean, filmBatch, filmOut;
@@ -137,6 +137,10 @@ struct hdcx_tracer {
     int rcStep = 0;
     hdcx_rc_stats rcStats{};
     DeviceBuffer rcMaster, rcM, rcV, rcGrad, rcW, rcWt, rcLoss, rcQueries, rcOut;
+    DeviceBuffer rcChunkQueries, rcCounters, rcRecords, rcRecordThroughput, rcSamples;
+    std::uint32_t rcFrame = 0;
+    bool rcCountersReady = false;
+    float sceneLo[3] = {0, 0, 0}, sceneInvExtent[3] = {1, 1, 1};
     // Instrumentation.
     std::uint32_t instrumentation = 0;
     hdcx_kernel_stats stats{};
@@ -292,6 +296,7 @@ struct hdcx_tracer {
         pb.shadowQueue = shadowQueue.as<std::uint32_t>();
         pb.counters = counters.as<std::uint32_t>();
         pb.totals = totals.as<unsigned long long>();
+        pb.lambda0 = lambda0.as<float>();
         pb.capacity = capacity;
         return pb;
     }
@@ -316,6 +321,7 @@ struct hdcx_tracer {
         queue0.Allocate(n * 4);
         queue1.Allocate(n * 4);
         shadowQueue.Allocate(n * 4);
+        lambda0.Allocate(n * 4);
         capacity = paths;
     }
 
@@ -400,6 +406,18 @@ void SetScene(hdcx_tracer* t, const hdcx_scene_view* view)
     const hdcx_scene_view& v = *view;
     hipStream_t st = t->stream;
 
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (std::uint32_t i = 0; i < v.vertex_count; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], v.positions[i * 4 + k]);
+                hi[k] = std::max(hi[k], v.positions[i * 4 + k]);
+            }
+        for (int k = 0; k < 3; ++k) {
+            t->sceneLo[k] = lo[k];
+            t->sceneInvExtent[k] = hi[k] > lo[k] ? 1.0F / (hi[k] - lo[k]) : 1.0F;
+        }
+    }
     Bvh4 bvh = BuildBvh4Host(v.positions, v.indices, v.triangle_count);
     ValidateBvh4(bvh, v.triangle_count);
     if (bvh.maxStack > static_cast<std::uint32_t>(kMaxStack))
@@ -520,6 +538,45 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     samplesPerChunk = std::min(samplesPerChunk, samples);
     t->EnsurePaths(pixelsPerChunk * samplesPerChunk * 3U);
     PathBuffers pb = t->Paths();
+    // Radiance cache: training pixel samples are spread evenly over the
+    // frame so that about train_samples vertices are recorded (2.5 per lane
+    // path on average, 3 lanes, up to 4 vertices each).
+    CacheParams cp{};
+    const bool cache = t->rcEnabled;
+    constexpr std::uint32_t kRecordVertices = 4;
+    if (cache) {
+        const std::uint64_t frameSamples = static_cast<std::uint64_t>(pixels) * samples;
+        const std::uint64_t wanted = std::max<std::uint64_t>(1, t->rcConfig.train_samples * 2ULL / 15ULL);
+        cp.enabled = 1;
+        cp.terminateBounce = t->rcConfig.terminate_bounce;
+        cp.trainStride = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, frameSamples / wanted));
+        cp.trainOffset = static_cast<std::uint32_t>((t->rcFrame * 2654435761ULL) % cp.trainStride);
+        cp.recordVertices = kRecordVertices;
+        cp.queryCapacity = t->capacity;
+        cp.recordSlots = pixelsPerChunk * samplesPerChunk / cp.trainStride + 2;
+        cp.sampleCapacity = t->rcConfig.train_samples;
+        for (int k = 0; k < 3; ++k) {
+            cp.lo[k] = t->sceneLo[k];
+            cp.invExtent[k] = t->sceneInvExtent[k];
+        }
+        t->rcChunkQueries.Allocate(std::max<std::size_t>(t->rcChunkQueries.bytes(), static_cast<std::size_t>(t->capacity) * sizeof(RcQuery)));
+        t->rcCounters.Allocate(64);
+        const std::size_t records = static_cast<std::size_t>(cp.recordSlots) * 3 * kRecordVertices;
+        t->rcRecords.Allocate(std::max<std::size_t>(t->rcRecords.bytes(), records * sizeof(RcQuery)));
+        t->rcRecordThroughput.Allocate(std::max<std::size_t>(t->rcRecordThroughput.bytes(), records * sizeof(float)));
+        t->rcSamples.Allocate(std::max<std::size_t>(t->rcSamples.bytes(), static_cast<std::size_t>(cp.sampleCapacity) * sizeof(RcQuery)));
+        cp.queries = t->rcChunkQueries.as<RcQuery>();
+        cp.queryCount = t->rcCounters.as<std::uint32_t>();
+        cp.records = t->rcRecords.as<RcQuery>();
+        cp.recordThroughput = t->rcRecordThroughput.as<float>();
+        cp.samples = t->rcSamples.as<RcQuery>();
+        cp.sampleCount = t->rcCounters.as<std::uint32_t>() + 1;
+        if (!t->rcCountersReady) {
+            Check(hipMemsetAsync(t->rcCounters.as<void>(), 0, 64, st), "rc counters");
+            t->rcCountersReady = true;
+        }
+        Check(hipMemsetAsync(t->rcCounters.as<void>(), 0, 8, st), "rc counters");
+    }
     TraversalCounters* ctr = (t->instrumentation & 2U) ? t->traversalCounters.as<TraversalCounters>() : nullptr;
     TraversalCounters* shadowCtr = ctr ? ctr + 1 : nullptr;
     ChunkDesc c{};
@@ -540,15 +597,25 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
             c.p0 = p0;
             c.p1 = p1;
             const std::uint32_t paths = (s1 - s0) * (p1 - p0) * 3U;
-            t->Timed(HDCX_KERNEL_RAYGEN, [&] { return LaunchRaygen(pb, c, paths, t->maxBlocks, st); });
+            t->Timed(HDCX_KERNEL_RAYGEN, [&] { return LaunchRaygen(pb, c, paths, cp, t->maxBlocks, st); });
             t->stats.rays[HDCX_KERNEL_RAYGEN] += paths;
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
                 t->Timed(HDCX_KERNEL_EXTEND, [&] { return LaunchExtend(t->scene, pb, in, paths, t->maxBlocks, ctr, st); });
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
-                    return LaunchShade(t->scene, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, paths, t->maxBlocks, st);
+                    return LaunchShade(t->scene, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, cp, paths, t->maxBlocks, st);
                 });
+                if (cache && b == cp.terminateBounce) {
+                    t->Timed(HDCX_KERNEL_CACHE, [&] {
+                        return LaunchRcInfer(t->RcNet(), cp.queries, cp.queryCount, paths, nullptr, pb.radiance, pb.throughput,
+                                             t->maxBlocks, st);
+                    });
+                }
                 t->Timed(HDCX_KERNEL_SHADOW, [&] { return LaunchShadow(t->scene, pb, paths, t->maxBlocks, shadowCtr, st); });
+            }
+            if (cache) {
+                const std::uint32_t records = cp.recordSlots * 3U * kRecordVertices;
+                Check(LaunchGatherTraining(pb, cp, records, t->maxBlocks, st), "gather training samples");
             }
             t->Timed(HDCX_KERNEL_FILM, [&] { return LaunchFilm(pb, c, t->filmBatch.as<double4>(), t->maxBlocks, st); });
             t->stats.rays[HDCX_KERNEL_FILM] += (p1 - p0);
@@ -560,6 +627,29 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     if (out) Check(hipMemcpyAsync(out, t->filmOut.as<void>(), static_cast<std::size_t>(pixels) * 16, hipMemcpyDeviceToHost, st),
                    "film readback");
     Check(hipStreamSynchronize(st), "render");
+    if (cache) {
+        // Online training on this frame's samples; the next frame uses the
+        // updated weights.
+        std::uint32_t counts[2] = {0, 0};
+        Check(hipMemcpy(counts, t->rcCounters.as<void>(), sizeof(counts), hipMemcpyDeviceToHost), "rc counters");
+        const std::uint32_t n = std::min(counts[1], cp.sampleCapacity);
+        hipEvent_t a = nullptr, b = nullptr;
+        if (t->instrumentation & 1U) {
+            a = t->NextEvent();
+            b = t->NextEvent();
+            Check(hipEventRecord(a, st), "hipEventRecord");
+        }
+        if (n > 0) t->RcTrain(cp.samples, nullptr, n, t->rcConfig.train_steps);
+        if (t->instrumentation & 1U) {
+            Check(hipEventRecord(b, st), "hipEventRecord");
+            Check(hipEventSynchronize(b), "hipEventSynchronize");
+            float ms = 0.0F;
+            Check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+            t->rcStats.train_ms += ms;
+        }
+        ++t->rcStats.frames;
+        ++t->rcFrame;
+    }
     t->CollectTimings();
 }
 
@@ -855,6 +945,12 @@ int hdcx_rc_get_stats(hdcx_tracer* tracer, hdcx_rc_stats* out_stats)
 {
     return Guard([&] {
         if (!tracer || !out_stats) throw std::invalid_argument("null argument");
+        if (tracer->rcCounters.bytes()) {
+            unsigned long long queries = 0;
+            Check(hipMemcpy(&queries, tracer->rcCounters.as<std::uint32_t>() + 2, sizeof(queries), hipMemcpyDeviceToHost),
+                  "rc stats");
+            tracer->rcStats.queries = queries;
+        }
         *out_stats = tracer->rcStats;
     });
 }

@@ -274,7 +274,8 @@ int hdcx_rc_get_stats(hdcx_tracer* tracer, hdcx_rc_stats* out_stats);
 #define HDCX_KERNEL_SHADE 2
 #define HDCX_KERNEL_SHADOW 3
 #define HDCX_KERNEL_FILM 4
-#define HDCX_KERNEL_COUNT 5
+#define HDCX_KERNEL_CACHE 5 /* radiance-cache inference */
+#define HDCX_KERNEL_COUNT 6
 
 typedef struct hdcx_kernel_stats {
     uint64_t launches[HDCX_KERNEL_COUNT];
