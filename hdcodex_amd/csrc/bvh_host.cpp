@@ -102,6 +102,15 @@ public:
         Bvh4 out;
         if (count == 0) return out;
         out.nodes.emplace_back();
+        if (nodes[0].count > 0) {
+            // A single leaf becomes the only child of an inner root: the
+            // traversal always starts at an inner node.
+            Bvh4Node root = EmptyNode();
+            SetChild(root, 0, nodes[0], out);
+            out.nodes[0] = root;
+            MeasureBvh4(out);
+            return out;
+        }
         Collapse(0, 0, out);
         MeasureBvh4(out);
         return out;
@@ -173,6 +182,49 @@ private:
         }
     }
 
+    static Bvh4Node EmptyNode()
+    {
+        Bvh4Node node{};
+        for (int i = 0; i < 4; ++i) {
+            node.lox[i] = node.loy[i] = node.loz[i] = INFINITY;
+            node.hix[i] = node.hiy[i] = node.hiz[i] = -INFINITY;
+            node.child[i] = kEmptyChild;
+        }
+        return node;
+    }
+
+    // Fills child slot i of `node` from BVH2 node c: a leaf writes its
+    // triangles to the slot array; an inner child reserves a BVH4 node and
+    // returns true (to be collapsed by the caller).
+    bool SetChild(Bvh4Node& node, int i, const Node2& c, Bvh4& out)
+    {
+        Box b = c.box;
+        PadBox(b);
+        node.lox[i] = b.lo[0];
+        node.loy[i] = b.lo[1];
+        node.loz[i] = b.lo[2];
+        node.hix[i] = b.hi[0];
+        node.hiy[i] = b.hi[1];
+        node.hiz[i] = b.hi[2];
+        if (c.count > 0) {
+            const std::uint32_t slot = static_cast<std::uint32_t>(out.triangles.size() / 12);
+            for (std::uint32_t j = 0; j < c.count; ++j) {
+                const std::uint32_t tri = refs[c.first + j].id;
+                for (int v = 0; v < 3; ++v) {
+                    const float* p = &positions[static_cast<std::size_t>(indices[tri * 3 + v]) * 4];
+                    float w = 0.0F;
+                    if (v == 0) std::memcpy(&w, &tri, 4);
+                    out.triangles.insert(out.triangles.end(), {p[0], p[1], p[2], w});
+                }
+            }
+            node.child[i] = EncodeLeaf(slot, c.count);
+            return false;
+        }
+        node.child[i] = static_cast<std::uint32_t>(out.nodes.size());
+        out.nodes.emplace_back();
+        return true;
+    }
+
     void Collapse(std::uint32_t n2, std::uint32_t n4, Bvh4& out)
     {
         std::uint32_t kids[4];
@@ -194,41 +246,11 @@ private:
             kids[best] = nodes[inner].left;
             kids[k++] = nodes[inner].right;
         }
-        Bvh4Node node{};
-        for (int i = 0; i < 4; ++i) {
-            node.lox[i] = node.loy[i] = node.loz[i] = INFINITY;
-            node.hix[i] = node.hiy[i] = node.hiz[i] = -INFINITY;
-            node.child[i] = kEmptyChild;
-        }
+        Bvh4Node node = EmptyNode();
         std::uint32_t pending[4];
         int pendingCount = 0;
         for (int i = 0; i < k; ++i) {
-            const Node2& c = nodes[kids[i]];
-            Box b = c.box;
-            PadBox(b);
-            node.lox[i] = b.lo[0];
-            node.loy[i] = b.lo[1];
-            node.loz[i] = b.lo[2];
-            node.hix[i] = b.hi[0];
-            node.hiy[i] = b.hi[1];
-            node.hiz[i] = b.hi[2];
-            if (c.count > 0) {
-                const std::uint32_t slot = static_cast<std::uint32_t>(out.triangles.size() / 12);
-                for (std::uint32_t j = 0; j < c.count; ++j) {
-                    const std::uint32_t tri = refs[c.first + j].id;
-                    for (int v = 0; v < 3; ++v) {
-                        const float* p = &positions[static_cast<std::size_t>(indices[tri * 3 + v]) * 4];
-                        float w = 0.0F;
-                        if (v == 0) std::memcpy(&w, &tri, 4);
-                        out.triangles.insert(out.triangles.end(), {p[0], p[1], p[2], w});
-                    }
-                }
-                node.child[i] = EncodeLeaf(slot, c.count);
-            } else {
-                node.child[i] = static_cast<std::uint32_t>(out.nodes.size());
-                out.nodes.emplace_back();
-                pending[pendingCount++] = static_cast<std::uint32_t>(i);
-            }
+            if (SetChild(node, i, nodes[kids[i]], out)) pending[pendingCount++] = static_cast<std::uint32_t>(i);
         }
         out.nodes[n4] = node;
         // Recurse after the node is stored (out.nodes may reallocate).

@@ -892,6 +892,7 @@ __global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < records; i += gridDim.x * blockDim.x) {
         RcQuery r = cp.records[i];
         if (r.lane > 2U) continue;
+        cp.records[i].lane = 0xffffffffU;  // consumed: never reused by a later chunk
         const float t = cp.recordThroughput[i];
         if (!(t > 0.0f)) continue;
         const float target = (pb.radiance[r.path] - r.target) / t;

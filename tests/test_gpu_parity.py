@@ -37,7 +37,13 @@ def _check_hits(scene, rays, gpu, ref, t64):
     tri = a["positions"][:, :3].astype(np.float64)[a["indices"][gpu["primitive"][ok]]]
     scale = np.abs(tri - rays[ok, None, 0:3].astype(np.float64)).max(axis=(1, 2))
     err = np.abs(gpu["t"][ok].astype(np.float64) - t64[ok])
-    assert np.all(err <= 32 * 2.0 ** -24 * scale)
+    # Edge functions and T carry absolute errors of a few eps x extent; the
+    # division by det (proportional to |cos| of the ray/normal angle)
+    # amplifies them for grazing rays.
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    cos = np.abs((nrm * rays[ok, 4:7].astype(np.float64)).sum(axis=1))
+    assert np.all(err * cos <= 16 * 2.0 ** -24 * scale)
     within = float(np.mean(ulp_distance(gpu["t"][ok], t64[ok].astype(np.float32)) <= 2))
     print(f"hits {int(hit.sum())}: t within 2 ulp of FP64 for {within:.4f}; max |dt|/scale/eps "
           f"{float((err / scale).max(initial=0) / 2.0 ** -24):.2f}")
