@@ -30,6 +30,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mpaths/s + rel-MSE at equal time, radiance-cache vs plain, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+MLP_FLOP_PER_QUERY = 2 * (111 * 64 + 4 * 64 * 64 + 64 * 3)  # 47,360 algorithmic FLOP per inference
+REFERENCE = ROOT / "tests" / "golden" / "c1_reference_4096spp.npz"
 WIDTH, HEIGHT, SPP, BOUNCES = 1920, 1080, 16, 8
 GRID, DOME = 1024, 2048
 
@@ -75,16 +78,70 @@ def cpu_baseline(scene, cam, threads: int) -> dict:
                       f"= {paths} paths in {dt:.2f} s"}
 
 
-def parity(scene, cam, frame: np.ndarray) -> float:
-    """rel-MSE of the GPU frame against the FP64 oracle with the same seeds
-    on every 64th row and 8th column."""
+def parity(scene, cam, frame: np.ndarray) -> dict:
+    """rel-MSE of the GPU frame against the FP64 oracle, and against the FP32
+    restatement, with the same seeds on every 16th row and 4th column."""
     import oracle
 
-    rows = np.arange(0, HEIGHT, 64, dtype=np.uint32)
-    ref = oracle.OracleScene(scene.view).render(cam, WIDTH, HEIGHT, 0, BOUNCES, SPP, precision=64, rows=rows,
-                                                col_step=8, threads=os.cpu_count() or 8) / SPP
-    gpu = frame[rows.astype(np.int64)][:, ::8, :3].astype(np.float64)
-    return float(np.mean((gpu - ref) ** 2 / (ref * ref + 0.01)))
+    rows = np.arange(0, HEIGHT, 16, dtype=np.uint32)
+    o = oracle.OracleScene(scene.view)
+    threads = min(os.cpu_count() or 8, 16)
+    gpu = frame[rows.astype(np.int64)][:, ::4, :3].astype(np.float64)
+    out = {"pixels": int(gpu.shape[0] * gpu.shape[1]), "sample": "rows 0,16,.. x columns 0,4,.. at the bench's seeds"}
+    for precision, key in ((64, "rel_mse_vs_fp64_oracle"), (32, "rel_mse_vs_fp32_restatement")):
+        ref = o.render(cam, WIDTH, HEIGHT, 0, BOUNCES, SPP, precision=precision, rows=rows, col_step=4,
+                       threads=threads) / SPP
+        out[key] = float(np.mean((gpu - ref) ** 2 / (ref * ref + 0.01)))
+    return out
+
+
+def rel_mse_vs_reference(frame: np.ndarray, ref: dict) -> float:
+    rows, cols = ref["rows"].astype(np.int64), ref["cols"].astype(np.int64)
+    y = 0.5 * (ref["half_a"] + ref["half_b"])
+    x = frame[rows][:, cols, :3].astype(np.float64)
+    return float(np.mean((x - y) ** 2 / (y * y + 0.01)))
+
+
+def equal_time(hx, tracer, cam, plain_frame: np.ndarray, plain_ms: float, warmup_frames: int) -> dict:
+    """Radiance cache vs plain at equal wall time, both scored against the
+    4096-spp FP64 oracle reference (tests/golden/c1_reference_4096spp.npz)."""
+    ref = np.load(REFERENCE)
+    floor = float(np.mean((ref["half_a"] - ref["half_b"]) ** 2 /
+                          ((0.5 * (ref["half_a"] + ref["half_b"])) ** 2 + 0.01))) / 4.0
+    out = {"reference": "FP64 oracle, 4096 spp, every 32nd row and column", "reference_noise_floor": floor,
+           "plain": {"spp": SPP, "ms": plain_ms, "rel_mse": rel_mse_vs_reference(plain_frame, ref)}}
+    tracer.set_radiance_cache(True, train_samples=65536, terminate_bounce=1, train_steps=4, learning_rate=1e-2, seed=1)
+    for i in range(warmup_frames):
+        tracer.render(cam, WIDTH, HEIGHT, 1000 + 4 * i, BOUNCES, 4, readback=False)
+    stats0 = tracer.rc_stats()
+    t0 = time.perf_counter()
+    tracer.render(cam, WIDTH, HEIGHT, 0, BOUNCES, 4, readback=False)
+    per_spp = (time.perf_counter() - t0) / 4
+    spp = max(1, int(plain_ms * 1e-3 / per_spp))
+    tracer.reset_kernel_stats()
+    tracer.set_instrumentation(1)
+    before = tracer.rc_stats()
+    t0 = time.perf_counter()
+    tracer.render(cam, WIDTH, HEIGHT, 0, BOUNCES, spp, readback=False)
+    cache_ms = (time.perf_counter() - t0) * 1e3
+    k = tracer.kernel_stats()
+    tracer.set_instrumentation(0)
+    frame = tracer.read_film(WIDTH, HEIGHT)
+    stats = tracer.rc_stats()
+    queries = stats["queries"] - before["queries"]
+    tracer.set_radiance_cache(False)
+    cache = {"spp": spp, "ms": cache_ms, "rel_mse": rel_mse_vs_reference(frame, ref), "warmup_frames": warmup_frames,
+             "train_loss": stats["last_loss"], "queries": int(queries),
+             "train_ms": stats["train_ms"] - before["train_ms"], "warmup_queries": int(before["queries"] - stats0["queries"])}
+    infer_ms = k["cache"]["ms"]
+    cache["mlp_infer_ms"] = infer_ms
+    if infer_ms > 0 and queries > 0:
+        tflops = queries * MLP_FLOP_PER_QUERY / (infer_ms * 1e-3) / 1e12
+        cache["mlp_mfma"] = {"bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": tflops / BF16_PEAK_TFLOPS, "flop_per_query": MLP_FLOP_PER_QUERY}
+    out["cache"] = cache
+    out["rel_mse_ratio_plain_over_cache"] = out["plain"]["rel_mse"] / max(cache["rel_mse"], 1e-30)
+    return out
 
 
 def main() -> int:
@@ -94,6 +151,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-time comparison")
+    ap.add_argument("--cache-warmup", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (every box has one GPU); "
@@ -186,9 +245,13 @@ def main() -> int:
         "traversal": roof,
         "roofline": roofline,
     }
+    if not args.no_cache:
+        log("radiance cache vs plain at equal time")
+        result["equal_time"] = equal_time(hx, tracer, cam, frame, ms_per_step, args.cache_warmup)
     if not args.no_parity:
         log("parity vs FP64 oracle (subset)")
-        result["parity_rel_mse_vs_fp64_oracle"] = parity(scene, cam, frame)
+        result["parity"] = parity(scene, cam, frame)
+        result["parity_rel_mse_vs_fp64_oracle"] = result["parity"]["rel_mse_vs_fp64_oracle"]
     if not args.no_cpu_baseline:
         log("cpu baseline")
         threads = min(os.cpu_count() or 1, 16)

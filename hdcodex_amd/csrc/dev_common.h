@@ -118,7 +118,7 @@ struct ChunkDesc {
     std::uint32_t sampleBase;        // sample number of s = 0
     std::uint32_t s0, s1, p0, p1;
     std::uint32_t maxBounces;
-    float cam[12];                   // origin, lowerLeft, horizontal, vertical
+    float cam[12];                   // origin, lowerLeft - origin, horizontal, vertical
 };
 
 struct TraversalCounters {

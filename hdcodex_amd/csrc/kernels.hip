@@ -68,9 +68,12 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc
         const float jx = randomFloat(rng), jy = randomFloat(rng);
         const float u = (static_cast<float>(px) + jx) / static_cast<float>(c.width);
         const float v = (static_cast<float>(py) + jy) / static_cast<float>(c.height);
+        // vulkan_path_tracer.cpp:1211-1213 with lowerLeft - origin taken
+        // from the host (rounded once from FP64): same direction, without
+        // the FP32 cancellation when |origin| is large.
         const V3 o = v3(c.cam[0], c.cam[1], c.cam[2]);
         const V3 d = normalize(v3(c.cam[3], c.cam[4], c.cam[5]) + v3(c.cam[6], c.cam[7], c.cam[8]) * u +
-                               v3(c.cam[9], c.cam[10], c.cam[11]) * v - o);
+                               v3(c.cam[9], c.cam[10], c.cam[11]) * v);
         pb.rayO[pid] = make_float4(o.x, o.y, o.z, 0.0f);
         pb.rayD[pid] = make_float4(d.x, d.y, d.z, 0.0f);
         pb.rng[pid] = rng;

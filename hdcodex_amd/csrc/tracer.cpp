@@ -585,7 +585,8 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     c.sampleBase = sampleIndex;
     c.maxBounces = bounces;
     std::memcpy(c.cam + 0, cam.origin, 12);
-    std::memcpy(c.cam + 3, cam.lower_left, 12);
+    for (int k = 0; k < 3; ++k)
+        c.cam[3 + k] = static_cast<float>(static_cast<double>(cam.lower_left[k]) - static_cast<double>(cam.origin[k]));
     std::memcpy(c.cam + 6, cam.horizontal, 12);
     std::memcpy(c.cam + 9, cam.vertical, 12);
     for (std::uint32_t s0 = 0; s0 < samples; s0 += samplesPerChunk) {

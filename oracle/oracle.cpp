@@ -1216,11 +1216,17 @@ struct Integrator {
         const R jx = Rand(rng), jy = Rand(rng);
         const R u = (R(px) + jx) / R(width), v = (R(py) + jy) / R(height);
         const V3<R> org(R(cam.origin[0]), R(cam.origin[1]), R(cam.origin[2]));
-        const V3<R> ll(R(cam.lower_left[0]), R(cam.lower_left[1]), R(cam.lower_left[2]));
         const V3<R> hz(R(cam.horizontal[0]), R(cam.horizontal[1]), R(cam.horizontal[2]));
         const V3<R> vt(R(cam.vertical[0]), R(cam.vertical[1]), R(cam.vertical[2]));
         V3<R> ro = org;
-        V3<R> rd = normalize(ll + hz * u + vt * v - ro);
+        // (:1211-1213) with lowerLeft - origin evaluated once in FP64 and
+        // rounded to R: the same direction without the FP32 cancellation
+        // that costs ~7 bits when |origin| is large against the image plane
+        // distance (the FP64 oracle is unchanged by this).
+        const V3<R> toLL(R(double(cam.lower_left[0]) - double(cam.origin[0])),
+                         R(double(cam.lower_left[1]) - double(cam.origin[1])),
+                         R(double(cam.lower_left[2]) - double(cam.origin[2])));
+        V3<R> rd = normalize(toLL + hz * u + vt * v);
         R throughput = R(1), radiance = R(0);
         bool envOnMiss = true, prevDelta = true;
         R prevBsdfPdf = R(0), prevEnvPdf = R(0);
