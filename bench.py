@@ -37,6 +37,22 @@ WIDTH, HEIGHT, SPP, BOUNCES = 1920, 1080, 16, 8
 GRID, DOME = 1024, 2048
 
 
+TRAFFIC = ROOT / "profiles" / "hbm_traffic_heightfield1024_1080p_16spp.md"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    try:
+        for line in TRAFFIC.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -144,6 +160,22 @@ def equal_time(hx, tracer, cam, plain_frame: np.ndarray, plain_ms: float, warmup
     return out
 
 
+def copy_calibration() -> int:
+    """A plain copy of known size: 256 MiB read + 256 MiB written per launch.
+    Its FETCH_SIZE / WRITE_SIZE readings give the scale that turns those
+    counters into HBM bytes for the path-tracing kernels."""
+    import torch
+
+    n = 256 << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n, "copy_calibration_bytes_written": n, "launches": 4}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,7 +185,11 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--cache-warmup", type=int, default=6)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy kernel 4 times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -217,7 +253,7 @@ def main() -> int:
     if rk in roof:
         r = roof[rk]
         roofline = {"kernel": rk, "bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": r["frac"], "traffic": None,
+                    "frac": r["frac"], "traffic": measured_traffic(rk),
                     "alg_bytes_per_launch": r["bytes_per_ray"] * r["rays_per_launch"],
                     "avg_launch_ms": r["avg_launch_ms"]}
 

@@ -94,7 +94,12 @@ class KernelStats(ctypes.Structure):
 
 
 class BvhInfo(ctypes.Structure):
-    _fields_ = [("node_count", c_uint32), ("slot_count", c_uint32), ("max_stack", c_uint32), ("depth", c_uint32)]
+    _fields_ = [("node_count", c_uint32), ("slot_count", c_uint32), ("max_stack", c_uint32), ("depth", c_uint32),
+                ("builder", ctypes.c_int32), ("top_clusters", c_uint32), ("refits", c_uint32),
+                ("reserved", c_uint32), ("build_ms", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
 class HdcxError(RuntimeError):
@@ -136,6 +141,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_get_kernel_stats.argtypes = [c_void_p, POINTER(KernelStats)]
     lib.hdcx_reset_kernel_stats.argtypes = [c_void_p]
     _bind_cache(lib)
+    _bind_bvh(lib)
     _lib = lib
     return lib
 
@@ -212,8 +218,7 @@ def build_bvh_host(scene: "ProceduralScene"):
     nodes = np.zeros(info.node_count, dtype=BVH_NODE_DTYPE)
     slots = np.zeros((info.slot_count, 3, 4), dtype=np.float32)
     _check(lib.hdcx_bvh_build_host(ctypes.byref(scene.view), ctypes.byref(info), nodes.ctypes.data, slots.ctypes.data))
-    return ({"node_count": info.node_count, "slot_count": info.slot_count, "max_stack": info.max_stack,
-             "depth": info.depth}, nodes, slots)
+    return info.as_dict(), nodes, slots
 
 
 def validate_scene(view: SceneView) -> None:
@@ -402,3 +407,47 @@ PathTracer.rc_stats = _tracer_rc_stats
 EXPORTED_SYMBOLS = EXPORTED_SYMBOLS + (
     "hdcx_set_radiance_cache", "hdcx_rc_get_params", "hdcx_rc_set_params", "hdcx_rc_infer", "hdcx_rc_gradient",
     "hdcx_rc_train", "hdcx_rc_get_stats")
+
+
+# ---- BVH builders and the tracer's BVH policy ----------------------------
+
+BVH_HOST_SAH, BVH_DEVICE_LBVH, BVH_SERIAL_LBVH = 0, 1, 2
+
+
+class BvhOptions(ctypes.Structure):
+    _fields_ = [("builder", ctypes.c_int32), ("top_clusters", c_uint32), ("refit", c_uint32), ("reserved", c_uint32)]
+
+
+def _bind_bvh(lib: ctypes.CDLL) -> None:
+    lib.hdcx_bvh_build.argtypes = [POINTER(SceneView), c_int, c_uint32, POINTER(BvhInfo), c_void_p, c_void_p]
+    lib.hdcx_set_bvh_options.argtypes = [c_void_p, POINTER(BvhOptions)]
+    lib.hdcx_get_bvh_info.argtypes = [c_void_p, POINTER(BvhInfo)]
+
+
+def build_bvh(scene: "ProceduralScene", builder: int = BVH_HOST_SAH, top_clusters: int = 2048):
+    """BVH4 of a scene from the chosen builder, validated on the host:
+    (info dict, nodes, slots[(n, 3, 4) float32]). BVH_DEVICE_LBVH needs a GPU."""
+    lib = library()
+    info = BvhInfo()
+    _check(lib.hdcx_bvh_build(ctypes.byref(scene.view), builder, top_clusters, ctypes.byref(info), None, None))
+    nodes = np.zeros(info.node_count, dtype=BVH_NODE_DTYPE)
+    slots = np.zeros((info.slot_count, 3, 4), dtype=np.float32)
+    _check(lib.hdcx_bvh_build(ctypes.byref(scene.view), builder, top_clusters, ctypes.byref(info), nodes.ctypes.data,
+                              slots.ctypes.data))
+    return info.as_dict(), nodes, slots
+
+
+def _tracer_set_bvh_options(self, builder: int = BVH_DEVICE_LBVH, top_clusters: int = 2048, refit: bool = True) -> None:
+    o = BvhOptions(builder, top_clusters, 1 if refit else 0, 0)
+    _check(library().hdcx_set_bvh_options(self._h, ctypes.byref(o)))
+
+
+def _tracer_bvh_info(self) -> dict:
+    info = BvhInfo()
+    _check(library().hdcx_get_bvh_info(self._h, ctypes.byref(info)))
+    return info.as_dict()
+
+
+PathTracer.set_bvh_options = _tracer_set_bvh_options
+PathTracer.bvh_info = _tracer_bvh_info
+EXPORTED_SYMBOLS = EXPORTED_SYMBOLS + ("hdcx_bvh_build", "hdcx_set_bvh_options", "hdcx_get_bvh_info")

@@ -40,8 +40,8 @@ HIP_FLAGS = [
     "-I", str(ROOT / "include"),
 ]
 
-HOST_SOURCES = ["packer.cpp", "procedural.cpp", "procedural_showcase.cpp", "bvh_host.cpp"]
-HIP_SOURCES = ["kernels.hip", "rcache.hip", "tracer.cpp"]
+HOST_SOURCES = ["packer.cpp", "procedural.cpp", "procedural_showcase.cpp", "bvh_host.cpp", "lbvh_host.cpp"]
+HIP_SOURCES = ["kernels.hip", "rcache.hip", "lbvh.hip", "tracer.cpp"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -57,7 +57,8 @@ def build_library() -> Path:
     for src in HOST_SOURCES:
         obj = BUILD / (src + ".o")
         objects.append(obj)
-        jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", str(ROOT / "include"), "-c",
+                     str(CSRC / src), "-o", str(obj)])
     for src in HIP_SOURCES:
         obj = BUILD / (src + ".o")
         objects.append(obj)

@@ -28,7 +28,7 @@ constexpr std::uint32_t kLeafFlag = 0x80000000U;
 constexpr std::uint32_t kEmptyChild = 0xffffffffU;
 constexpr std::uint32_t kMaxLeafTriangles = 8U;
 
-inline std::uint32_t EncodeLeaf(std::uint32_t first, std::uint32_t count)
+constexpr std::uint32_t EncodeLeaf(std::uint32_t first, std::uint32_t count)
 {
     return kLeafFlag | (first << 3) | (count - 1U);
 }

@@ -1,0 +1,243 @@
+// Host parts of the device BVH builder; see lbvh_host.hpp.
+#include "lbvh_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+
+namespace hdcx {
+namespace {
+
+using lbvh::Box4;
+using lbvh::FrontierItem;
+
+double Area(const float lo[3], const float hi[3])
+{
+    const double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+void Grow(float lo[3], float hi[3], const float blo[3], const float bhi[3])
+{
+    for (int x = 0; x < 3; ++x) {
+        lo[x] = std::min(lo[x], blo[x]);
+        hi[x] = std::max(hi[x], bhi[x]);
+    }
+}
+
+float Centre(const FrontierItem& f, int axis) { return 0.5F * (f.box.lo[axis] + f.box.hi[axis]); }
+
+struct SerialAtomics {
+    static std::uint32_t Arrive(std::uint32_t* c) { return (*c)++; }
+    static std::uint32_t Append(std::uint32_t* c) { return (*c)++; }
+    static void Max(std::uint32_t* c, std::uint32_t v) { *c = std::max(*c, v); }
+    static void Flag(std::uint32_t* c) { *c = 1U; }
+};
+
+} // namespace
+
+std::uint32_t BuildSahTop(std::vector<FrontierItem>& items, std::uint32_t firstId, std::vector<std::uint32_t>& child2,
+                          std::vector<std::uint32_t>& range2, std::vector<Box4>& box)
+{
+    constexpr int kBins = 32;
+    if (items.size() < 2) throw std::invalid_argument("SAH top needs two frontier items");
+    child2.clear();
+    range2.clear();
+    box.clear();
+    // Work items: [b, e) of `items` and the (node, side) slot its reference
+    // goes into; node == kNone for the root.
+    struct Work { std::uint32_t b, e, node, side; };
+    std::vector<Work> work{{0, static_cast<std::uint32_t>(items.size()), lbvh::kNone, 0}};
+    std::uint32_t root = lbvh::kNone;
+    while (!work.empty()) {
+        const Work w = work.back();
+        work.pop_back();
+        std::uint32_t ref;
+        if (w.e - w.b == 1) {
+            ref = items[w.b].ref;
+        } else {
+            float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (std::uint32_t i = w.b; i < w.e; ++i) {
+                Grow(lo, hi, items[i].box.lo, items[i].box.hi);
+                for (int a = 0; a < 3; ++a) {
+                    clo[a] = std::min(clo[a], Centre(items[i], a));
+                    chi[a] = std::max(chi[a], Centre(items[i], a));
+                }
+            }
+            double best = std::numeric_limits<double>::infinity();
+            int bestAxis = -1, bestBin = 0;
+            for (int a = 0; a < 3; ++a) {
+                const float ext = chi[a] - clo[a];
+                if (!(ext > 0.0F)) continue;
+                float blo[kBins][3], bhi[kBins][3];
+                double cnt[kBins] = {};
+                for (int k = 0; k < kBins; ++k)
+                    for (int x = 0; x < 3; ++x) {
+                        blo[k][x] = INFINITY;
+                        bhi[k][x] = -INFINITY;
+                    }
+                const float scale = kBins / ext;
+                for (std::uint32_t i = w.b; i < w.e; ++i) {
+                    const int k = std::clamp(static_cast<int>((Centre(items[i], a) - clo[a]) * scale), 0, kBins - 1);
+                    Grow(blo[k], bhi[k], items[i].box.lo, items[i].box.hi);
+                    cnt[k] += items[i].size;
+                }
+                double rightArea[kBins], rightCnt[kBins];
+                float alo[3] = {INFINITY, INFINITY, INFINITY}, ahi[3] = {-INFINITY, -INFINITY, -INFINITY};
+                double acc = 0.0;
+                for (int k = kBins - 1; k > 0; --k) {
+                    Grow(alo, ahi, blo[k], bhi[k]);
+                    acc += cnt[k];
+                    rightArea[k] = acc > 0.0 ? Area(alo, ahi) : 0.0;
+                    rightCnt[k] = acc;
+                }
+                for (int x = 0; x < 3; ++x) {
+                    alo[x] = INFINITY;
+                    ahi[x] = -INFINITY;
+                }
+                acc = 0.0;
+                for (int k = 0; k < kBins - 1; ++k) {
+                    Grow(alo, ahi, blo[k], bhi[k]);
+                    acc += cnt[k];
+                    if (acc == 0.0 || rightCnt[k + 1] == 0.0) continue;
+                    const double cost = Area(alo, ahi) * acc + rightArea[k + 1] * rightCnt[k + 1];
+                    if (cost < best) {
+                        best = cost;
+                        bestAxis = a;
+                        bestBin = k + 1;
+                    }
+                }
+            }
+            std::uint32_t mid = w.b + (w.e - w.b) / 2;  // coincident centres: split by count
+            if (bestAxis >= 0) {
+                const int a = bestAxis;
+                const float scale = kBins / (chi[a] - clo[a]);
+                auto m = std::partition(items.begin() + w.b, items.begin() + w.e, [&](const FrontierItem& f) {
+                    return std::clamp(static_cast<int>((Centre(f, a) - clo[a]) * scale), 0, kBins - 1) < bestBin;
+                });
+                const auto split = static_cast<std::uint32_t>(m - items.begin());
+                if (split > w.b && split < w.e) mid = split;
+            }
+            const auto self = static_cast<std::uint32_t>(box.size());
+            child2.push_back(lbvh::kNone);
+            child2.push_back(lbvh::kNone);
+            range2.push_back(0U);
+            range2.push_back(lbvh::kTopRangeHi);
+            Box4 b{};
+            for (int x = 0; x < 3; ++x) {
+                b.lo[x] = lo[x];
+                b.hi[x] = hi[x];
+            }
+            box.push_back(b);
+            ref = firstId + self;
+            work.push_back({mid, w.e, self, 1});
+            work.push_back({w.b, mid, self, 0});
+        }
+        if (w.node == lbvh::kNone) root = ref;
+        else child2[2 * w.node + w.side] = ref;
+    }
+    return root;
+}
+
+void VertexBounds(const float* positions, std::uint32_t vertexCount, float lo[3], float hi[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = INFINITY;
+        hi[k] = -INFINITY;
+    }
+    for (std::uint32_t i = 0; i < vertexCount; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], positions[4 * static_cast<std::size_t>(i) + k]);
+            hi[k] = std::max(hi[k], positions[4 * static_cast<std::size_t>(i) + k]);
+        }
+}
+
+Bvh4 BuildBvh4LbvhSerial(const float* positions, const std::uint32_t* indices, std::uint32_t n,
+                         std::uint32_t targetClusters, LbvhBuildInfo* info)
+{
+    if (n < 2 || n > (1U << 28)) throw std::invalid_argument("LBVH needs 2 .. 2^28 triangles");
+    std::uint32_t maxIndex = 0;
+    for (std::size_t i = 0; i < 3 * static_cast<std::size_t>(n); ++i) maxIndex = std::max(maxIndex, indices[i]);
+    float lo[3], hi[3];
+    VertexBounds(positions, maxIndex + 1, lo, hi);
+    float scale[3];
+    for (int k = 0; k < 3; ++k) scale[k] = hi[k] > lo[k] ? 1024.0F / (hi[k] - lo[k]) : 0.0F;
+
+    std::vector<std::uint32_t> morton(n), ids(n);
+    std::vector<Box4> triBox(n);
+    for (std::uint32_t i = 0; i < n; ++i) {
+        morton[i] = lbvh::PrepareTriangle(positions, indices, i, lo, scale, triBox[i]);
+        ids[i] = i;
+    }
+    // The device sort is a stable LSD radix sort: equal codes keep id order.
+    std::stable_sort(ids.begin(), ids.end(), [&](std::uint32_t a, std::uint32_t b) { return morton[a] < morton[b]; });
+    std::vector<std::uint32_t> keys(n);
+    for (std::uint32_t k = 0; k < n; ++k) keys[k] = morton[ids[k]];
+
+    const std::uint32_t S = FrontierThreshold(n, targetClusters);
+    const std::uint32_t cap = FrontierCapacity(n, S);
+    const std::size_t inner = n - 1 + static_cast<std::size_t>(cap);
+    std::vector<std::uint32_t> child2(2 * inner, lbvh::kNone), range2(2 * inner, 0U), parent(n, lbvh::kNone),
+        leafParent(n, lbvh::kNone), arrivals(n, 0U);
+    std::vector<Box4> nodeBox(inner);
+    for (std::uint32_t i = 0; i + 1 < n; ++i)
+        lbvh::RadixTreeNode(keys.data(), static_cast<int>(n), static_cast<int>(i), child2.data(), parent.data(),
+                            leafParent.data(), range2.data());
+    parent[0] = lbvh::kNone;
+    for (std::uint32_t k = 0; k < n; ++k)
+        lbvh::BottomUpLeaf<SerialAtomics>(k, child2.data(), parent.data(), leafParent.data(), arrivals.data(),
+                                          ids.data(), triBox.data(), nodeBox.data());
+
+    std::uint32_t root = 0;
+    if (info) *info = LbvhBuildInfo{};
+    if (S && n > S) {
+        std::vector<FrontierItem> items(cap);
+        std::uint32_t count = 0;
+        for (std::uint32_t i = 0; i + 1 < n; ++i)
+            lbvh::FrontierNode<SerialAtomics>(i, S, child2.data(), range2.data(), nodeBox.data(), triBox.data(),
+                                              ids.data(), &count, cap, items.data());
+        if (count >= 2 && count <= cap) {
+            items.resize(count);
+            std::sort(items.begin(), items.end(), [](const FrontierItem& a, const FrontierItem& b) { return a.ref < b.ref; });
+            std::vector<std::uint32_t> tc, tr;
+            std::vector<Box4> tb;
+            root = BuildSahTop(items, n - 1, tc, tr, tb);
+            std::copy(tc.begin(), tc.end(), child2.begin() + 2 * (static_cast<std::size_t>(n) - 1));
+            std::copy(tr.begin(), tr.end(), range2.begin() + 2 * (static_cast<std::size_t>(n) - 1));
+            std::copy(tb.begin(), tb.end(), nodeBox.begin() + (static_cast<std::size_t>(n) - 1));
+            if (info) info->topClusters = count;
+        }
+    }
+
+    const std::uint32_t nodeCap = n - 1;
+    Bvh4 out;
+    out.nodes.resize(nodeCap);
+    std::vector<std::uint32_t> source(nodeCap, 0U), nodeStack(nodeCap, 0U);
+    std::uint32_t counters[4] = {1U, 0U, 0U, 0U};
+    source[0] = root;
+    std::vector<std::uint32_t> levels{0U};
+    std::uint32_t begin = 0, end = 1, level = 0;
+    while (begin < end) {
+        for (std::uint32_t id = begin; id < end; ++id)
+            lbvh::CollapseNode<SerialAtomics>(id, level, source.data(), child2.data(), range2.data(), nodeBox.data(),
+                                              triBox.data(), ids.data(), nodeStack.data(), counters, nodeCap,
+                                              out.nodes.data());
+        if (counters[3]) throw std::runtime_error("LBVH collapse overflowed its node pool");
+        levels.push_back(end);
+        begin = end;
+        end = counters[0];
+        ++level;
+    }
+    out.nodes.resize(counters[0]);
+    out.maxStack = counters[1];
+    out.depth = counters[2];
+    out.triangles.resize(12 * static_cast<std::size_t>(n));
+    for (std::uint32_t k = 0; k < n; ++k) lbvh::WriteSlot(positions, indices, ids[k], out.triangles.data() + 12 * static_cast<std::size_t>(k));
+    if (info) info->levelBegin = levels;
+    return out;
+}
+
+} // namespace hdcx

@@ -1,0 +1,45 @@
+// Host parts of the device BVH builder: the binned-SAH top levels built
+// over the frontier subtrees, and a serial CPU run of every device step
+// (same per-element code, lbvh_core.h) used to check the builder without a
+// GPU.
+#pragma once
+
+#include "bvh.hpp"
+#include "lbvh_core.h"
+
+#include <cstdint>
+#include <vector>
+
+namespace hdcx {
+
+// Binned SAH (32 bins, triangle-count weighted) binary tree over the
+// frontier items. Appends its inner nodes as BVH2 node ids firstId,
+// firstId + 1, ... (child2 / range2: two words per node, box: one Box4 per
+// node) and returns the root reference. items is reordered. count >= 2.
+std::uint32_t BuildSahTop(std::vector<lbvh::FrontierItem>& items, std::uint32_t firstId,
+                          std::vector<std::uint32_t>& child2, std::vector<std::uint32_t>& range2,
+                          std::vector<lbvh::Box4>& box);
+
+struct LbvhBuildInfo {
+    std::uint32_t topClusters = 0;
+    std::vector<std::uint32_t> levelBegin;  // BVH4 ids of level l: [levelBegin[l], levelBegin[l + 1])
+};
+
+// The device builder's steps run one element at a time on the host.
+// positions: float4 per vertex; indices: 3 per triangle; n >= 2.
+Bvh4 BuildBvh4LbvhSerial(const float* positions, const std::uint32_t* indices, std::uint32_t n,
+                         std::uint32_t targetClusters, LbvhBuildInfo* info = nullptr);
+
+// Morton quantisation bounds of a vertex set (the same on both paths).
+void VertexBounds(const float* positions, std::uint32_t vertexCount, float lo[3], float hi[3]);
+
+// Frontier size threshold and capacity for n triangles and a target count.
+inline std::uint32_t FrontierThreshold(std::uint32_t n, std::uint32_t targetClusters)
+{
+    if (targetClusters == 0) return 0;
+    const std::uint32_t s = (n + targetClusters - 1) / targetClusters;
+    return s < 8U ? 8U : s;
+}
+inline std::uint32_t FrontierCapacity(std::uint32_t n, std::uint32_t S) { return S ? 4U * (n / S) + 8U : 0U; }
+
+} // namespace hdcx

@@ -4,7 +4,10 @@
 // a status code and a thread-local message.
 #include "../../include/hdcx.h"
 #include "dev_traverse.h"
+#include "device_buffer.h"
 #include "kernel_api.h"
+#include "lbvh.h"
+#include "lbvh_host.hpp"
 #include "packer.hpp"
 #include "procedural.hpp"
 #include "rcache.h"
@@ -12,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -24,19 +28,6 @@ namespace hdcx {
 namespace {
 
 thread_local std::string gLastError;
-
-struct Failure : std::runtime_error {
-    int code;
-    Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void Check(hipError_t e, const char* what)
-{
-    if (e == hipSuccess) return;
-    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
-        throw Failure(HDCX_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
-    throw Failure(HDCX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
 
 template <class F>
 int Guard(F&& f)
@@ -59,40 +50,6 @@ int Guard(F&& f)
         return HDCX_ERR_INTERNAL;
     }
 }
-
-// Owning device allocation.
-class DeviceBuffer {
-public:
-    DeviceBuffer() = default;
-    DeviceBuffer(const DeviceBuffer&) = delete;
-    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    ~DeviceBuffer() { Reset(); }
-    void Reset()
-    {
-        if (ptr_) (void)hipFree(ptr_);
-        ptr_ = nullptr;
-        bytes_ = 0;
-    }
-    void Allocate(std::size_t bytes)
-    {
-        if (bytes == bytes_ && ptr_) return;
-        Reset();
-        if (bytes == 0) return;
-        Check(hipMalloc(&ptr_, bytes), "hipMalloc");
-        bytes_ = bytes;
-    }
-    void Upload(const void* host, std::size_t bytes, hipStream_t st)
-    {
-        Allocate(bytes);
-        if (bytes) Check(hipMemcpyAsync(ptr_, host, bytes, hipMemcpyHostToDevice, st), "upload");
-    }
-    template <class T> T* as() const { return static_cast<T*>(ptr_); }
-    [[nodiscard]] std::size_t bytes() const { return bytes_; }
-
-private:
-    void* ptr_ = nullptr;
-    std::size_t bytes_ = 0;
-};
 
 constexpr std::uint32_t kChunkPixelSamples = 1U << 21;  // 6.3M lane paths per chunk
 
@@ -118,7 +75,14 @@ struct hdcx_tracer {
     // Scene storage.
     DeviceBuffer nodes, tris, positions, indices, triMaterial, texcoords, normals, emissive, materials, lights,
         textures, texels, importance, importancePool;
-    Bvh4 hostBvhInfo;  // depth / stack statistics of the uploaded BVH
+    // BVH: builder policy, the device builder's workspace, refit order and
+    // the triangles (vertex count, index buffer) the BVH was built over.
+    hdcx_bvh_options bvhOptions{HDCX_BVH_DEVICE_LBVH, 2048U, 1U, 0U};
+    hdcx_bvh_info bvhInfo{};
+    DeviceBvhBuilder lbvh;
+    RefitPlan refitPlan;
+    std::vector<std::uint32_t> builtIndices;
+    std::uint32_t builtVertexCount = 0;
     // Path state.
     DeviceBuffer rayO, rayD, hit, rng, wavelength, throughput, radiance, flags, mis, medium, shadowO, shadowD,
         shadowLight, queue0, queue1, shadowQueue, counters, totals, traversalCounters, lambda0;
@@ -397,35 +361,97 @@ void ValidateScene(const hdcx_scene_view& v)
     }
 }
 
+void UploadHostBvh(hdcx_tracer* t, const Bvh4& bvh, int builder)
+{
+    if (bvh.maxStack > static_cast<std::uint32_t>(kMaxStack))
+        throw Failure(HDCX_ERR_INVALID_SCENE, "BVH deeper than the traversal stack");
+    t->nodes.Upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), t->stream);
+    t->tris.Upload(bvh.triangles.data(), bvh.triangles.size() * sizeof(float), t->stream);
+    MakeRefitPlan(bvh.nodes, t->stream, t->refitPlan);
+    t->bvhInfo = hdcx_bvh_info{};
+    t->bvhInfo.node_count = static_cast<std::uint32_t>(bvh.nodes.size());
+    t->bvhInfo.slot_count = static_cast<std::uint32_t>(bvh.triangles.size() / 12);
+    t->bvhInfo.max_stack = bvh.maxStack;
+    t->bvhInfo.depth = bvh.depth;
+    t->bvhInfo.builder = builder;
+}
+
+// Builds the BVH of the uploaded positions / indices with the configured
+// builder. The device builder falls back to the host one for a single
+// triangle or a tree deeper than the traversal stack.
+void BuildSceneBvh(hdcx_tracer* t, const hdcx_scene_view& v, const float lo[3], const float hi[3])
+{
+    const std::uint32_t n = v.triangle_count;
+    int builder = t->bvhOptions.builder;
+    if (builder != HDCX_BVH_HOST_SAH && n < 2) builder = HDCX_BVH_HOST_SAH;
+    if (builder == HDCX_BVH_DEVICE_LBVH) {
+        DeviceBvhStats st;
+        t->lbvh.Build(t->positions.as<float4>(), t->indices.as<std::uint32_t>(), n, lo, hi, t->bvhOptions.top_clusters,
+                      t->stream, t->nodes, t->tris, t->refitPlan, st);
+        if (st.maxStack <= static_cast<std::uint32_t>(kMaxStack)) {
+            t->bvhInfo = hdcx_bvh_info{};
+            t->bvhInfo.node_count = st.nodeCount;
+            t->bvhInfo.slot_count = n;
+            t->bvhInfo.max_stack = st.maxStack;
+            t->bvhInfo.depth = st.depth;
+            t->bvhInfo.builder = HDCX_BVH_DEVICE_LBVH;
+            t->bvhInfo.top_clusters = st.topClusters;
+            return;
+        }
+        builder = HDCX_BVH_HOST_SAH;
+    }
+    if (builder == HDCX_BVH_SERIAL_LBVH) {
+        LbvhBuildInfo info;
+        Bvh4 bvh = BuildBvh4LbvhSerial(v.positions, v.indices, n, t->bvhOptions.top_clusters, &info);
+        ValidateBvh4(bvh, n);
+        if (bvh.maxStack <= static_cast<std::uint32_t>(kMaxStack)) {
+            UploadHostBvh(t, bvh, HDCX_BVH_SERIAL_LBVH);
+            t->bvhInfo.top_clusters = info.topClusters;
+            return;
+        }
+    }
+    Bvh4 bvh = BuildBvh4Host(v.positions, v.indices, n);
+    ValidateBvh4(bvh, n);
+    UploadHostBvh(t, bvh, HDCX_BVH_HOST_SAH);
+}
+
 void SetScene(hdcx_tracer* t, const hdcx_scene_view* view)
 {
     t->Activate();
+    const bool hadGeometry = t->geometry;
     t->geometry = false;
-    if (!view || view->vertex_count == 0 || view->triangle_count == 0) return;
+    if (!view || view->vertex_count == 0 || view->triangle_count == 0) {
+        t->builtIndices.clear();
+        t->builtVertexCount = 0;
+        return;
+    }
     ValidateScene(*view);
     const hdcx_scene_view& v = *view;
     hipStream_t st = t->stream;
 
-    {
-        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (std::uint32_t i = 0; i < v.vertex_count; ++i)
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = std::min(lo[k], v.positions[i * 4 + k]);
-                hi[k] = std::max(hi[k], v.positions[i * 4 + k]);
-            }
-        for (int k = 0; k < 3; ++k) {
-            t->sceneLo[k] = lo[k];
-            t->sceneInvExtent[k] = hi[k] > lo[k] ? 1.0F / (hi[k] - lo[k]) : 1.0F;
-        }
+    float lo[3], hi[3];
+    VertexBounds(v.positions, v.vertex_count, lo, hi);
+    for (int k = 0; k < 3; ++k) {
+        t->sceneLo[k] = lo[k];
+        t->sceneInvExtent[k] = hi[k] > lo[k] ? 1.0F / (hi[k] - lo[k]) : 1.0F;
     }
-    Bvh4 bvh = BuildBvh4Host(v.positions, v.indices, v.triangle_count);
-    ValidateBvh4(bvh, v.triangle_count);
-    if (bvh.maxStack > static_cast<std::uint32_t>(kMaxStack))
-        throw Failure(HDCX_ERR_INVALID_SCENE, "BVH deeper than the traversal stack");
-    t->nodes.Upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), st);
-    t->tris.Upload(bvh.triangles.data(), bvh.triangles.size() * sizeof(float), st);
+    const bool refit = t->bvhOptions.refit != 0U && hadGeometry && t->builtVertexCount == v.vertex_count &&
+                       t->builtIndices.size() == static_cast<std::size_t>(v.triangle_count) * 3 &&
+                       std::memcmp(t->builtIndices.data(), v.indices, t->builtIndices.size() * 4) == 0;
     t->positions.Upload(v.positions, static_cast<std::size_t>(v.vertex_count) * 16, st);
     t->indices.Upload(v.indices, static_cast<std::size_t>(v.triangle_count) * 12, st);
+    const auto buildStart = std::chrono::steady_clock::now();
+    if (refit) {
+        RefitBvh4(t->positions.as<float4>(), t->indices.as<std::uint32_t>(), t->bvhInfo.slot_count,
+                  t->nodes.as<Bvh4Node>(), t->tris.as<float4>(), t->refitPlan, st);
+        ++t->bvhInfo.refits;
+    } else {
+        BuildSceneBvh(t, v, lo, hi);
+        t->builtIndices.assign(v.indices, v.indices + static_cast<std::size_t>(v.triangle_count) * 3);
+        t->builtVertexCount = v.vertex_count;
+    }
+    t->bvhInfo.build_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - buildStart).count();
     t->triMaterial.Upload(v.triangle_materials, static_cast<std::size_t>(v.triangle_count) * 4, st);
     t->texcoords.Upload(v.texcoords, static_cast<std::size_t>(v.triangle_count) * 24, st);
     t->normals.Upload(v.normals, static_cast<std::size_t>(v.triangle_count) * 48, st);
@@ -498,13 +524,11 @@ void SetScene(hdcx_tracer* t, const hdcx_scene_view* view)
     s.opaque = v.opaque ? 1U : 0U;
     s.hasEmissive = v.emissive[(static_cast<std::size_t>(v.triangle_count) - 1) * 4] > 0.0F ? 1U : 0U;
     s.textureCount = v.texture_count;
-    s.nodeCount = static_cast<std::uint32_t>(bvh.nodes.size());
-    t->hostBvhInfo.maxStack = bvh.maxStack;
-    t->hostBvhInfo.depth = bvh.depth;
+    s.nodeCount = t->bvhInfo.node_count;
     t->stats.node_bytes = sizeof(Bvh4Node);
     t->stats.triangle_bytes = 48;
-    t->stats.bvh_nodes = bvh.nodes.size();
-    t->stats.bvh_triangles = bvh.triangles.size() / 12;
+    t->stats.bvh_nodes = t->bvhInfo.node_count;
+    t->stats.bvh_triangles = t->bvhInfo.slot_count;
     t->geometry = true;
 }
 
@@ -674,6 +698,50 @@ void Query(hdcx_tracer* t, const hdcx_ray* rays, std::uint32_t count, void* out,
 } // namespace
 } // namespace hdcx
 
+namespace {
+
+void CopyBvhOut(const Bvh4& bvh, hdcx_bvh_info* out_info, void* out_nodes, float* out_slots)
+{
+    out_info->node_count = static_cast<std::uint32_t>(bvh.nodes.size());
+    out_info->slot_count = static_cast<std::uint32_t>(bvh.triangles.size() / 12);
+    out_info->max_stack = bvh.maxStack;
+    out_info->depth = bvh.depth;
+    if (out_nodes) std::memcpy(out_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node));
+    if (out_slots) std::memcpy(out_slots, bvh.triangles.data(), bvh.triangles.size() * sizeof(float));
+}
+
+Bvh4 BuildOnDevice(const hdcx_scene_view& v, std::uint32_t topClusters, std::uint32_t& clusters)
+{
+    struct Stream {
+        hipStream_t s = nullptr;
+        ~Stream() { if (s) (void)hipStreamDestroy(s); }
+    } stream;
+    Check(hipStreamCreate(&stream.s), "hipStreamCreate");
+    DeviceBuffer pos, idx, nodes, slots;
+    pos.Upload(v.positions, static_cast<std::size_t>(v.vertex_count) * 16, stream.s);
+    idx.Upload(v.indices, static_cast<std::size_t>(v.triangle_count) * 12, stream.s);
+    float lo[3], hi[3];
+    VertexBounds(v.positions, v.vertex_count, lo, hi);
+    DeviceBvhBuilder builder;
+    RefitPlan plan;
+    DeviceBvhStats st;
+    builder.Build(pos.as<float4>(), idx.as<std::uint32_t>(), v.triangle_count, lo, hi, topClusters, stream.s, nodes,
+                  slots, plan, st);
+    Bvh4 bvh;
+    bvh.nodes.resize(st.nodeCount);
+    bvh.triangles.resize(static_cast<std::size_t>(v.triangle_count) * 12);
+    Check(hipMemcpy(bvh.nodes.data(), nodes.as<void>(), bvh.nodes.size() * sizeof(Bvh4Node), hipMemcpyDeviceToHost),
+          "BVH download");
+    Check(hipMemcpy(bvh.triangles.data(), slots.as<void>(), bvh.triangles.size() * sizeof(float), hipMemcpyDeviceToHost),
+          "BVH download");
+    bvh.maxStack = st.maxStack;
+    bvh.depth = st.depth;
+    clusters = st.topClusters;
+    return bvh;
+}
+
+} // namespace
+
 extern "C" {
 
 const char* hdcx_last_error(void) { return hdcx::gLastError.c_str(); }
@@ -826,17 +894,62 @@ int hdcx_validate_scene(const hdcx_scene_view* scene)
 
 int hdcx_bvh_build_host(const hdcx_scene_view* scene, hdcx_bvh_info* out_info, void* out_nodes, float* out_slots)
 {
+    return hdcx_bvh_build(scene, HDCX_BVH_HOST_SAH, 0U, out_info, out_nodes, out_slots);
+}
+
+int hdcx_bvh_build(const hdcx_scene_view* scene, int builder, uint32_t top_clusters, hdcx_bvh_info* out_info,
+                   void* out_nodes, float* out_slots)
+{
     return Guard([&] {
         if (!scene || !out_info) throw std::invalid_argument("null argument");
         if (scene->triangle_count && (!scene->positions || !scene->indices)) throw std::invalid_argument("no geometry");
-        Bvh4 bvh = BuildBvh4Host(scene->positions, scene->indices, scene->triangle_count);
+        if (builder != HDCX_BVH_HOST_SAH && builder != HDCX_BVH_DEVICE_LBVH && builder != HDCX_BVH_SERIAL_LBVH)
+            throw std::invalid_argument("unknown BVH builder");
+        if (builder != HDCX_BVH_HOST_SAH && scene->triangle_count < 2)
+            throw std::invalid_argument("the LBVH builders need at least two triangles");
+        for (std::size_t i = 0; i < static_cast<std::size_t>(scene->triangle_count) * 3; ++i)
+            if (scene->indices[i] >= scene->vertex_count) throw std::invalid_argument("index out of range");
+        const auto t0 = std::chrono::steady_clock::now();
+        std::uint32_t clusters = 0;
+        Bvh4 bvh;
+        if (builder == HDCX_BVH_HOST_SAH) {
+            bvh = BuildBvh4Host(scene->positions, scene->indices, scene->triangle_count);
+        } else if (builder == HDCX_BVH_SERIAL_LBVH) {
+            LbvhBuildInfo info;
+            bvh = BuildBvh4LbvhSerial(scene->positions, scene->indices, scene->triangle_count, top_clusters, &info);
+            clusters = info.topClusters;
+        } else {
+            bvh = BuildOnDevice(*scene, top_clusters, clusters);
+        }
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         ValidateBvh4(bvh, scene->triangle_count);
-        out_info->node_count = static_cast<std::uint32_t>(bvh.nodes.size());
-        out_info->slot_count = static_cast<std::uint32_t>(bvh.triangles.size() / 12);
-        out_info->max_stack = bvh.maxStack;
-        out_info->depth = bvh.depth;
-        if (out_nodes) std::memcpy(out_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node));
-        if (out_slots) std::memcpy(out_slots, bvh.triangles.data(), bvh.triangles.size() * sizeof(float));
+        *out_info = hdcx_bvh_info{};
+        CopyBvhOut(bvh, out_info, out_nodes, out_slots);
+        out_info->builder = builder;
+        out_info->top_clusters = clusters;
+        out_info->build_ms = ms;
+    });
+}
+
+int hdcx_set_bvh_options(hdcx_tracer* tracer, const hdcx_bvh_options* options)
+{
+    return Guard([&] {
+        if (!tracer || !options) throw std::invalid_argument("null argument");
+        if (options->builder != HDCX_BVH_HOST_SAH && options->builder != HDCX_BVH_DEVICE_LBVH &&
+            options->builder != HDCX_BVH_SERIAL_LBVH)
+            throw std::invalid_argument("unknown BVH builder");
+        tracer->bvhOptions = *options;
+        // The next hdcx_set_scene builds with the new options.
+        tracer->builtIndices.clear();
+        tracer->builtVertexCount = 0;
+    });
+}
+
+int hdcx_get_bvh_info(hdcx_tracer* tracer, hdcx_bvh_info* out_info)
+{
+    return Guard([&] {
+        if (!tracer || !out_info) throw std::invalid_argument("null argument");
+        *out_info = tracer->geometry ? tracer->bvhInfo : hdcx_bvh_info{};
     });
 }
 

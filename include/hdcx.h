@@ -190,13 +190,26 @@ int hdcx_scene_destroy(hdcx_scene* scene);
 /* ---- BVH (no counterpart in the reference: its acceleration structure is
  * built inside the Vulkan driver, :3032-3125). ---- */
 
+/* BVH builders. The reference leaves acceleration-structure builds to the
+ * Vulkan driver (src/gpu/vulkan_path_tracer.cpp:3032-3125, PREFER_FAST_TRACE
+ * BLAS + TLAS); all three produce the same four-wide layout, so traversal
+ * and hit results do not depend on the choice. */
+#define HDCX_BVH_HOST_SAH 0    /* binned SAH (16 bins) on the host */
+#define HDCX_BVH_DEVICE_LBVH 1 /* on the GPU: Morton radix tree under binned-SAH top levels */
+#define HDCX_BVH_SERIAL_LBVH 2 /* the device builder's steps run serially on the host */
+
 /* Layout summary of a four-wide BVH: 128-byte nodes, 48-byte triangle
  * slots (three float4 vertices, primitive id in the first w). */
 typedef struct hdcx_bvh_info {
     uint32_t node_count;
     uint32_t slot_count;
-    uint32_t max_stack;  /* worst-case traversal stack entries */
+    uint32_t max_stack;    /* worst-case traversal stack entries */
     uint32_t depth;
+    int32_t builder;       /* HDCX_BVH_* that produced it */
+    uint32_t top_clusters; /* LBVH: subtrees under the SAH top levels (0: none) */
+    uint32_t refits;       /* refits applied since the last build */
+    uint32_t reserved;
+    double build_ms;       /* wall time of the last build or refit */
 } hdcx_bvh_info;
 
 /* Builds the host (binned-SAH) BVH of a scene view and validates it. Call
@@ -204,6 +217,27 @@ typedef struct hdcx_bvh_info {
  * buffers of node_count*128 bytes and slot_count*12 floats. */
 int hdcx_bvh_build_host(const hdcx_scene_view* scene, hdcx_bvh_info* out_info,
                         void* out_nodes, float* out_slots);
+
+/* As hdcx_bvh_build_host with a choice of builder (HDCX_BVH_*);
+ * HDCX_BVH_DEVICE_LBVH runs on the current HIP device. top_clusters: LBVH
+ * subtrees under the SAH top levels (0 = radix tree only). The result is
+ * validated on the host (every triangle once, leaves in range, stack fits). */
+int hdcx_bvh_build(const hdcx_scene_view* scene, int builder, uint32_t top_clusters,
+                   hdcx_bvh_info* out_info, void* out_nodes, float* out_slots);
+
+/* Tracer BVH policy. refit != 0: hdcx_set_scene with the same triangles
+ * (vertex count and index buffer unchanged, e.g. skinned points) refits the
+ * existing BVH on the device instead of rebuilding it. Defaults:
+ * HDCX_BVH_DEVICE_LBVH, 2048 top clusters, refit on. */
+typedef struct hdcx_bvh_options {
+    int32_t builder;
+    uint32_t top_clusters;
+    uint32_t refit;
+    uint32_t reserved;
+} hdcx_bvh_options;
+
+int hdcx_set_bvh_options(hdcx_tracer* tracer, const hdcx_bvh_options* options);
+int hdcx_get_bvh_info(hdcx_tracer* tracer, hdcx_bvh_info* out_info);
 
 /* ---- Radiance cache (no counterpart in the reference: its neural
  * rendering plan is unimplemented, docs/neural-rendering-plan.md:32-48).

@@ -7,6 +7,8 @@ and asserts that every node, slot and stack access stays in bounds. Its
 hits must equal the oracle's brute-force closest hits bit for bit.
 """
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -65,7 +67,7 @@ def _intersect(o, pre, p0, p1, p2, tmin, tmax):
     return t, F(V / det), F(W / det)
 
 
-def emulate_closest(nodes, slots, o, d, tmin=F(0.001), tmax=F(1e4)):
+def emulate_closest(nodes, slots, o, d, tmin=F(0.001), tmax=F(1e4), counter=None):
     o = o.astype(F)
     d = d.astype(F)
     pre = _prepare(o, d)
@@ -89,6 +91,8 @@ def emulate_closest(nodes, slots, o, d, tmin=F(0.001), tmax=F(1e4)):
         else:
             assert cur < len(nodes), "inner reference out of range"
             n = nodes[cur]
+            if counter is not None:
+                counter[0] += 1
             tn, ref = [], []
             for i in range(4):
                 c = int(n["child"][i])
@@ -113,10 +117,14 @@ def emulate_closest(nodes, slots, o, d, tmin=F(0.001), tmax=F(1e4)):
     return (best_t, best_p, best_uv) if found else (F(0), EMPTY, (F(0), F(0)))
 
 
+BUILDERS = [(hx.BVH_HOST_SAH, 0), (hx.BVH_SERIAL_LBVH, 0), (hx.BVH_SERIAL_LBVH, 2048), (hx.BVH_SERIAL_LBVH, 16)]
+
+
+@pytest.mark.parametrize("builder,clusters", BUILDERS)
 @pytest.mark.parametrize("name,args", [("cornell", (0,)), ("materials", (0,)), ("heightfield", (2, 48, 16))])
-def test_host_bvh_structure(name, args):
+def test_bvh_structure(name, args, builder, clusters):
     scene = hx.ProceduralScene(name, *args)
-    info, nodes, slots = hx.build_bvh_host(scene)
+    info, nodes, slots = hx.build_bvh(scene, builder, clusters)
     assert info["slot_count"] == scene.triangle_count
     assert info["max_stack"] <= MAX_STACK
     prims = np.sort(slots[:, 0, 3].view(np.uint32))
@@ -127,11 +135,12 @@ def test_host_bvh_structure(name, args):
                 assert n["lox"][i] > n["hix"][i]
 
 
+@pytest.mark.parametrize("builder,clusters", BUILDERS)
 @pytest.mark.parametrize("name,args,count", [("cornell", (0,), 300), ("materials", (0,), 200),
                                               ("heightfield", (2, 48, 16), 200)])
-def test_traversal_emulation_matches_oracle(name, args, count):
+def test_traversal_emulation_matches_oracle(name, args, count, builder, clusters):
     scene = hx.ProceduralScene(name, *args)
-    _, nodes, slots = hx.build_bvh_host(scene)
+    _, nodes, slots = hx.build_bvh(scene, builder, clusters)
     cam = scene.camera(64, 64)
     a = scene.arrays()["positions"][:, :3]
     rays = scene_rays(cam, count, 5, inside=(a.min(axis=0), a.max(axis=0)))
@@ -152,3 +161,79 @@ def test_empty_slot_box_alone_would_pass_the_slab_test():
     o, inv = F(0.3), F(1 / 0.7)
     t0, t1 = (inf - o) * inv, (-inf - o) * inv
     assert max(min(t0, t1), F(0.001)) <= min(max(t0, t1), F(1e4))
+
+
+class _Geometry:
+    """A scene view over raw positions / indices (what the builders read)."""
+
+    def __init__(self, positions, indices):
+        self.positions = np.ascontiguousarray(np.c_[positions, np.ones(len(positions))], dtype=np.float32)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32).reshape(-1)
+        v = hx.SceneView()
+        v.vertex_count = len(self.positions)
+        v.triangle_count = len(self.indices) // 3
+        v.positions = self.positions.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        v.indices = self.indices.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        self.view = v
+        self.triangle_count = v.triangle_count
+
+
+def _soup(kind, n, seed=0):
+    rng = np.random.default_rng(seed)
+    if kind == "identical":  # one Morton code for every triangle: ties broken by position
+        base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float64)
+        pos = np.tile(base, (n, 1))
+    elif kind == "geometric":  # centres at 2^-k: a chain of single-triangle splits
+        c = np.r_[[1.0], 0.5 ** np.arange(1, n)][:, None] * np.array([[1.0, 0.3, 0.7]])
+        pos = np.repeat(c, 3, axis=0) + np.tile(np.array([[0, 0, 0], [1e-3, 0, 0], [0, 1e-3, 0]]), (n, 1))
+    elif kind == "random":
+        c = rng.uniform(-10, 10, (n, 3))
+        pos = np.repeat(c, 3, axis=0) + rng.normal(0, 0.3, (3 * n, 3))
+    else:  # "planar": coincident plane, zero extent along y
+        c = rng.uniform(-5, 5, (n, 3))
+        c[:, 1] = 0.0
+        pos = np.repeat(c, 3, axis=0) + rng.normal(0, 0.2, (3 * n, 3)) * np.array([1.0, 0.0, 1.0])
+    return _Geometry(pos, np.arange(3 * n))
+
+
+@pytest.mark.parametrize("clusters", [0, 8, 2048])
+@pytest.mark.parametrize("kind,n", [("identical", 2), ("identical", 5), ("identical", 300), ("geometric", 60),
+                                    ("random", 3), ("random", 4000), ("planar", 500)])
+def test_lbvh_degenerate_inputs(kind, n, clusters):
+    """Validated by the library (every triangle once, leaves in range, empty
+    slots inverted); traversal of the result equals brute force."""
+    g = _soup(kind, n)
+    info, nodes, slots = hx.build_bvh(g, hx.BVH_SERIAL_LBVH, clusters)
+    assert info["slot_count"] == n and info["max_stack"] <= MAX_STACK
+    lo, hi = g.positions[:, :3].min(axis=0), g.positions[:, :3].max(axis=0)
+    rng = np.random.default_rng(n)
+    o = rng.uniform(lo - 1, hi + 1, (40, 3)).astype(F)
+    tgt = g.positions[rng.integers(0, len(g.positions), 40), :3]
+    d = (tgt - o).astype(np.float64)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = hx.rays_array(o, d.astype(F))
+    tris = g.positions[g.indices.reshape(-1, 3), :3]
+    for r in rays:
+        t, p, _ = emulate_closest(nodes, slots, r[0:3], r[4:7], F(r[3]), F(r[7]))
+        best = (F(r[7]), EMPTY)
+        pre = _prepare(r[0:3], r[4:7])
+        for k, tri in enumerate(tris):
+            h = _intersect(r[0:3], pre, tri[0], tri[1], tri[2], F(r[3]), best[0])
+            if h is not None and (best[1] == EMPTY or h[0] < best[0] or (h[0] == best[0] and k < best[1])):
+                best = (h[0], k)
+        assert p == best[1]
+
+
+def test_lbvh_sah_top_never_costs_more_nodes_visited_on_the_heightfield():
+    scene = hx.ProceduralScene("heightfield", 4, 96, 16)
+    cam = scene.camera(64, 36)
+    a = scene.arrays()["positions"][:, :3]
+    rays = scene_rays(cam, 150, 9, inside=(a.min(axis=0), a.max(axis=0)))
+    visits = {}
+    for clusters in (0, 2048):
+        _, nodes, slots = hx.build_bvh(scene, hx.BVH_SERIAL_LBVH, clusters)
+        count = [0]
+        for r in rays:
+            emulate_closest(nodes, slots, r[0:3], r[4:7], F(r[3]), F(r[7]), counter=count)
+        visits[clusters] = count[0]
+    assert visits[2048] <= visits[0] * 1.05, visits
