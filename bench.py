@@ -185,6 +185,8 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--cache-warmup", type=int, default=6)
+    ap.add_argument("--bvh", choices=("lbvh", "sah"), default="lbvh",
+                    help="BVH builder: device LBVH with SAH top levels, or host binned SAH")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel 4 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -201,9 +203,11 @@ def main() -> int:
     scene = hx.ProceduralScene("heightfield", 0, GRID, DOME)
     cam = scene.camera(WIDTH, HEIGHT)
     tracer = hx.PathTracer(0)
+    tracer.set_bvh_options(hx.BVH_DEVICE_LBVH if args.bvh == "lbvh" else hx.BVH_HOST_SAH, 2048, True)
     t0 = time.perf_counter()
     tracer.set_scene(scene)
     set_scene_s = time.perf_counter() - t0
+    bvh = tracer.bvh_info()
     log(f"set_scene {set_scene_s:.2f} s, {scene.triangle_count} triangles")
 
     for i in range(args.warmup):
@@ -276,6 +280,8 @@ def main() -> int:
         "mrays_per_s": mrays,
         "lane_paths_per_s_M": 3 * mpaths,
         "set_scene_s": set_scene_s,
+        "bvh": {"builder": args.bvh, "build_ms": bvh["build_ms"], "nodes": bvh["node_count"],
+                "max_stack": bvh["max_stack"], "depth": bvh["depth"], "top_clusters": bvh["top_clusters"]},
         "dominant_kernel": dominant,
         "kernels": kernels,
         "traversal": roof,

@@ -326,7 +326,7 @@ HIT_DTYPE = np.dtype([("t", np.float32), ("primitive", np.uint32), ("u", np.floa
 class RadianceCacheConfig(ctypes.Structure):
     _fields_ = [("enabled", c_uint32), ("train_samples", c_uint32), ("terminate_bounce", c_uint32),
                 ("train_steps", c_uint32), ("learning_rate", c_float), ("beta1", c_float), ("beta2", c_float),
-                ("epsilon", c_float), ("seed", c_uint64)]
+                ("epsilon", c_float), ("ema_decay", c_float), ("spread_threshold", c_float), ("seed", c_uint64)]
 
 
 class RcStats(ctypes.Structure):
@@ -351,9 +351,10 @@ def _bind_cache(lib: ctypes.CDLL) -> None:
 
 
 def _tracer_set_radiance_cache(self, enabled: bool = True, train_samples: int = 65536, terminate_bounce: int = 2,
-                               train_steps: int = 4, learning_rate: float = 1e-2, seed: int = 0) -> None:
+                               train_steps: int = 4, learning_rate: float = 1e-2, seed: int = 0,
+                               ema_decay: float = 0.0, spread_threshold: float = 0.0) -> None:
     c = RadianceCacheConfig(int(enabled), train_samples, terminate_bounce, train_steps, learning_rate, 0.9, 0.99,
-                            1e-8, seed)
+                            1e-8, ema_decay, spread_threshold, seed)
     _check(library().hdcx_set_radiance_cache(self._h, ctypes.byref(c)))
 
 

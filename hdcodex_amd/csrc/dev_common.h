@@ -83,6 +83,7 @@ struct PathBuffers {
     std::uint32_t* counters; // [0],[1] extend queue sizes; [2] shadow queue size
     unsigned long long* totals;  // [0] extend rays, [1] shadow-queued paths, [2] shadow rays traced
     float* lambda0;          // wavelength of lane 0 of the path's hero triple
+    float2* spread;          // radiance cache: primary footprint a0, accumulated sqrt spread
     std::uint32_t capacity;
 };
 
@@ -91,10 +92,15 @@ struct RcQuery;
 // Radiance-cache hooks of the integrator. Training pixel samples (all three
 // lanes) are those with (ps + trainOffset) % trainStride == 0; they run the
 // plain estimator and record their first recordVertices vertices. Other
-// paths stop at vertex terminateBounce and queue a cache query.
+// paths stop at the first vertex from terminateBounce on whose path spread
+// exceeds spreadThreshold x the primary footprint and queue a cache query
+// (Mueller et al., "Real-time Neural Radiance Caching for Path Tracing",
+// SIGGRAPH 2021, section 3.4: a(x1..xn) = (sum_i sqrt(|x_{i-1} - x_i|^2 /
+// (p(w_i) |cos theta_i|)))^2 against a0 = |x0 - x1|^2 / (4 pi cos theta_1)).
 struct CacheParams {
     std::uint32_t enabled;
     std::uint32_t terminateBounce;
+    float spreadThreshold;         // <= 0: stop at terminateBounce unconditionally
     std::uint32_t trainStride;
     std::uint32_t trainOffset;
     std::uint32_t recordVertices;

@@ -16,6 +16,8 @@ hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st);
+// Counts the answered cache queries of a bounce and empties the queue.
+hipError_t LaunchConsumeQueries(const CacheParams& cp, hipStream_t st);
 hipError_t LaunchResolve(double4* mean, const double4* batch, float4* out, std::uint32_t pixels, std::uint32_t sampleIndex,
                          std::uint32_t sampleCount, unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchZero(double4* p, std::uint32_t n, unsigned maxBlocks, hipStream_t st);

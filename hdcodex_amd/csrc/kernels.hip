@@ -59,6 +59,7 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc
             wl0 = 380.0f + 400.0f * (f0 - floorf(f0));
         }
         pb.lambda0[pid] = wl0;
+        if (cp.enabled) pb.spread[pid] = make_float2(0.0f, 0.0f);
         if (cp.enabled && (ps + cp.trainOffset) % cp.trainStride == 0U) {
             const std::uint32_t slot = (ps + cp.trainOffset) / cp.trainStride;
             if (slot < cp.recordSlots)
@@ -486,10 +487,22 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
                                 cp.records[r] = rq;
                                 cp.recordThroughput[r] = throughput;
                             }
-                        } else if (bounce == static_cast<int>(cp.terminateBounce)) {
-                            cacheStop = true;
-                            wantQuery = true;
-                            query = rq;
+                        } else {
+                            // Path spread at this vertex; a delta lobe before it adds nothing.
+                            float2 sp = pb.spread[pid];
+                            const float cosIn = fmaxf(fabsf(dot(ogn, rd)), 1e-4f);
+                            if (bounce == 0) {
+                                sp = make_float2(distance * distance / (4.0f * kPi * cosIn), 0.0f);
+                            } else if (!(flags & kPrevDelta) && mis.x > 0.0f) {
+                                sp.y += sqrtf(distance * distance / (mis.x * cosIn));
+                            }
+                            pb.spread[pid] = sp;
+                            if (bounce >= static_cast<int>(cp.terminateBounce) &&
+                                (cp.spreadThreshold <= 0.0f || sp.y * sp.y > cp.spreadThreshold * sp.x)) {
+                                cacheStop = true;
+                                wantQuery = true;
+                                query = rq;
+                            }
                         }
                     }
 
@@ -890,8 +903,6 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(DeviceScene s, PathBuffer
 // scattered at j toward the previous vertex).
 __global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32_t records)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        *reinterpret_cast<unsigned long long*>(cp.queryCount + 2) += min(*cp.queryCount, cp.queryCapacity);
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < records; i += gridDim.x * blockDim.x) {
         RcQuery r = cp.records[i];
         if (r.lane > 2U) continue;
@@ -906,6 +917,14 @@ __global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32
             cp.samples[k] = r;
         }
     }
+}
+
+// After the queries of a bounce are answered: counts them (queryCount + 2,
+// 64-bit) and empties the queue for the next bounce.
+__global__ void consumeQueriesKernel(CacheParams cp)
+{
+    *reinterpret_cast<unsigned long long*>(cp.queryCount + 2) += min(*cp.queryCount, cp.queryCapacity);
+    *cp.queryCount = 0U;
 }
 
 // ---------------------------------------------------------------------------
@@ -1040,6 +1059,11 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32
 hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st)
 {
     hipLaunchKernelGGL(filmKernel, dim3(GridFor(c.p1 - c.p0, maxBlocks)), dim3(kBlock), 0, st, pb, c, batch);
+    return hipGetLastError();
+}
+hipError_t LaunchConsumeQueries(const CacheParams& cp, hipStream_t st)
+{
+    hipLaunchKernelGGL(consumeQueriesKernel, dim3(1), dim3(1), 0, st, cp);
     return hipGetLastError();
 }
 hipError_t LaunchResolve(double4* mean, const double4* batch, float4* out, std::uint32_t pixels, std::uint32_t sampleIndex,

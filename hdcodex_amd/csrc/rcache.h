@@ -48,9 +48,13 @@ struct RcState {
     float* m;          // Adam first moment
     float* v;          // Adam second moment
     float* grad;       // accumulated gradient
-    __bf16* w;         // BF16 weights
+    __bf16* w;         // BF16 weights (training passes)
     __bf16* wt;        // BF16 transposes of W1..W4
     float* loss;       // [2]: summed relative L2 loss, sample count
+    float* ema;        // FP32 exponential moving average of the weights
+    __bf16* we;        // BF16 inference weights: the average (or the weights when emaDecay == 0)
+    __bf16* wte;       // their transposes
+    float emaDecay;    // per Adam step
 };
 
 // Inference: out[i * 3 + k] for every query (when out != nullptr), and
@@ -64,12 +68,13 @@ hipError_t LaunchRcInfer(const RcNetwork& net, const RcQuery* queries, const std
 hipError_t LaunchRcGradient(const RcNetwork& net, const RcQuery* samples, const std::uint32_t* count,
                             std::uint32_t maxCount, float* partial, float* loss, unsigned maxBlocks, hipStream_t st);
 // Adam step (bias-corrected) on all parameters from the sum of `blocks`
-// partial rows in s.grad (cleared afterwards); refreshes the BF16 copies.
+// partial rows in s.grad (cleared afterwards); updates the weight average
+// and refreshes both BF16 copies.
 hipError_t LaunchRcAdamBlocks(const RcState& s, int blocks, int step, float lr, float beta1, float beta2, float eps,
                               hipStream_t st);
 // out[i] = sum of the partial rows (cleared afterwards).
 hipError_t LaunchRcReduce(float* partial, int blocks, float* out, hipStream_t st);
-// BF16 copies from the FP32 master weights.
+// Resets the average to the FP32 master weights and publishes both BF16 copies.
 hipError_t LaunchRcPublish(const RcState& s, hipStream_t st);
 
 } // namespace hdcx

@@ -353,12 +353,12 @@ __device__ __forceinline__ float reducePartials(float* partial, int blocks, int 
     return gsum;
 }
 
-__device__ __forceinline__ void publish(const RcState& s, int i, float w)
+__device__ __forceinline__ void publish(__bf16* w, __bf16* wt, int i, float value)
 {
-    s.w[i] = static_cast<__bf16>(w);
+    w[i] = static_cast<__bf16>(value);
     if (i >= kRcW1 && i < kRcW5) {
         const int r = i - kRcW1, layer = r / 4096, o = (r % 4096) / 64, k = r % 64;
-        s.wt[layer * 4096 + k * 64 + o] = static_cast<__bf16>(w);
+        wt[layer * 4096 + k * 64 + o] = static_cast<__bf16>(value);
     }
 }
 
@@ -374,7 +374,10 @@ __global__ void rcAdamKernel(RcState s, int blocks, int step, float lr, float be
         s.v[i] = v;
         const float w = s.master[i] - lr * (m / c1) / (sqrtf(v / c2) + eps);
         s.master[i] = w;
-        publish(s, i, w);
+        publish(s.w, s.wt, i, w);
+        const float e = s.emaDecay * s.ema[i] + (1.0f - s.emaDecay) * w;
+        s.ema[i] = e;
+        publish(s.we, s.wte, i, e);
     }
 }
 
@@ -386,8 +389,12 @@ __global__ void rcReduceKernel(float* partial, int blocks, float* out)
 
 __global__ void rcPublishKernel(RcState s)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kRcParams; i += gridDim.x * blockDim.x)
-        publish(s, i, s.master[i]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kRcParams; i += gridDim.x * blockDim.x) {
+        const float w = s.master[i];
+        s.ema[i] = w;
+        publish(s.w, s.wt, i, w);
+        publish(s.we, s.wte, i, w);
+    }
 }
 
 } // namespace

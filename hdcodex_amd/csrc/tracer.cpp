@@ -85,7 +85,7 @@ struct hdcx_tracer {
     std::uint32_t builtVertexCount = 0;
     // Path state.
     DeviceBuffer rayO, rayD, hit, rng, wavelength, throughput, radiance, flags, mis, medium, shadowO, shadowD,
-        shadowLight, queue0, queue1, shadowQueue, counters, totals, traversalCounters, lambda0;
+        shadowLight, queue0, queue1, shadowQueue, counters, totals, traversalCounters, lambda0, spread;
     std::uint32_t capacity = 0;
     // Film.
     DeviceBuffer filmMean, filmBatch, filmOut;
@@ -100,7 +100,7 @@ struct hdcx_tracer {
     std::uint64_t rcSeed = 0;
     int rcStep = 0;
     hdcx_rc_stats rcStats{};
-    DeviceBuffer rcMaster, rcM, rcV, rcGrad, rcW, rcWt, rcLoss, rcQueries, rcOut;
+    DeviceBuffer rcMaster, rcM, rcV, rcGrad, rcW, rcWt, rcLoss, rcQueries, rcOut, rcEma, rcWe, rcWte;
     DeviceBuffer rcChunkQueries, rcCounters, rcRecords, rcRecordThroughput, rcSamples;
     std::uint32_t rcFrame = 0;
     bool rcCountersReady = false;
@@ -168,11 +168,17 @@ struct hdcx_tracer {
         r.grad = rcGrad.as<float>();
         r.w = rcW.as<__bf16>();
         r.wt = rcWt.as<__bf16>();
+        r.ema = rcEma.as<float>();
+        r.we = rcWe.as<__bf16>();
+        r.wte = rcWte.as<__bf16>();
+        r.emaDecay = rcConfig.ema_decay;
         r.loss = rcLoss.as<float>();
         return r;
     }
 
+    // Weights of the training passes, and the (averaged) inference weights.
     RcNetwork RcNet() { return RcNetwork{rcW.as<__bf16>(), rcWt.as<__bf16>()}; }
+    RcNetwork RcInferNet() { return RcNetwork{rcWe.as<__bf16>(), rcWte.as<__bf16>()}; }
 
     // Allocates the cache state and (re)initialises the weights for `seed`:
     // He-uniform bounds sqrt(6 / fan_in), the output layer scaled by 0.5;
@@ -187,6 +193,9 @@ struct hdcx_tracer {
         rcGrad.Allocate(static_cast<std::size_t>(kRcGradBlocks) * kRcParams * sizeof(float));
         rcW.Allocate(kRcParams * sizeof(std::uint16_t));
         rcWt.Allocate(4 * 4096 * sizeof(std::uint16_t));
+        rcEma.Allocate(kRcParams * sizeof(float));
+        rcWe.Allocate(kRcParams * sizeof(std::uint16_t));
+        rcWte.Allocate(4 * 4096 * sizeof(std::uint16_t));
         rcLoss.Allocate(2 * sizeof(float));
         std::vector<float> w(kRcParams);
         std::uint64_t state = seed ^ 0x5deece66dULL;
@@ -261,6 +270,7 @@ struct hdcx_tracer {
         pb.counters = counters.as<std::uint32_t>();
         pb.totals = totals.as<unsigned long long>();
         pb.lambda0 = lambda0.as<float>();
+        pb.spread = spread.as<float2>();
         pb.capacity = capacity;
         return pb;
     }
@@ -286,6 +296,7 @@ struct hdcx_tracer {
         queue1.Allocate(n * 4);
         shadowQueue.Allocate(n * 4);
         lambda0.Allocate(n * 4);
+        spread.Allocate(n * 8);
         capacity = paths;
     }
 
@@ -573,6 +584,7 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
         const std::uint64_t wanted = std::max<std::uint64_t>(1, t->rcConfig.train_samples * 2ULL / 15ULL);
         cp.enabled = 1;
         cp.terminateBounce = t->rcConfig.terminate_bounce;
+        cp.spreadThreshold = t->rcConfig.spread_threshold;
         cp.trainStride = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, frameSamples / wanted));
         cp.trainOffset = static_cast<std::uint32_t>((t->rcFrame * 2654435761ULL) % cp.trainStride);
         cp.recordVertices = kRecordVertices;
@@ -630,11 +642,13 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
                     return LaunchShade(t->scene, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, cp, paths, t->maxBlocks, st);
                 });
-                if (cache && b == cp.terminateBounce) {
+                if (cache && b >= cp.terminateBounce) {
+                    // Queries of this bounce: answered, then the queue is emptied.
                     t->Timed(HDCX_KERNEL_CACHE, [&] {
-                        return LaunchRcInfer(t->RcNet(), cp.queries, cp.queryCount, paths, nullptr, pb.radiance, pb.throughput,
+                        return LaunchRcInfer(t->RcInferNet(), cp.queries, cp.queryCount, paths, nullptr, pb.radiance, pb.throughput,
                                              t->maxBlocks, st);
                     });
+                    Check(LaunchConsumeQueries(cp, st), "rc queries");
                 }
                 t->Timed(HDCX_KERNEL_SHADOW, [&] { return LaunchShadow(t->scene, pb, paths, t->maxBlocks, shadowCtr, st); });
             }
@@ -969,6 +983,7 @@ int hdcx_set_radiance_cache(hdcx_tracer* tracer, const hdcx_radiance_cache_confi
         if (!(c.beta2 > 0.0F)) c.beta2 = 0.99F;
         if (!(c.epsilon > 0.0F)) c.epsilon = 1e-8F;
         if (c.terminate_bounce >= HDCX_MAX_BOUNCES) throw std::invalid_argument("terminate_bounce out of range");
+        if (!(c.ema_decay >= 0.0F && c.ema_decay < 1.0F)) throw std::invalid_argument("ema_decay outside [0, 1)");
         tracer->rcConfig = c;
         tracer->EnsureRc(c.seed);
         tracer->rcEnabled = true;
@@ -1006,7 +1021,7 @@ int hdcx_rc_infer(hdcx_tracer* tracer, const hdcx_rc_query* queries, uint32_t co
         tracer->EnsureRc(tracer->rcInitialised ? tracer->rcSeed : 0);
         tracer->rcQueries.Upload(queries, static_cast<std::size_t>(count) * sizeof(RcQuery), tracer->stream);
         tracer->rcOut.Allocate(static_cast<std::size_t>(count) * 3 * sizeof(float));
-        Check(LaunchRcInfer(tracer->RcNet(), tracer->rcQueries.as<RcQuery>(), nullptr, count, tracer->rcOut.as<float>(),
+        Check(LaunchRcInfer(tracer->RcInferNet(), tracer->rcQueries.as<RcQuery>(), nullptr, count, tracer->rcOut.as<float>(),
                             nullptr, nullptr, tracer->maxBlocks, tracer->stream),
               "rc infer");
         Check(hipMemcpyAsync(out, tracer->rcOut.as<void>(), static_cast<std::size_t>(count) * 3 * sizeof(float),

@@ -269,6 +269,13 @@ typedef struct hdcx_radiance_cache_config {
     float beta1;               /* default 0.9 */
     float beta2;               /* default 0.99 */
     float epsilon;             /* default 1e-8 */
+    float ema_decay;           /* inference uses an exponential moving average of the
+                                * weights with this decay per Adam step, in [0, 1);
+                                * 0 = the current weights (default) */
+    float spread_threshold;    /* a path stops into the cache at the first vertex from
+                                * terminate_bounce on where its spread exceeds this
+                                * times its primary footprint (Mueller et al. 2021);
+                                * <= 0: at terminate_bounce unconditionally */
     uint64_t seed;             /* weight initialisation */
 } hdcx_radiance_cache_config;
 
@@ -289,7 +296,8 @@ int hdcx_set_radiance_cache(hdcx_tracer* tracer, const hdcx_radiance_cache_confi
 /* FP32 master weights, row-major W0 [64][128], W1..W4 [64][64], W5 [3][64]. */
 int hdcx_rc_get_params(hdcx_tracer* tracer, float* out_params);
 int hdcx_rc_set_params(hdcx_tracer* tracer, const float* params);
-/* Network outputs (3 per query) for host-side queries. */
+/* Network outputs (3 per query) for host-side queries, with the inference
+ * weights (the moving average when ema_decay > 0). */
 int hdcx_rc_infer(hdcx_tracer* tracer, const hdcx_rc_query* queries, uint32_t count, float* out);
 /* Gradient of the mean relative L2 loss over the samples, without an
  * update; out_loss receives the mean loss. */
