@@ -142,6 +142,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_reset_kernel_stats.argtypes = [c_void_p]
     _bind_cache(lib)
     _bind_bvh(lib)
+    _bind_batch(lib)
     _lib = lib
     return lib
 
@@ -350,9 +351,9 @@ def _bind_cache(lib: ctypes.CDLL) -> None:
     lib.hdcx_rc_get_stats.argtypes = [c_void_p, POINTER(RcStats)]
 
 
-def _tracer_set_radiance_cache(self, enabled: bool = True, train_samples: int = 65536, terminate_bounce: int = 2,
+def _tracer_set_radiance_cache(self, enabled: bool = True, train_samples: int = 65536, terminate_bounce: int = 1,
                                train_steps: int = 4, learning_rate: float = 1e-2, seed: int = 0,
-                               ema_decay: float = 0.0, spread_threshold: float = 0.0) -> None:
+                               ema_decay: float = 0.99, spread_threshold: float = 0.0) -> None:
     c = RadianceCacheConfig(int(enabled), train_samples, terminate_bounce, train_steps, learning_rate, 0.9, 0.99,
                             1e-8, ema_decay, spread_threshold, seed)
     _check(library().hdcx_set_radiance_cache(self._h, ctypes.byref(c)))
@@ -452,3 +453,102 @@ def _tracer_bvh_info(self) -> dict:
 PathTracer.set_bvh_options = _tracer_set_bvh_options
 PathTracer.bvh_info = _tracer_bvh_info
 EXPORTED_SYMBOLS = EXPORTED_SYMBOLS + ("hdcx_bvh_build", "hdcx_set_bvh_options", "hdcx_get_bvh_info")
+
+
+# ---- Batched multi-scene renderer -----------------------------------------
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("wave_paths", c_uint32), ("tile_pixels", c_uint32), ("max_blocks", c_uint32), ("use_graph", c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("pixel_samples", c_uint64), ("lane_paths", c_uint64), ("waves", c_uint32),
+                ("graph_launches", c_uint32), ("milliseconds", c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def _bind_batch(lib: ctypes.CDLL) -> None:
+    lib.hdcx_batch_create.argtypes = [c_int, POINTER(c_void_p)]
+    lib.hdcx_batch_destroy.argtypes = [c_void_p]
+    lib.hdcx_batch_add_scene.argtypes = [c_void_p, POINTER(SceneView), POINTER(BvhOptions), POINTER(c_uint32)]
+    lib.hdcx_batch_set_view.argtypes = [c_void_p, c_uint32, POINTER(Camera), c_uint32, c_uint32, c_uint32]
+    lib.hdcx_batch_set_tuning.argtypes = [c_void_p, POINTER(BatchTuning)]
+    lib.hdcx_batch_get_tuning.argtypes = [c_void_p, POINTER(BatchTuning)]
+    lib.hdcx_batch_autotune.argtypes = [c_void_p, c_uint32, c_char_p, POINTER(BatchTuning)]
+    lib.hdcx_batch_render.argtypes = [c_void_p, c_uint32, c_uint32]
+    lib.hdcx_batch_read_film.argtypes = [c_void_p, c_uint32, c_void_p]
+    lib.hdcx_batch_get_stats.argtypes = [c_void_p, POINTER(BatchStats)]
+
+
+class BatchRenderer:
+    """Many scenes rendered together (hdcx_batch_*): one wavefront over all
+    scenes, per-wave launches replayed from a captured hipGraph."""
+
+    def __init__(self, device: int = 0):
+        self._h = c_void_p()
+        _check(library().hdcx_batch_create(device, ctypes.byref(self._h)))
+        self._sizes = []
+
+    def add_scene(self, scene: "ProceduralScene | SceneView", builder: int = BVH_DEVICE_LBVH,
+                  top_clusters: int = 2048) -> int:
+        view = scene.view if isinstance(scene, ProceduralScene) else scene
+        o = BvhOptions(builder, top_clusters, 1, 0)
+        index = c_uint32()
+        _check(library().hdcx_batch_add_scene(self._h, ctypes.byref(view), ctypes.byref(o), ctypes.byref(index)))
+        self._sizes.append((0, 0))
+        return int(index.value)
+
+    def set_view(self, scene: int, camera: Camera, width: int, height: int, samples: int) -> None:
+        _check(library().hdcx_batch_set_view(self._h, scene, ctypes.byref(camera), width, height, samples))
+        self._sizes[scene] = (width, height)
+
+    def set_tuning(self, wave_paths: int, tile_pixels: int, max_blocks: int, use_graph: bool = True) -> None:
+        t = BatchTuning(wave_paths, tile_pixels, max_blocks, 1 if use_graph else 0)
+        _check(library().hdcx_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        _check(library().hdcx_batch_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, max_bounces: int = 8, cache_path: "str | None" = None) -> dict:
+        t = BatchTuning()
+        path = cache_path.encode() if cache_path else None
+        _check(library().hdcx_batch_autotune(self._h, max_bounces, path, ctypes.byref(t)))
+        return t.as_dict()
+
+    def render(self, sample_index: int = 0, max_bounces: int = 8) -> None:
+        _check(library().hdcx_batch_render(self._h, sample_index, max_bounces))
+
+    def read_film(self, scene: int) -> np.ndarray:
+        w, h = self._sizes[scene]
+        out = np.empty((h, w, 4), dtype=np.float32)
+        _check(library().hdcx_batch_read_film(self._h, scene, out.ctypes.data))
+        return out
+
+    def stats(self) -> dict:
+        s = BatchStats()
+        _check(library().hdcx_batch_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            _check(library().hdcx_batch_destroy(self._h))
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+EXPORTED_SYMBOLS = EXPORTED_SYMBOLS + (
+    "hdcx_batch_create", "hdcx_batch_destroy", "hdcx_batch_add_scene", "hdcx_batch_set_view", "hdcx_batch_set_tuning",
+    "hdcx_batch_get_tuning", "hdcx_batch_autotune", "hdcx_batch_render", "hdcx_batch_read_film",
+    "hdcx_batch_get_stats")

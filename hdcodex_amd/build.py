@@ -40,8 +40,9 @@ HIP_FLAGS = [
     "-I", str(ROOT / "include"),
 ]
 
-HOST_SOURCES = ["packer.cpp", "procedural.cpp", "procedural_showcase.cpp", "bvh_host.cpp", "lbvh_host.cpp"]
-HIP_SOURCES = ["kernels.hip", "rcache.hip", "lbvh.hip", "tracer.cpp"]
+HOST_SOURCES = ["packer.cpp", "procedural.cpp", "procedural_showcase.cpp", "procedural_mixed.cpp", "bvh_host.cpp",
+                "lbvh_host.cpp"]
+HIP_SOURCES = ["kernels.hip", "rcache.hip", "lbvh.hip", "scene_store.cpp", "tracer.cpp", "batch.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -84,7 +84,21 @@ struct PathBuffers {
     unsigned long long* totals;  // [0] extend rays, [1] shadow-queued paths, [2] shadow rays traced
     float* lambda0;          // wavelength of lane 0 of the path's hero triple
     float2* spread;          // radiance cache: primary footprint a0, accumulated sqrt spread
+    std::uint32_t* pathScene;  // batch renderer: scene of each path
     std::uint32_t capacity;
+};
+
+// Scene of a path: one scene for the whole launch (held by value in the
+// kernel arguments), or an entry of a device table of scenes selected per
+// path (the batch renderer).
+struct SingleScene {
+    DeviceScene s;
+    __device__ __forceinline__ const DeviceScene& Get(std::uint32_t) const { return s; }
+};
+struct SceneTable {
+    const DeviceScene* table;
+    const std::uint32_t* pathScene;
+    __device__ __forceinline__ const DeviceScene& Get(std::uint32_t pid) const { return table[pathScene[pid]]; }
 };
 
 struct RcQuery;
@@ -125,6 +139,32 @@ struct ChunkDesc {
     std::uint32_t s0, s1, p0, p1;
     std::uint32_t maxBounces;
     float cam[12];                   // origin, lowerLeft - origin, horizontal, vertical
+};
+
+// Batch renderer: one view per scene, and tiles (scene, pixel range, all of
+// the view's samples) packed into waves of at most the path-pool capacity.
+// Lane paths of a tile: pathOffset + ((s * (p1 - p0) + (p - p0)) * 3 + lane).
+struct BatchView {
+    float cam[12];               // origin, lowerLeft - origin, horizontal, vertical
+    std::uint32_t width, height;
+    std::uint32_t sampleBase;    // sample number of the first sample of the frame
+    std::uint32_t samples;       // samples per pixel in this frame
+    double4* batch;              // FP64 per-pixel sums of this frame's samples
+};
+struct BatchTile {
+    std::uint32_t scene;
+    std::uint32_t p0, p1;
+    std::uint32_t pathOffset;    // within its wave
+    std::uint32_t pixelOffset;   // within its wave
+    std::uint32_t pad[3];
+};
+struct BatchWaves {
+    const BatchView* views;
+    const BatchTile* tiles;          // wave by wave
+    const std::uint32_t* waveTiles;  // wave w: tiles [waveTiles[w], waveTiles[w + 1])
+    const std::uint32_t* wavePaths;  // lane paths of wave w
+    const std::uint32_t* wavePixels; // tile pixels of wave w
+    std::uint32_t* cursor;           // [0]: the wave being rendered
 };
 
 struct TraversalCounters {

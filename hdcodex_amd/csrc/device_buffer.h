@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <new>
 #include <stdexcept>
 #include <string>
 
@@ -25,11 +26,57 @@ inline void Check(hipError_t e, const char* what)
     throw Failure(HDCX_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Thread-local message of the last failed C-ABI call (hdcx_last_error).
+inline std::string& LastError()
+{
+    thread_local std::string message;
+    return message;
+}
+
+// Runs one C-ABI entry point: exceptions become status codes and a message.
+template <class F>
+int Guard(F&& f)
+{
+    try {
+        f();
+        LastError().clear();
+        return HDCX_OK;
+    } catch (const Failure& e) {
+        LastError() = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        LastError() = "host allocation failed";
+        return HDCX_ERR_OUT_OF_MEMORY;
+    } catch (const std::invalid_argument& e) {
+        LastError() = e.what();
+        return HDCX_ERR_INVALID_ARGUMENT;
+    } catch (const std::exception& e) {
+        LastError() = e.what();
+        return HDCX_ERR_INTERNAL;
+    }
+}
+
 class DeviceBuffer {
 public:
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_)
+    {
+        o.ptr_ = nullptr;
+        o.bytes_ = 0;
+    }
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept
+    {
+        if (this != &o) {
+            Reset();
+            ptr_ = o.ptr_;
+            bytes_ = o.bytes_;
+            o.ptr_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
     ~DeviceBuffer() { Reset(); }
     void Reset()
     {

@@ -36,6 +36,55 @@ __device__ __forceinline__ void waveAppend(bool keep, std::uint32_t item, std::u
 // ---------------------------------------------------------------------------
 // raygen: seed, hero wavelength, jittered pinhole ray (:1198-1224).
 // ---------------------------------------------------------------------------
+// One lane path: seed, hero wavelength, jittered pinhole ray and the
+// initial path state. cam: origin, lowerLeft - origin, horizontal, vertical.
+__device__ __forceinline__ void startPath(const PathBuffers& pb, const CacheParams& cp, std::uint32_t pid,
+                                          std::uint32_t ps, std::uint32_t lane, std::uint32_t pixel,
+                                          std::uint32_t sampleNumber, std::uint32_t width, std::uint32_t height,
+                                          const float* cam)
+{
+    const std::uint32_t px = pixel % width, py = pixel / width;
+    std::uint32_t rng = hashState(pixel ^ (sampleNumber * 0x9e3779b9U) ^ 0xa511e9b3U);
+    float wl, wl0;
+    {
+#pragma clang fp contract(off)
+        const float shift = __uint2float_rn(hashState(pixel ^ 0x68bc21ebU)) * (1.0f / 4294967296.0f);
+        const float phase = (__uint2float_rn(sampleNumber & 31U) + 0.5f) / 32.0f;
+        const float step = __uint2float_rn(lane) / 3.0f;
+        const float f = shift + phase + step;
+        wl = 380.0f + 400.0f * (f - floorf(f));
+        const float f0 = shift + phase;
+        wl0 = 380.0f + 400.0f * (f0 - floorf(f0));
+    }
+    pb.lambda0[pid] = wl0;
+    if (cp.enabled) pb.spread[pid] = make_float2(0.0f, 0.0f);
+    if (cp.enabled && (ps + cp.trainOffset) % cp.trainStride == 0U) {
+        const std::uint32_t slot = (ps + cp.trainOffset) / cp.trainStride;
+        if (slot < cp.recordSlots)
+            for (std::uint32_t j = 0; j < cp.recordVertices; ++j)
+                cp.records[(slot * 3U + lane) * cp.recordVertices + j].lane = 0xffffffffU;
+    }
+    const float jx = randomFloat(rng), jy = randomFloat(rng);
+    const float u = (static_cast<float>(px) + jx) / static_cast<float>(width);
+    const float v = (static_cast<float>(py) + jy) / static_cast<float>(height);
+    // vulkan_path_tracer.cpp:1211-1213 with lowerLeft - origin taken from
+    // the host (rounded once from FP64): same direction, without the FP32
+    // cancellation when |origin| is large.
+    const V3 o = v3(cam[0], cam[1], cam[2]);
+    const V3 d = normalize(v3(cam[3], cam[4], cam[5]) + v3(cam[6], cam[7], cam[8]) * u + v3(cam[9], cam[10], cam[11]) * v);
+    pb.rayO[pid] = make_float4(o.x, o.y, o.z, 0.0f);
+    pb.rayD[pid] = make_float4(d.x, d.y, d.z, 0.0f);
+    pb.rng[pid] = rng;
+    pb.wavelength[pid] = wl;
+    pb.throughput[pid] = 1.0f;
+    pb.radiance[pid] = 0.0f;
+    pb.flags[pid] = kEnvOnMiss | kPrevDelta;
+    pb.mis[pid] = make_float2(0.0f, 0.0f);
+    pb.medium[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int k = 0; k < kShadowSlots; ++k) pb.shadowLight[k * pb.capacity + pid] = 0xffffffffU;
+    pb.queue[0][pid] = pid;
+}
+
 __global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc c, std::uint32_t pathCount,
                                                       CacheParams cp)
 {
@@ -44,53 +93,46 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(PathBuffers pb, ChunkDesc
         const std::uint32_t ps = pid / 3U, lane = pid - ps * 3U;
         const std::uint32_t sOff = ps / pixelsInChunk;
         const std::uint32_t pixel = c.p0 + (ps - sOff * pixelsInChunk);
-        const std::uint32_t sampleNumber = c.sampleBase + c.s0 + sOff;
-        const std::uint32_t px = pixel % c.width, py = pixel / c.width;
-        std::uint32_t rng = hashState(pixel ^ (sampleNumber * 0x9e3779b9U) ^ 0xa511e9b3U);
-        float wl, wl0;
-        {
-#pragma clang fp contract(off)
-            const float shift = __uint2float_rn(hashState(pixel ^ 0x68bc21ebU)) * (1.0f / 4294967296.0f);
-            const float phase = (__uint2float_rn(sampleNumber & 31U) + 0.5f) / 32.0f;
-            const float step = __uint2float_rn(lane) / 3.0f;
-            const float f = shift + phase + step;
-            wl = 380.0f + 400.0f * (f - floorf(f));
-            const float f0 = shift + phase;
-            wl0 = 380.0f + 400.0f * (f0 - floorf(f0));
-        }
-        pb.lambda0[pid] = wl0;
-        if (cp.enabled) pb.spread[pid] = make_float2(0.0f, 0.0f);
-        if (cp.enabled && (ps + cp.trainOffset) % cp.trainStride == 0U) {
-            const std::uint32_t slot = (ps + cp.trainOffset) / cp.trainStride;
-            if (slot < cp.recordSlots)
-                for (std::uint32_t j = 0; j < cp.recordVertices; ++j)
-                    cp.records[(slot * 3U + lane) * cp.recordVertices + j].lane = 0xffffffffU;
-        }
-        const float jx = randomFloat(rng), jy = randomFloat(rng);
-        const float u = (static_cast<float>(px) + jx) / static_cast<float>(c.width);
-        const float v = (static_cast<float>(py) + jy) / static_cast<float>(c.height);
-        // vulkan_path_tracer.cpp:1211-1213 with lowerLeft - origin taken
-        // from the host (rounded once from FP64): same direction, without
-        // the FP32 cancellation when |origin| is large.
-        const V3 o = v3(c.cam[0], c.cam[1], c.cam[2]);
-        const V3 d = normalize(v3(c.cam[3], c.cam[4], c.cam[5]) + v3(c.cam[6], c.cam[7], c.cam[8]) * u +
-                               v3(c.cam[9], c.cam[10], c.cam[11]) * v);
-        pb.rayO[pid] = make_float4(o.x, o.y, o.z, 0.0f);
-        pb.rayD[pid] = make_float4(d.x, d.y, d.z, 0.0f);
-        pb.rng[pid] = rng;
-        pb.wavelength[pid] = wl;
-        pb.throughput[pid] = 1.0f;
-        pb.radiance[pid] = 0.0f;
-        pb.flags[pid] = kEnvOnMiss | kPrevDelta;
-        pb.mis[pid] = make_float2(0.0f, 0.0f);
-        pb.medium[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        for (int k = 0; k < kShadowSlots; ++k) pb.shadowLight[k * pb.capacity + pid] = 0xffffffffU;
-        pb.queue[0][pid] = pid;
+        startPath(pb, cp, pid, ps, lane, pixel, c.sampleBase + c.s0 + sOff, c.width, c.height, c.cam);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         pb.counters[0] = pathCount;
         if (cp.enabled) *cp.queryCount = 0U;
     }
+}
+
+// Last tile of [a, b) whose offset (selected by `field`) is <= x.
+template <class Field>
+__device__ __forceinline__ std::uint32_t findTile(const BatchTile* tiles, std::uint32_t a, std::uint32_t b,
+                                                  std::uint32_t x, Field field)
+{
+    while (b - a > 1U) {
+        const std::uint32_t m = (a + b) >> 1;
+        if (field(tiles[m]) <= x) a = m;
+        else b = m;
+    }
+    return a;
+}
+
+// Batch raygen: the lane paths of the current wave (read from the device
+// cursor, so that the same launch serves every wave of a captured graph).
+__global__ __launch_bounds__(kBlock) void raygenBatchKernel(PathBuffers pb, BatchWaves bw)
+{
+    const std::uint32_t w = *bw.cursor;
+    const std::uint32_t ta = bw.waveTiles[w], tb = bw.waveTiles[w + 1];
+    const std::uint32_t paths = bw.wavePaths[w];
+    const CacheParams none{};
+    for (std::uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x; pid < paths; pid += gridDim.x * blockDim.x) {
+        const BatchTile& t = bw.tiles[findTile(bw.tiles, ta, tb, pid, [](const BatchTile& x) { return x.pathOffset; })];
+        const BatchView& v = bw.views[t.scene];
+        const std::uint32_t local = (pid - t.pathOffset) / 3U, lane = pid - t.pathOffset - local * 3U;
+        const std::uint32_t tilePixels = t.p1 - t.p0;
+        const std::uint32_t so = local / tilePixels;
+        const std::uint32_t pixel = t.p0 + (local - so * tilePixels);
+        pb.pathScene[pid] = t.scene;
+        startPath(pb, none, pid, local, lane, pixel, v.sampleBase + so, v.width, v.height, v.cam);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) pb.counters[0] = paths;
 }
 
 // Index of this lane's entry in a compacted array (one atomic per wave), or
@@ -141,8 +183,8 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // ---------------------------------------------------------------------------
 // extend: closest hit for every queued path (:1228-1243).
 // ---------------------------------------------------------------------------
-template <bool kCount>
-__global__ __launch_bounds__(kBlock) void extendKernel(DeviceScene s, PathBuffers pb, std::uint32_t in,
+template <bool kCount, class Scenes>
+__global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in,
                                                       TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -158,6 +200,7 @@ __global__ __launch_bounds__(kBlock) void extendKernel(DeviceScene s, PathBuffer
     st.stride = kBlock;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const std::uint32_t pid = queue[i];
+        const DeviceScene& s = scenes.Get(pid);
         const float4 o = pb.rayO[pid], d = pb.rayD[pid];
         const RayPre r = prepareRay(o.x, o.y, o.z, d.x, d.y, d.z);
         HitRec h;
@@ -233,7 +276,8 @@ __device__ __forceinline__ void queueShadow(const PathBuffers& pb, std::uint32_t
 // shadow stage adds it in slot order, which keeps the reference's order of
 // additions (emission, analytic NEE, mesh-light NEE).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers pb, std::uint32_t in, int bounce,
+template <class Scenes>
+__global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int bounce,
                                                      int lastBounce, CacheParams cp)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -255,6 +299,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
         RcQuery query{};
         if (i < count) {
             pid = queue[i];
+            const DeviceScene& s = scenes.Get(pid);
             const float4 hit4 = pb.hit[pid];
             const std::uint32_t prim = __float_as_uint(hit4.y);
             float4 ro4 = pb.rayO[pid], rd4 = pb.rayD[pid];
@@ -802,8 +847,8 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(DeviceScene s, PathBuffers
 // radiance in slot order (shadowVisibility :855-889, meshLightVisible
 // :891-912).
 // ---------------------------------------------------------------------------
-template <bool kCount>
-__global__ __launch_bounds__(kBlock) void shadowKernel(DeviceScene s, PathBuffers pb, TraversalCounters* ctr)
+template <bool kCount, class Scenes>
+__global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffers pb, TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
     TraversalStack st;
@@ -817,6 +862,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(DeviceScene s, PathBuffer
     if (blockIdx.x == 0 && threadIdx.x == 0) pb.totals[1] += count;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const std::uint32_t pid = pb.shadowQueue[i];
+        const DeviceScene& s = scenes.Get(pid);
         const float wl = pb.wavelength[pid];
         float radiance = pb.radiance[pid];
         for (int k = 0; k < kShadowSlots; ++k) {
@@ -960,6 +1006,44 @@ __global__ void filmKernel(PathBuffers pb, ChunkDesc c, double4* batch)
     }
 }
 
+// Batch film: per tile pixel of the current wave, the same FP64 sum as
+// filmKernel, into its view's batch buffer (each pixel is in one tile).
+__global__ void filmBatchKernel(PathBuffers pb, BatchWaves bw)
+{
+    const std::uint32_t w = *bw.cursor;
+    const std::uint32_t ta = bw.waveTiles[w], tb = bw.waveTiles[w + 1];
+    const std::uint32_t pixels = bw.wavePixels[w];
+    for (std::uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < pixels; q += gridDim.x * blockDim.x) {
+        const BatchTile& t = bw.tiles[findTile(bw.tiles, ta, tb, q, [](const BatchTile& x) { return x.pixelOffset; })];
+        const BatchView& v = bw.views[t.scene];
+        const std::uint32_t k = q - t.pixelOffset, tilePixels = t.p1 - t.p0;
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (std::uint32_t so = 0; so < v.samples; ++so) {
+            const std::uint32_t base = t.pathOffset + (so * tilePixels + k) * 3U;
+            double sample[3] = {0.0, 0.0, 0.0};
+            for (std::uint32_t lane = 0; lane < 3; ++lane) {
+                double sens[3];
+                spectralSensor64(static_cast<double>(pb.wavelength[base + lane]), sens);
+                const double L = static_cast<double>(pb.radiance[base + lane]);
+                sample[0] += sens[0] * L;
+                sample[1] += sens[1] * L;
+                sample[2] += sens[2] * L;
+            }
+            acc[0] += sample[0] / 3.0;
+            acc[1] += sample[1] / 3.0;
+            acc[2] += sample[2] / 3.0;
+        }
+        double4 b = v.batch[t.p0 + k];
+        b.x += acc[0];
+        b.y += acc[1];
+        b.z += acc[2];
+        v.batch[t.p0 + k] = b;
+    }
+}
+
+// Moves the wave cursor on (last node of a wave's graph).
+__global__ void advanceWaveKernel(BatchWaves bw) { *bw.cursor += 1U; }
+
 // Progressive mean (:1886-1890) in FP64; alpha = 1.
 __global__ void resolveKernel(double4* mean, const double4* batch, float4* out, std::uint32_t pixels,
                               std::uint32_t sampleIndex, std::uint32_t sampleCount)
@@ -1036,24 +1120,83 @@ hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, st
     hipLaunchKernelGGL(gatherTrainingKernel, dim3(GridFor(records, maxBlocks)), dim3(kBlock), 0, st, pb, cp, records);
     return hipGetLastError();
 }
+template <class Scenes>
+hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+{
+    if (ctr)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, in, ctr);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, in, ctr);
+    return hipGetLastError();
+}
+template <class Scenes>
+hipError_t LaunchShadeT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
+                        const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb,
+                       in, bounce, last, cp);
+    return hipGetLastError();
+}
+template <class Scenes>
+hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                         TraversalCounters* ctr, hipStream_t st)
+{
+    if (ctr)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, ctr);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, ctr);
+    return hipGetLastError();
+}
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
-    if (ctr) hipLaunchKernelGGL(extendKernel<true>, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb, in, ctr);
-    else hipLaunchKernelGGL(extendKernel<false>, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb, in, ctr);
-    return hipGetLastError();
+    return LaunchExtendT(SingleScene{s}, pb, in, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
                        const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(shadeKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb, in, bounce, last, cp);
-    return hipGetLastError();
+    return LaunchShadeT(SingleScene{s}, pb, in, bounce, last, cp, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st)
 {
-    if (ctr) hipLaunchKernelGGL(shadowKernel<true>, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb, ctr);
-    else hipLaunchKernelGGL(shadowKernel<false>, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb, ctr);
+    return LaunchShadowT(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st);
+}
+hipError_t LaunchExtendBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+                             unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+{
+    return LaunchExtendT(SceneTable{table, pb.pathScene}, pb, in, maxWork, maxBlocks, ctr, st);
+}
+hipError_t LaunchShadeBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
+                            std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+{
+    return LaunchShadeT(SceneTable{table, pb.pathScene}, pb, in, bounce, last, CacheParams{}, maxWork, maxBlocks, st);
+}
+hipError_t LaunchShadowBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                             TraversalCounters* ctr, hipStream_t st)
+{
+    return LaunchShadowT(SceneTable{table, pb.pathScene}, pb, maxWork, maxBlocks, ctr, st);
+}
+hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(raygenBatchKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, pb, bw);
+    return hipGetLastError();
+}
+hipError_t LaunchFilmBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(filmBatchKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, pb, bw);
+    return hipGetLastError();
+}
+hipError_t LaunchAdvanceWave(const BatchWaves& bw, hipStream_t st)
+{
+    hipLaunchKernelGGL(advanceWaveKernel, dim3(1), dim3(1), 0, st, bw);
     return hipGetLastError();
 }
 hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st)

@@ -14,26 +14,6 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// SplitMix64 (Steele, Lea, Flood 2014).
-struct SplitMix64 {
-    std::uint64_t state;
-    std::uint64_t Next()
-    {
-        std::uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
-        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-        return z ^ (z >> 31);
-    }
-    double Uniform() { return static_cast<double>(Next() >> 11) * (1.0 / 9007199254740992.0); }
-    float Range(double lo, double hi) { return static_cast<float>(lo + (hi - lo) * Uniform()); }
-};
-
-std::uint64_t StreamSeed(std::uint64_t seed, std::uint64_t stream)
-{
-    SplitMix64 s{seed ^ (stream * 0xd1b54a32d192ed03ULL)};
-    return s.Next();
-}
-
 void AddQuad(SceneMesh& m, const double p[4][3])
 {
     const std::uint32_t base = static_cast<std::uint32_t>(m.positions.size() / 3);
@@ -498,6 +478,8 @@ ProceduralScene MakeProceduralScene(const std::string& name, std::uint64_t seed,
     if (name == "triangle") return MakeTriangle();
     if (name == "materials") return MakeMaterialShowcase(seed);
     if (name == "furnace") return MakeFurnace();
+    if (name == "mixed") return MakeMixedScene(seed, param == 0 ? 20000 : param, domeWidth == 0 ? 256 : domeWidth, 8);
+    if (name == "mixed_lights") return MakeMixedScene(seed, param == 0 ? 20000 : param, domeWidth == 0 ? 256 : domeWidth, 63);
     throw std::invalid_argument("unknown procedural scene '" + name + "'");
 }
 

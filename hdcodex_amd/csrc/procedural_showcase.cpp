@@ -14,62 +14,11 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// Unit icosphere of `level` subdivisions; smooth per-corner normals and
-// spherical UVs.
 SceneMesh Icosphere(const std::string& id, int level, const double c[3], double radius)
 {
-    const double t = (1.0 + std::sqrt(5.0)) / 2.0;
-    std::vector<std::array<double, 3>> v = {
-        {-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
-        {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
-    for (auto& p : v) {
-        const double l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-        for (double& x : p) x /= l;
-    }
-    std::vector<std::array<std::uint32_t, 3>> f = {
-        {0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
-        {11, 10, 2}, {10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8},
-        {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
-    for (int l = 0; l < level; ++l) {
-        std::map<std::pair<std::uint32_t, std::uint32_t>, std::uint32_t> mid;
-        auto midpoint = [&](std::uint32_t a, std::uint32_t b) {
-            const auto key = std::make_pair(std::min(a, b), std::max(a, b));
-            if (auto it = mid.find(key); it != mid.end()) return it->second;
-            std::array<double, 3> p{(v[a][0] + v[b][0]) / 2, (v[a][1] + v[b][1]) / 2, (v[a][2] + v[b][2]) / 2};
-            const double len = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-            for (double& x : p) x /= len;
-            v.push_back(p);
-            const auto index = static_cast<std::uint32_t>(v.size() - 1);
-            mid.emplace(key, index);
-            return index;
-        };
-        std::vector<std::array<std::uint32_t, 3>> next;
-        for (const auto& tri : f) {
-            const std::uint32_t a = midpoint(tri[0], tri[1]);
-            const std::uint32_t b = midpoint(tri[1], tri[2]);
-            const std::uint32_t cc = midpoint(tri[2], tri[0]);
-            next.push_back({tri[0], a, cc});
-            next.push_back({tri[1], b, a});
-            next.push_back({tri[2], cc, b});
-            next.push_back({a, b, cc});
-        }
-        f = std::move(next);
-    }
     SceneMesh m;
     m.id = id;
-    for (const auto& p : v)
-        for (int k = 0; k < 3; ++k) m.positions.push_back(static_cast<float>(c[k] + radius * p[k]));
-    for (const auto& tri : f) {
-        for (int k = 0; k < 3; ++k) {
-            const auto& p = v[tri[k]];
-            m.indices.push_back(tri[k]);
-            m.normals.insert(m.normals.end(), {static_cast<float>(p[0]), static_cast<float>(p[1]),
-                                               static_cast<float>(p[2])});
-            const double u = 0.5 + std::atan2(p[0], p[2]) / (2.0 * kPi);
-            const double w = 0.5 + std::asin(std::max(-1.0, std::min(1.0, p[1]))) / kPi;
-            m.texcoords.insert(m.texcoords.end(), {static_cast<float>(2.0 * u), static_cast<float>(w)});
-        }
-    }
+    AppendIcosphere(m, level, c, radius);
     return m;
 }
 

@@ -156,6 +156,9 @@ __global__ __launch_bounds__(kInferBlock) void rcInferKernel(RcNetwork net, cons
                                                             float* out, float* radiance, const float* throughput)
 {
     __shared__ uint4 tiles[kWeightTiles * 64];  // 52 KB of MFMA-ready weight tiles
+    const std::uint32_t count = countPtr ? min(*countPtr, maxCount) : maxCount;
+    // Blocks without a tile of queries leave before staging the weights.
+    if (blockIdx.x * (blockDim.x >> 6) >= (count + 31U) / 32U) return;
     for (int c = threadIdx.x; c < kWeightTiles * 64; c += blockDim.x) {
         const int b = c >> 6, l = c & 63;
         int layer, t, s;
@@ -165,7 +168,6 @@ __global__ __launch_bounds__(kInferBlock) void rcInferKernel(RcNetwork net, cons
         tiles[c] = __builtin_bit_cast(uint4, weightTile(net.w, layer, t, s, l));
     }
     __syncthreads();
-    const std::uint32_t count = countPtr ? min(*countPtr, maxCount) : maxCount;
     const int lane = threadIdx.x & 63;
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;

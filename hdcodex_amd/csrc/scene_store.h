@@ -1,0 +1,55 @@
+// Device-side state shared by the single-scene tracer and the batch
+// renderer: one scene's packed arrays and BVH (SceneStore), and the
+// wavefront path state (PathPool).
+#pragma once
+
+#include "../../include/hdcx.h"
+#include "bvh.hpp"
+#include "dev_common.h"
+#include "device_buffer.h"
+#include "lbvh.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace hdcx {
+
+struct SceneStore {
+    DeviceScene scene{};
+    bool geometry = false;
+    DeviceBuffer nodes, tris, positions, indices, triMaterial, texcoords, normals, emissive, materials, lights,
+        textures, texels, importance, importancePool;
+    // BVH: builder policy, the device builder's workspace, refit order and
+    // the triangles (vertex count, index buffer) the BVH was built over.
+    hdcx_bvh_options bvhOptions{HDCX_BVH_DEVICE_LBVH, 2048U, 1U, 0U};
+    hdcx_bvh_info bvhInfo{};
+    DeviceBvhBuilder lbvh;
+    RefitPlan refitPlan;
+    std::vector<std::uint32_t> builtIndices;
+    std::uint32_t builtVertexCount = 0;
+    // Vertex bounds (radiance-cache position normalisation).
+    float sceneLo[3] = {0, 0, 0}, sceneInvExtent[3] = {1, 1, 1};
+};
+
+// The checks of hdcx_set_scene (throws Failure HDCX_ERR_INVALID_SCENE).
+void ValidateScene(const hdcx_scene_view& v);
+
+// Uploads a scene to the current device: validates it, uploads every
+// packed array and builds the BVH, or refits it when the triangles are the
+// same as last time and refit is on. A null or empty view leaves no
+// geometry. Synchronises st.
+void UploadScene(SceneStore* s, const hdcx_scene_view* view, hipStream_t st);
+
+struct PathPool {
+    DeviceBuffer rayO, rayD, hit, rng, wavelength, throughput, radiance, flags, mis, medium, shadowO, shadowD,
+        shadowLight, queue0, queue1, shadowQueue, lambda0, spread, pathScene, counters, totals;
+    std::uint32_t capacity = 0;
+
+    // Room for `paths` lane paths (grows only).
+    void EnsurePaths(std::uint32_t paths);
+    PathBuffers Paths();
+};
+
+} // namespace hdcx

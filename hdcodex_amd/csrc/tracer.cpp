@@ -5,6 +5,7 @@
 #include "../../include/hdcx.h"
 #include "dev_traverse.h"
 #include "device_buffer.h"
+#include "scene_store.h"
 #include "kernel_api.h"
 #include "lbvh.h"
 #include "lbvh_host.hpp"
@@ -27,30 +28,6 @@
 namespace hdcx {
 namespace {
 
-thread_local std::string gLastError;
-
-template <class F>
-int Guard(F&& f)
-{
-    try {
-        f();
-        gLastError.clear();
-        return HDCX_OK;
-    } catch (const Failure& e) {
-        gLastError = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        gLastError = "host allocation failed";
-        return HDCX_ERR_OUT_OF_MEMORY;
-    } catch (const std::invalid_argument& e) {
-        gLastError = e.what();
-        return HDCX_ERR_INVALID_ARGUMENT;
-    } catch (const std::exception& e) {
-        gLastError = e.what();
-        return HDCX_ERR_INTERNAL;
-    }
-}
-
 constexpr std::uint32_t kChunkPixelSamples = 1U << 21;  // 6.3M lane paths per chunk
 
 } // namespace
@@ -63,30 +40,16 @@ struct hdcx_scene {
     PackedScene packed;
 };
 
-struct hdcx_tracer {
+// The scene copy (SceneStore) and the wavefront path state (PathPool) are
+// shared with the batch renderer.
+struct hdcx_tracer : SceneStore, PathPool {
     int device = 0;
     hipStream_t stream = nullptr;
     unsigned maxBlocks = 1024;
     int shadingMode = HDCX_SHADING_FUSED;
-    bool geometry = false;
     bool fallbackInit = false;
     bool fallbackZUp = false;
-    DeviceScene scene{};
-    // Scene storage.
-    DeviceBuffer nodes, tris, positions, indices, triMaterial, texcoords, normals, emissive, materials, lights,
-        textures, texels, importance, importancePool;
-    // BVH: builder policy, the device builder's workspace, refit order and
-    // the triangles (vertex count, index buffer) the BVH was built over.
-    hdcx_bvh_options bvhOptions{HDCX_BVH_DEVICE_LBVH, 2048U, 1U, 0U};
-    hdcx_bvh_info bvhInfo{};
-    DeviceBvhBuilder lbvh;
-    RefitPlan refitPlan;
-    std::vector<std::uint32_t> builtIndices;
-    std::uint32_t builtVertexCount = 0;
-    // Path state.
-    DeviceBuffer rayO, rayD, hit, rng, wavelength, throughput, radiance, flags, mis, medium, shadowO, shadowD,
-        shadowLight, queue0, queue1, shadowQueue, counters, totals, traversalCounters, lambda0, spread;
-    std::uint32_t capacity = 0;
+    DeviceBuffer traversalCounters;
     // Film.
     DeviceBuffer filmMean, filmBatch, filmOut;
     std::uint32_t filmWidth = 0, filmHeight = 0;
@@ -104,7 +67,6 @@ struct hdcx_tracer {
     DeviceBuffer rcChunkQueries, rcCounters, rcRecords, rcRecordThroughput, rcSamples;
     std::uint32_t rcFrame = 0;
     bool rcCountersReady = false;
-    float sceneLo[3] = {0, 0, 0}, sceneInvExtent[3] = {1, 1, 1};
     // Instrumentation.
     std::uint32_t instrumentation = 0;
     hdcx_kernel_stats stats{};
@@ -248,58 +210,6 @@ struct hdcx_tracer {
         return rcStats.last_loss;
     }
 
-    PathBuffers Paths()
-    {
-        PathBuffers pb{};
-        pb.rayO = rayO.as<float4>();
-        pb.rayD = rayD.as<float4>();
-        pb.hit = hit.as<float4>();
-        pb.rng = rng.as<std::uint32_t>();
-        pb.wavelength = wavelength.as<float>();
-        pb.throughput = throughput.as<float>();
-        pb.radiance = radiance.as<float>();
-        pb.flags = flags.as<std::uint32_t>();
-        pb.mis = mis.as<float2>();
-        pb.medium = medium.as<float4>();
-        pb.shadowO = shadowO.as<float4>();
-        pb.shadowD = shadowD.as<float4>();
-        pb.shadowLight = shadowLight.as<std::uint32_t>();
-        pb.queue[0] = queue0.as<std::uint32_t>();
-        pb.queue[1] = queue1.as<std::uint32_t>();
-        pb.shadowQueue = shadowQueue.as<std::uint32_t>();
-        pb.counters = counters.as<std::uint32_t>();
-        pb.totals = totals.as<unsigned long long>();
-        pb.lambda0 = lambda0.as<float>();
-        pb.spread = spread.as<float2>();
-        pb.capacity = capacity;
-        return pb;
-    }
-
-    void EnsurePaths(std::uint32_t paths)
-    {
-        if (paths <= capacity) return;
-        const std::size_t n = paths;
-        rayO.Allocate(n * 16);
-        rayD.Allocate(n * 16);
-        hit.Allocate(n * 16);
-        rng.Allocate(n * 4);
-        wavelength.Allocate(n * 4);
-        throughput.Allocate(n * 4);
-        radiance.Allocate(n * 4);
-        flags.Allocate(n * 4);
-        mis.Allocate(n * 8);
-        medium.Allocate(n * 16);
-        shadowO.Allocate(n * 16 * kShadowSlots);
-        shadowD.Allocate(n * 16 * kShadowSlots);
-        shadowLight.Allocate(n * 4 * kShadowSlots);
-        queue0.Allocate(n * 4);
-        queue1.Allocate(n * 4);
-        shadowQueue.Allocate(n * 4);
-        lambda0.Allocate(n * 4);
-        spread.Allocate(n * 8);
-        capacity = paths;
-    }
-
     void EnsureFilm(std::uint32_t w, std::uint32_t h)
     {
         if (w == filmWidth && h == filmHeight && filmMean.bytes()) return;
@@ -317,230 +227,14 @@ struct hdcx_tracer {
 namespace hdcx {
 namespace {
 
-void ValidateTextureRef(std::uint32_t ti, std::uint32_t textureCount, const char* what)
-{
-    if (ti == kMissingTexture) return;
-    if (ti & kUdimTextureFlag) {
-        const std::uint32_t base = ti & 0xffU;
-        const std::uint32_t tiles = static_cast<std::uint32_t>(__builtin_popcount((ti >> 8U) & kUdimTileMask));
-        if (base + tiles > textureCount) throw Failure(HDCX_ERR_INVALID_SCENE, std::string(what) + ": UDIM set out of range");
-        return;
-    }
-    if (ti >= textureCount) throw Failure(HDCX_ERR_INVALID_SCENE, std::string(what) + ": texture index out of range");
-}
-
-void ValidateScene(const hdcx_scene_view& v)
-{
-    if (v.triangle_count && (!v.indices || !v.triangle_materials || !v.texcoords || !v.normals || !v.emissive))
-        throw Failure(HDCX_ERR_INVALID_SCENE, "triangle arrays missing");
-    if (v.vertex_count && !v.positions) throw Failure(HDCX_ERR_INVALID_SCENE, "positions missing");
-    if (v.material_count == 0 || !v.materials) throw Failure(HDCX_ERR_INVALID_SCENE, "no materials");
-    if (v.light_count > HDCX_MAX_LIGHTS) throw Failure(HDCX_ERR_INVALID_SCENE, "more than 64 lights");
-    if (!v.lights) throw Failure(HDCX_ERR_INVALID_SCENE, "light table missing");
-    if (v.texture_count > HDCX_MAX_TEXTURES) throw Failure(HDCX_ERR_INVALID_SCENE, "more than 256 textures");
-    if (v.texture_count && !v.textures) throw Failure(HDCX_ERR_INVALID_SCENE, "texture views missing");
-    for (std::uint64_t i = 0; i < static_cast<std::uint64_t>(v.vertex_count) * 4; ++i)
-        if (!std::isfinite(v.positions[i])) throw Failure(HDCX_ERR_INVALID_SCENE, "non-finite vertex position");
-    for (std::uint64_t i = 0; i < static_cast<std::uint64_t>(v.triangle_count) * 3; ++i)
-        if (v.indices[i] >= v.vertex_count) throw Failure(HDCX_ERR_INVALID_SCENE, "vertex index out of range");
-    for (std::uint32_t t = 0; t < v.triangle_count; ++t)
-        if (v.triangle_materials[t] >= v.material_count) throw Failure(HDCX_ERR_INVALID_SCENE, "material index out of range");
-    for (std::uint32_t i = 0; i < v.texture_count; ++i) {
-        const hdcx_texture_view& t = v.textures[i];
-        if (t.width == 0 || t.height == 0 || !t.texels || t.format > HDCX_TEXTURE_RGBA32F)
-            throw Failure(HDCX_ERR_INVALID_SCENE, "invalid texture");
-    }
-    const auto* mats = static_cast<const GpuMaterialRecord*>(v.materials);
-    for (std::uint32_t m = 0; m < v.material_count; ++m) {
-        const U4* slots[6] = {&mats[m].textureIndices0, &mats[m].textureIndices1, &mats[m].textureIndices2,
-                              &mats[m].textureIndices3, &mats[m].textureIndices4, &mats[m].textureIndices5};
-        for (const U4* u : slots) {
-            ValidateTextureRef(u->x, v.texture_count, "material");
-            ValidateTextureRef(u->y, v.texture_count, "material");
-            ValidateTextureRef(u->z, v.texture_count, "material");
-            ValidateTextureRef(u->w, v.texture_count, "material");
-        }
-    }
-    const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
-    for (std::uint32_t l = 0; l < v.light_count; ++l) {
-        ValidateTextureRef(ls[l].textureInfo.x, v.texture_count, "light");
-        if (ls[l].textureInfo.w != kMissingTexture) {
-            if (ls[l].textureInfo.w >= v.texture_count ||
-                v.textures[ls[l].textureInfo.w].format != HDCX_TEXTURE_RGBA32F)
-                throw Failure(HDCX_ERR_INVALID_SCENE, "light importance texture invalid");
-        }
-    }
-}
-
-void UploadHostBvh(hdcx_tracer* t, const Bvh4& bvh, int builder)
-{
-    if (bvh.maxStack > static_cast<std::uint32_t>(kMaxStack))
-        throw Failure(HDCX_ERR_INVALID_SCENE, "BVH deeper than the traversal stack");
-    t->nodes.Upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), t->stream);
-    t->tris.Upload(bvh.triangles.data(), bvh.triangles.size() * sizeof(float), t->stream);
-    MakeRefitPlan(bvh.nodes, t->stream, t->refitPlan);
-    t->bvhInfo = hdcx_bvh_info{};
-    t->bvhInfo.node_count = static_cast<std::uint32_t>(bvh.nodes.size());
-    t->bvhInfo.slot_count = static_cast<std::uint32_t>(bvh.triangles.size() / 12);
-    t->bvhInfo.max_stack = bvh.maxStack;
-    t->bvhInfo.depth = bvh.depth;
-    t->bvhInfo.builder = builder;
-}
-
-// Builds the BVH of the uploaded positions / indices with the configured
-// builder. The device builder falls back to the host one for a single
-// triangle or a tree deeper than the traversal stack.
-void BuildSceneBvh(hdcx_tracer* t, const hdcx_scene_view& v, const float lo[3], const float hi[3])
-{
-    const std::uint32_t n = v.triangle_count;
-    int builder = t->bvhOptions.builder;
-    if (builder != HDCX_BVH_HOST_SAH && n < 2) builder = HDCX_BVH_HOST_SAH;
-    if (builder == HDCX_BVH_DEVICE_LBVH) {
-        DeviceBvhStats st;
-        t->lbvh.Build(t->positions.as<float4>(), t->indices.as<std::uint32_t>(), n, lo, hi, t->bvhOptions.top_clusters,
-                      t->stream, t->nodes, t->tris, t->refitPlan, st);
-        if (st.maxStack <= static_cast<std::uint32_t>(kMaxStack)) {
-            t->bvhInfo = hdcx_bvh_info{};
-            t->bvhInfo.node_count = st.nodeCount;
-            t->bvhInfo.slot_count = n;
-            t->bvhInfo.max_stack = st.maxStack;
-            t->bvhInfo.depth = st.depth;
-            t->bvhInfo.builder = HDCX_BVH_DEVICE_LBVH;
-            t->bvhInfo.top_clusters = st.topClusters;
-            return;
-        }
-        builder = HDCX_BVH_HOST_SAH;
-    }
-    if (builder == HDCX_BVH_SERIAL_LBVH) {
-        LbvhBuildInfo info;
-        Bvh4 bvh = BuildBvh4LbvhSerial(v.positions, v.indices, n, t->bvhOptions.top_clusters, &info);
-        ValidateBvh4(bvh, n);
-        if (bvh.maxStack <= static_cast<std::uint32_t>(kMaxStack)) {
-            UploadHostBvh(t, bvh, HDCX_BVH_SERIAL_LBVH);
-            t->bvhInfo.top_clusters = info.topClusters;
-            return;
-        }
-    }
-    Bvh4 bvh = BuildBvh4Host(v.positions, v.indices, n);
-    ValidateBvh4(bvh, n);
-    UploadHostBvh(t, bvh, HDCX_BVH_HOST_SAH);
-}
-
 void SetScene(hdcx_tracer* t, const hdcx_scene_view* view)
 {
     t->Activate();
-    const bool hadGeometry = t->geometry;
-    t->geometry = false;
-    if (!view || view->vertex_count == 0 || view->triangle_count == 0) {
-        t->builtIndices.clear();
-        t->builtVertexCount = 0;
-        return;
-    }
-    ValidateScene(*view);
-    const hdcx_scene_view& v = *view;
-    hipStream_t st = t->stream;
-
-    float lo[3], hi[3];
-    VertexBounds(v.positions, v.vertex_count, lo, hi);
-    for (int k = 0; k < 3; ++k) {
-        t->sceneLo[k] = lo[k];
-        t->sceneInvExtent[k] = hi[k] > lo[k] ? 1.0F / (hi[k] - lo[k]) : 1.0F;
-    }
-    const bool refit = t->bvhOptions.refit != 0U && hadGeometry && t->builtVertexCount == v.vertex_count &&
-                       t->builtIndices.size() == static_cast<std::size_t>(v.triangle_count) * 3 &&
-                       std::memcmp(t->builtIndices.data(), v.indices, t->builtIndices.size() * 4) == 0;
-    t->positions.Upload(v.positions, static_cast<std::size_t>(v.vertex_count) * 16, st);
-    t->indices.Upload(v.indices, static_cast<std::size_t>(v.triangle_count) * 12, st);
-    const auto buildStart = std::chrono::steady_clock::now();
-    if (refit) {
-        RefitBvh4(t->positions.as<float4>(), t->indices.as<std::uint32_t>(), t->bvhInfo.slot_count,
-                  t->nodes.as<Bvh4Node>(), t->tris.as<float4>(), t->refitPlan, st);
-        ++t->bvhInfo.refits;
-    } else {
-        BuildSceneBvh(t, v, lo, hi);
-        t->builtIndices.assign(v.indices, v.indices + static_cast<std::size_t>(v.triangle_count) * 3);
-        t->builtVertexCount = v.vertex_count;
-    }
-    t->bvhInfo.build_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - buildStart).count();
-    t->triMaterial.Upload(v.triangle_materials, static_cast<std::size_t>(v.triangle_count) * 4, st);
-    t->texcoords.Upload(v.texcoords, static_cast<std::size_t>(v.triangle_count) * 24, st);
-    t->normals.Upload(v.normals, static_cast<std::size_t>(v.triangle_count) * 48, st);
-    t->emissive.Upload(v.emissive, static_cast<std::size_t>(v.triangle_count) * 16, st);
-    t->materials.Upload(v.materials, static_cast<std::size_t>(v.material_count) * HDCX_MATERIAL_BYTES, st);
-    const std::uint32_t lightRecords = std::max<std::uint32_t>(v.light_count, 1);
-    t->lights.Upload(v.lights, static_cast<std::size_t>(lightRecords) * HDCX_LIGHT_BYTES, st);
-
-    // Texel pool (16-byte aligned entries).
-    std::vector<TexDesc> descs(std::max<std::uint32_t>(v.texture_count, 1));
-    std::size_t poolBytes = 0;
-    for (std::uint32_t i = 0; i < v.texture_count; ++i) {
-        const hdcx_texture_view& tv = v.textures[i];
-        const std::size_t bytes = static_cast<std::size_t>(tv.width) * tv.height * (tv.format == HDCX_TEXTURE_RGBA32F ? 16 : 4);
-        descs[i] = TexDesc{poolBytes, tv.width, tv.height, tv.format, 0};
-        poolBytes += (bytes + 15) & ~static_cast<std::size_t>(15);
-    }
-    std::vector<unsigned char> pool(std::max<std::size_t>(poolBytes, 16));
-    for (std::uint32_t i = 0; i < v.texture_count; ++i) {
-        const hdcx_texture_view& tv = v.textures[i];
-        const std::size_t bytes = static_cast<std::size_t>(tv.width) * tv.height * (tv.format == HDCX_TEXTURE_RGBA32F ? 16 : 4);
-        std::memcpy(pool.data() + descs[i].offset, tv.texels, bytes);
-    }
-    t->textures.Upload(descs.data(), descs.size() * sizeof(TexDesc), st);
-    t->texels.Upload(pool.data(), pool.size(), st);
-
-    // Dome importance tables re-laid out per light: marginal CDF column,
-    // conditional CDF and pdf planes of the packed importance texture.
-    const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
-    std::vector<ImportanceDesc> imp(lightRecords, ImportanceDesc{0, 0, 0, 0, 0});
-    std::vector<float> impPool;
-    for (std::uint32_t l = 0; l < v.light_count; ++l) {
-        const std::uint32_t ti = ls[l].textureInfo.w;
-        if (ls[l].textureInfo.z != 0U || ti == kMissingTexture) continue;
-        const hdcx_texture_view& tv = v.textures[ti];
-        const float* px = static_cast<const float*>(tv.texels);
-        ImportanceDesc d{};
-        d.width = tv.width;
-        d.height = tv.height;
-        d.marginal = impPool.size();
-        for (std::uint32_t y = 0; y < tv.height; ++y) impPool.push_back(px[(static_cast<std::size_t>(y) * tv.width) * 4 + 1]);
-        d.conditional = impPool.size();
-        for (std::size_t p = 0; p < static_cast<std::size_t>(tv.width) * tv.height; ++p) impPool.push_back(px[p * 4 + 0]);
-        d.pdf = impPool.size();
-        for (std::size_t p = 0; p < static_cast<std::size_t>(tv.width) * tv.height; ++p) impPool.push_back(px[p * 4 + 2]);
-        imp[l] = d;
-    }
-    if (impPool.empty()) impPool.push_back(0.0F);
-    t->importance.Upload(imp.data(), imp.size() * sizeof(ImportanceDesc), st);
-    t->importancePool.Upload(impPool.data(), impPool.size() * sizeof(float), st);
-    Check(hipStreamSynchronize(st), "scene upload");
-
-    DeviceScene& s = t->scene;
-    s.nodes = t->nodes.as<Bvh4Node>();
-    s.tris = t->tris.as<float4>();
-    s.positions = t->positions.as<float4>();
-    s.indices = t->indices.as<std::uint32_t>();
-    s.triMaterial = t->triMaterial.as<std::uint32_t>();
-    s.texcoords = t->texcoords.as<float2>();
-    s.normals = t->normals.as<float4>();
-    s.emissive = t->emissive.as<float4>();
-    s.materials = t->materials.as<GpuMaterialRecord>();
-    s.lights = t->lights.as<GpuLightRecord>();
-    s.textures = t->textures.as<TexDesc>();
-    s.texels = t->texels.as<unsigned char>();
-    s.importance = t->importance.as<ImportanceDesc>();
-    s.importancePool = t->importancePool.as<float>();
-    s.primitiveCount = v.triangle_count;
-    s.lightCount = v.light_count;
-    s.opaque = v.opaque ? 1U : 0U;
-    s.hasEmissive = v.emissive[(static_cast<std::size_t>(v.triangle_count) - 1) * 4] > 0.0F ? 1U : 0U;
-    s.textureCount = v.texture_count;
-    s.nodeCount = t->bvhInfo.node_count;
+    UploadScene(t, view, t->stream);
     t->stats.node_bytes = sizeof(Bvh4Node);
     t->stats.triangle_bytes = 48;
     t->stats.bvh_nodes = t->bvhInfo.node_count;
     t->stats.bvh_triangles = t->bvhInfo.slot_count;
-    t->geometry = true;
 }
 
 void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32_t h, std::uint32_t sampleIndex,
@@ -559,10 +253,7 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     t->EnsureFilm(w, h);
     hipStream_t st = t->stream;
     Check(LaunchZero(t->filmBatch.as<double4>(), pixels, t->maxBlocks, st), "zero batch");
-    if (!t->counters.bytes()) {
-        t->counters.Allocate(64);
-        t->totals.Allocate(64);
-        Check(hipMemsetAsync(t->totals.as<void>(), 0, 64, st), "memset totals");
+    if (!t->traversalCounters.bytes()) {
         t->traversalCounters.Allocate(sizeof(TraversalCounters) * 2);
         Check(hipMemsetAsync(t->traversalCounters.as<void>(), 0, sizeof(TraversalCounters) * 2, st), "memset counters");
     }
@@ -642,7 +333,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
                     return LaunchShade(t->scene, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, cp, paths, t->maxBlocks, st);
                 });
-                if (cache && b >= cp.terminateBounce) {
+                // Queries come from the terminate vertex, or from any later
+                // one when the spread rule is on.
+                if (cache && (b == cp.terminateBounce || (b > cp.terminateBounce && cp.spreadThreshold > 0.0F))) {
                     // Queries of this bounce: answered, then the queue is emptied.
                     t->Timed(HDCX_KERNEL_CACHE, [&] {
                         return LaunchRcInfer(t->RcInferNet(), cp.queries, cp.queryCount, paths, nullptr, pb.radiance, pb.throughput,
@@ -758,7 +451,7 @@ Bvh4 BuildOnDevice(const hdcx_scene_view& v, std::uint32_t topClusters, std::uin
 
 extern "C" {
 
-const char* hdcx_last_error(void) { return hdcx::gLastError.c_str(); }
+const char* hdcx_last_error(void) { return hdcx::LastError().c_str(); }
 
 int hdcx_create(int device_ordinal, hdcx_tracer** out_tracer)
 {

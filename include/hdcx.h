@@ -239,6 +239,56 @@ typedef struct hdcx_bvh_options {
 int hdcx_set_bvh_options(hdcx_tracer* tracer, const hdcx_bvh_options* options);
 int hdcx_get_bvh_info(hdcx_tracer* tracer, hdcx_bvh_info* out_info);
 
+/* ---- Batched multi-scene rendering (no counterpart in the reference: one
+ * VulkanPathTracer renders one scene per Render call). Many independent
+ * scenes, each with its own BVH, camera, image size and sample count,
+ * render together: their pixels are cut into tiles (all of a pixel's
+ * samples in one tile) and packed into waves of at most wave_paths lane
+ * paths; one wave runs raygen -> (extend -> shade -> shadow) x bounces ->
+ * film as single launches over every scene at once (each path carries its
+ * scene index), and the per-wave launch sequence is captured once in a
+ * hipGraph and replayed for every wave (the wave index is a device counter).
+ * Each scene's film follows hdcx_render's progressive-mean semantics. */
+typedef struct hdcx_batch hdcx_batch;
+
+typedef struct hdcx_batch_tuning {
+    uint32_t wave_paths;  /* lane paths per wave (path-pool size) */
+    uint32_t tile_pixels; /* pixels per tile (fewer when samples * 3 * tile > wave_paths) */
+    uint32_t max_blocks;  /* workgroups per stage launch (128 threads each) */
+    uint32_t use_graph;   /* nonzero: replay the captured per-wave graph */
+} hdcx_batch_tuning;
+
+typedef struct hdcx_batch_stats {
+    uint64_t pixel_samples;  /* last hdcx_batch_render */
+    uint64_t lane_paths;
+    uint32_t waves;
+    uint32_t graph_launches;
+    double milliseconds;     /* wall time of the last hdcx_batch_render (device work incl.) */
+} hdcx_batch_stats;
+
+int hdcx_batch_create(int device_ordinal, hdcx_batch** out_batch);
+int hdcx_batch_destroy(hdcx_batch* batch);
+/* Uploads a scene (as hdcx_set_scene, with the BVH builder from options or
+ * the default when NULL) and returns its index. */
+int hdcx_batch_add_scene(hdcx_batch* batch, const hdcx_scene_view* scene, const hdcx_bvh_options* options,
+                         uint32_t* out_index);
+/* Camera, image size and samples per pixel per hdcx_batch_render call. */
+int hdcx_batch_set_view(hdcx_batch* batch, uint32_t scene, const hdcx_camera* camera, uint32_t width,
+                        uint32_t height, uint32_t samples);
+int hdcx_batch_set_tuning(hdcx_batch* batch, const hdcx_batch_tuning* tuning);
+int hdcx_batch_get_tuning(hdcx_batch* batch, hdcx_batch_tuning* out_tuning);
+/* Times candidate tunings on one sample per pixel of every view and keeps
+ * the fastest. With cache_path, a result stored there for the same device
+ * and workload is reused, and a new one is appended. The films hold the
+ * trial renders afterwards (render at sample_index 0 to restart them). */
+int hdcx_batch_autotune(hdcx_batch* batch, uint32_t max_bounces, const char* cache_path,
+                        hdcx_batch_tuning* out_tuning);
+/* Renders samples [sample_index, sample_index + samples) of every view. */
+int hdcx_batch_render(hdcx_batch* batch, uint32_t sample_index, uint32_t max_bounces);
+/* Running mean of a scene's film: width * height RGBA32F, row 0 bottom. */
+int hdcx_batch_read_film(hdcx_batch* batch, uint32_t scene, float* out_rgba);
+int hdcx_batch_get_stats(hdcx_batch* batch, hdcx_batch_stats* out_stats);
+
 /* ---- Radiance cache (no counterpart in the reference: its neural
  * rendering plan is unimplemented, docs/neural-rendering-plan.md:32-48).
  * An MLP (128 encoded inputs -> 5 x 64 ReLU -> 3) trained online on the
