@@ -178,6 +178,68 @@ def equal_time(hx, tracer, cam, plain_frame: np.ndarray, plain_ms: float, warmup
     return out
 
 
+def batch_workload(hx, threads: int, frames: int = 1, graph: bool = True) -> dict:
+    """64 independent seeded mixed-material scenes (triangle targets
+    log-uniform in [1e3, 5e5], square images of 128..1024 pixels, 8..64 spp,
+    8 bounces) rendered together by the batch renderer: autotuned wave /
+    tile / grid sizes, per-wave hipGraph replay. Reports aggregate Mpaths/s
+    and each scene's rel-MSE against the FP64 oracle on 32 pixels (four rows
+    of eight) at the same seeds."""
+    import oracle
+
+    rng = np.random.default_rng(2024)
+    specs = []
+    for i in range(64):
+        tris = int(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        res = int(rng.choice([128, 256, 384, 512, 640, 768, 896, 1024]))
+        spp = int(rng.integers(8, 65))
+        specs.append((i, tris, res, spp))
+    b = hx.BatchRenderer(0)
+    scenes = []
+    t0 = time.perf_counter()
+    for i, tris, res, spp in specs:
+        scene = hx.ProceduralScene("mixed", i, tris, 256)
+        scenes.append(scene)
+        b.add_scene(scene, hx.BVH_DEVICE_LBVH, 2048)
+        b.set_view(i, scene.camera(res, res), res, res, spp)
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if graph:
+        tuning = b.autotune(BOUNCES, None)
+    else:
+        # Plain launches (for profilers that do not follow graph launches),
+        # with the sizes the autotuner picks on this workload.
+        b.set_tuning(3 << 21, 16384, 4096, False)
+        tuning = b.tuning()
+    tune_s = time.perf_counter() - t0
+    b.render(0, BOUNCES)  # warm-up frame
+    times = []
+    for f in range(frames):
+        t0 = time.perf_counter()
+        b.render(0, BOUNCES)
+        times.append(time.perf_counter() - t0)
+    stats = b.stats()
+    errs = []
+    for (i, tris, res, spp), scene in zip(specs, scenes):
+        film = b.read_film(i)
+        rows = np.linspace(res // 8, res - res // 8, 4).astype(np.uint32)
+        col_step = res // 8
+        ref = oracle.OracleScene(scene.view).render(scene.camera(res, res), res, res, 0, BOUNCES, spp, precision=64,
+                                                    rows=rows, col_step=col_step, threads=threads) / spp
+        x = film[rows.astype(np.int64)][:, ::col_step, :3].astype(np.float64)
+        errs.append(float(np.mean((x - ref) ** 2 / (ref * ref + 0.01))))
+    b.close()
+    ms = min(times) * 1e3
+    return {"workload": "64 mixed-material scenes, 1e3..5e5 triangles (log-uniform), 128^2..1024^2, 8..64 spp, "
+                        "8 bounces, one batch (seeded)",
+            "scenes": 64, "triangles": int(sum(sc.triangle_count for sc in scenes)),
+            "pixel_samples": stats["pixel_samples"], "ms_per_frame": ms,
+            "mpaths_per_s": stats["pixel_samples"] / (ms * 1e-3) / 1e6, "waves": stats["waves"],
+            "graph_launches": stats["graph_launches"], "tuning": tuning, "autotune_s": tune_s, "setup_s": setup_s,
+            "rel_mse_vs_fp64_oracle": {"max": max(errs), "median": float(np.median(errs)), "per_scene": errs,
+                                       "sample": "32 pixels per scene (4 rows x 8 columns), same seeds"}}
+
+
 def copy_calibration() -> int:
     """A plain copy of known size: 256 MiB read + 256 MiB written per launch.
     Its FETCH_SIZE / WRITE_SIZE readings give the scale that turns those
@@ -205,11 +267,22 @@ def main() -> int:
     ap.add_argument("--cache-warmup", type=int, default=50)
     ap.add_argument("--bvh", choices=("lbvh", "sah"), default="sah",
                     help="BVH builder: device LBVH with SAH top levels, or host binned SAH")
+    ap.add_argument("--batch", action="store_true",
+                    help="also render the 64-scene batch workload (adds about a minute)")
+    ap.add_argument("--batch-only", action="store_true", help="only the 64-scene batch workload")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch workload with plain launches instead of graph replay, no autotuning")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel 4 times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
+    if args.batch_only:
+        import hdcodex_amd as hx
+
+        print(json.dumps({"batch": batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph)}),
+              flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -247,6 +320,11 @@ def main() -> int:
         log(f"step {i}: {times[-1] * 1e3:.1f} ms")
     timed = tracer.kernel_stats()
     frame = tracer.read_film(WIDTH, HEIGHT)
+    # The same frame with the RGBA32F result copied to host memory (the
+    # boundary's output buffer), outside the timed steps.
+    t0 = time.perf_counter()
+    tracer.render(cam, WIDTH, HEIGHT, 0, BOUNCES, SPP, readback=True)
+    pcie_s = time.perf_counter() - t0
 
     # Counting pass (separate instantiation of the traversal kernels).
     tracer.reset_kernel_stats()
@@ -302,6 +380,8 @@ def main() -> int:
                                f"dome, {WIDTH}x{HEIGHT}, {SPP} spp x 3 lanes, {BOUNCES} bounces, plain path tracer",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "bounces": BOUNCES, "radiance_cache": False},
         "mrays_per_s": mrays,
+        "pcie_inclusive": {"ms_per_step": pcie_s * 1e3, "mpaths_per_s": paths / pcie_s / 1e6,
+                           "note": "same frame with the 33 MB RGBA32F readback to host memory"},
         "lane_paths_per_s_M": 3 * mpaths,
         "set_scene_s": set_scene_s,
         "bvh": {"builder": args.bvh, "build_ms": bvh["build_ms"], "nodes": bvh["node_count"],
@@ -320,6 +400,9 @@ def main() -> int:
         log("parity vs FP64 oracle (subset)")
         result["parity"] = parity(scene, cam, frame)
         result["parity_rel_mse_vs_fp64_oracle"] = result["parity"]["rel_mse_vs_fp64_oracle"]
+    if args.batch:
+        log("batch workload")
+        result["batch"] = batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         threads = min(os.cpu_count() or 1, 16)
