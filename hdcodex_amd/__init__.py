@@ -38,7 +38,7 @@ EXPORTED_SYMBOLS = (
     "hdcx_last_error", "hdcx_create", "hdcx_destroy", "hdcx_set_scene", "hdcx_validate_scene", "hdcx_set_shading_mode",
     "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_has_geometry", "hdcx_trace_closest",
     "hdcx_trace_occluded", "hdcx_scene_create_procedural", "hdcx_scene_view_get", "hdcx_scene_camera_get",
-    "hdcx_scene_destroy", "hdcx_bvh_build_host", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
+    "hdcx_scene_destroy", "hdcx_scene_flatten", "hdcx_bvh_build_host", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
 )
 
 
@@ -73,6 +73,8 @@ class SceneView(ctypes.Structure):
         ("materials", c_void_p), ("lights", c_void_p),
         ("texture_count", c_uint32), ("opaque", c_uint32),
         ("textures", POINTER(TextureView)),
+        ("prototype_count", c_uint32), ("prototypes", c_void_p),
+        ("instance_count", c_uint32), ("instances", c_void_p),
     ]
 
 
@@ -136,6 +138,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_scene_view_get.argtypes = [c_void_p, POINTER(SceneView)]
     lib.hdcx_scene_camera_get.argtypes = [c_void_p, c_uint32, c_uint32, POINTER(Camera)]
     lib.hdcx_scene_destroy.argtypes = [c_void_p]
+    lib.hdcx_scene_flatten.argtypes = [c_void_p, POINTER(c_void_p)]
     lib.hdcx_bvh_build_host.argtypes = [POINTER(SceneView), POINTER(BvhInfo), c_void_p, c_void_p]
     lib.hdcx_set_instrumentation.argtypes = [c_void_p, c_uint32]
     lib.hdcx_get_kernel_stats.argtypes = [c_void_p, POINTER(KernelStats)]
@@ -168,6 +171,21 @@ class ProceduralScene:
         cam = Camera()
         _check(library().hdcx_scene_camera_get(self._h, width, height, ctypes.byref(cam)))
         return cam
+
+    def flattened(self) -> "ProceduralScene":
+        """The same scene with every instance written out as triangles."""
+        lib = library()
+        out = ProceduralScene.__new__(ProceduralScene)
+        out._h = c_void_p()
+        _check(lib.hdcx_scene_flatten(self._h, ctypes.byref(out._h)))
+        out.name = self.name + "/flattened"
+        out.view = SceneView()
+        _check(lib.hdcx_scene_view_get(out._h, ctypes.byref(out.view)))
+        return out
+
+    @property
+    def instance_count(self) -> int:
+        return int(self.view.instance_count)
 
     @property
     def triangle_count(self) -> int:

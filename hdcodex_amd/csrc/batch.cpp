@@ -45,7 +45,7 @@ struct Plan {
 struct GraphKey {
     const void* pool = nullptr;
     const void* tables[7] = {};
-    std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0;
+    std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0, twoLevel = 0;
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 
@@ -195,17 +195,19 @@ struct hdcx_batch : PathPool {
         bw.wavePixels = wavePixels.as<std::uint32_t>();
         bw.cursor = cursor.as<std::uint32_t>();
         const DeviceScene* table = sceneTable.as<DeviceScene>();
+        bool twoLevel = false;
+        for (const DeviceScene& d : sceneHost) twoLevel = twoLevel || d.instanceCount > 0;
         const PathBuffers pb = Paths();
         const std::uint32_t maxPaths = plan.maxPaths, maxPixels = plan.maxPixels;
         auto enqueueWave = [&] {
             Check(LaunchRaygenBatch(pb, bw, maxPaths, maxBlocks, st), "batch raygen");
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
-                Check(LaunchExtendBatch(table, pb, in, maxPaths, maxBlocks, nullptr, st), "batch extend");
-                Check(LaunchShadeBatch(table, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, maxPaths, maxBlocks,
-                                       st),
+                Check(LaunchExtendBatch(table, twoLevel, pb, in, maxPaths, maxBlocks, nullptr, st), "batch extend");
+                Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, maxPaths,
+                                       maxBlocks, st),
                       "batch shade");
-                Check(LaunchShadowBatch(table, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
+                Check(LaunchShadowBatch(table, twoLevel, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
             }
             Check(LaunchFilmBatch(pb, bw, maxPixels, maxBlocks, st), "batch film");
             Check(LaunchAdvanceWave(bw, st), "batch cursor");
@@ -221,6 +223,7 @@ struct hdcx_batch : PathPool {
             key.maxBlocks = maxBlocks;
             key.maxPaths = maxPaths;
             key.maxPixels = maxPixels;
+            key.twoLevel = twoLevel ? 1U : 0U;
             if (!graphExec || !(key == graphKey)) {
                 DropGraph();
                 Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");

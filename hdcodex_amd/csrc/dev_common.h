@@ -7,6 +7,7 @@
 #include "../../include/hdcx.h"
 #include "bvh.hpp"
 #include "records.h"
+#include "instance_math.h"
 
 namespace hdcx {
 
@@ -49,7 +50,15 @@ struct DeviceScene {
     std::uint32_t hasEmissive;
     std::uint32_t textureCount;
     std::uint32_t nodeCount;
-    std::uint32_t pad;
+    // Two-level scenes (instancing); instanceCount == 0 for one level.
+    std::uint32_t instanceCount;
+    std::uint32_t drawnTriangles;      // primitive ids below this are drawn triangles
+    std::uint32_t tlasRoot;            // instance tree root (node 0)
+    std::uint32_t drawnRoot;           // BLAS of the drawn triangles, kEmptyChild when none
+    const InstanceRecord* instances;
+    const PrototypeRecord* prototypes;
+    const std::uint32_t* instanceBase; // flattened id of each instance's first triangle
+    const float* instancePad;          // per instance: 4 floats bounding the object-space slab padding
 };
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,329 @@
+// Two-level traversal (instancing). The instance tree (TLAS) is a BVH4 whose
+// leaves are items: an instance, or (item == instanceCount) the BVH of the
+// drawn triangles. Entering an instance pushes an exit marker and switches
+// the box tests to the object-space ray; triangles are still tested in world
+// space, on vertices transformed with the same FP32 formula the host uses to
+// flatten the scene, so hits equal those of the flattened scene bit for bit.
+// The object-space box tests widen every box by a bound on the rounding of
+// the ray transform and of the vertex transform, so no box that contains a
+// world-space hit is culled.
+#pragma once
+
+#include "dev_traverse.h"
+
+namespace hdcx {
+
+constexpr std::uint32_t kInstanceExit = 0xfffffffeU;  // stack marker: back to the instance tree
+
+struct BoxRay {
+    float ox, oy, oz;
+    float ix, iy, iz;
+    float pad;  // widening of every box (object space)
+};
+
+__device__ __forceinline__ BoxRay worldBoxRay(const RayPre& r) { return BoxRay{r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, 0.0f}; }
+
+// Object-space ray of instance i and the box widening for hits up to tmax.
+__device__ __forceinline__ BoxRay objectBoxRay(const DeviceScene& s, std::uint32_t i, const RayPre& r, float tmax)
+{
+    const InstanceRecord& in = s.instances[i];
+    float o[3], d[3];
+    TransformPoint(in.toObject, r.ox, r.oy, r.oz, o);
+    {
+#pragma clang fp contract(off)
+        for (int k = 0; k < 3; ++k) d[k] = (in.toObject[4 * k] * r.dx + in.toObject[4 * k + 1] * r.dy) + in.toObject[4 * k + 2] * r.dz;
+    }
+    const float4 p = reinterpret_cast<const float4*>(s.instancePad)[i];  // invNorm, norm, |T|, object extent
+    const float oInf = fmaxf(fmaxf(fabsf(r.ox), fabsf(r.oy)), fabsf(r.oz));
+    const float dInf = fmaxf(fmaxf(fabsf(r.dx), fabsf(r.dy)), fabsf(r.dz));
+    const float tm = fminf(tmax, 1e30f);
+    const float pad = 16.0f * 5.9604645e-8f * (p.x * (oInf + 2.0f * p.z + tm * dInf + p.y * p.w) + p.w) + 1e-6f * p.w;
+    return BoxRay{o[0], o[1], o[2], safeRcp(d[0]), safeRcp(d[1]), safeRcp(d[2]), pad};
+}
+
+__device__ __forceinline__ void testNodeBox(const Bvh4Node* __restrict__ node, const BoxRay& r, float tmin, float tmax,
+                                            float tn[4], std::uint32_t ref[4])
+{
+    const float4* n4 = reinterpret_cast<const float4*>(node);
+    const float4 lox = n4[0], hix = n4[1], loy = n4[2], hiy = n4[3], loz = n4[4], hiz = n4[5];
+    const uint4 ch = reinterpret_cast<const uint4*>(node)[6];
+    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
+    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    const std::uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float tx0 = (lx[i] - r.pad - r.ox) * r.ix, tx1 = (hx[i] + r.pad - r.ox) * r.ix;
+        const float ty0 = (ly[i] - r.pad - r.oy) * r.iy, ty1 = (hy[i] + r.pad - r.oy) * r.iy;
+        const float tz0 = (lz[i] - r.pad - r.oz) * r.iz, tz1 = (hz[i] + r.pad - r.oz) * r.iz;
+        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const bool hit = c[i] != kEmptyChild && tnear <= tfar * 1.00000048f;
+        tn[i] = hit ? tnear : INFINITY;
+        ref[i] = c[i];
+    }
+}
+
+// World-space vertices of leaf slot k (object space, transformed when inside
+// an instance).
+__device__ __forceinline__ void slotWorld(const DeviceScene& s, std::uint32_t k, int inst, float4& a, float4& b, float4& c)
+{
+    const float4* tp = s.tris + static_cast<std::size_t>(k) * 3;
+    a = tp[0];
+    b = tp[1];
+    c = tp[2];
+    if (inst >= 0) {
+        const float* m = s.instances[inst].toWorld;
+        float w[3];
+        TransformPoint(m, a.x, a.y, a.z, w);
+        a.x = w[0]; a.y = w[1]; a.z = w[2];
+        TransformPoint(m, b.x, b.y, b.z, w);
+        b.x = w[0]; b.y = w[1]; b.z = w[2];
+        TransformPoint(m, c.x, c.y, c.z, w);
+        c.x = w[0]; c.y = w[1]; c.z = w[2];
+    }
+}
+
+#define HDCX_SWAP_IF2(a, b)                                       \
+    if (tn[b] < tn[a]) {                                          \
+        const float tt = tn[a]; tn[a] = tn[b]; tn[b] = tt;        \
+        const std::uint32_t rr = ref[a]; ref[a] = ref[b]; ref[b] = rr; \
+    }
+
+// Closest hit over the instance tree, with the same lexicographic lower
+// bound as traceClosest. Primitive ids are flattened ids.
+template <bool kCount>
+__device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin, float tmax, float loT,
+                               std::uint32_t loPrim, HitRec& best, TraversalStack& st, TraversalCounters* ctr)
+{
+    best.t = tmax;
+    best.prim = 0xffffffffU;
+    best.u = best.v = 0.0f;
+    if (s.nodeCount == 0) return false;
+    bool found = false;
+    st.sp = 0;
+    std::uint32_t cur = s.tlasRoot;
+    bool inBlas = false;
+    int inst = -1;
+    std::uint32_t base = 0, first = 0;
+    BoxRay box = worldBoxRay(r);
+    unsigned long long nodes = 0, tris = 0;
+    for (;;) {
+        if (cur == kInstanceExit) {
+            inBlas = false;
+            inst = -1;
+            box = worldBoxRay(r);
+        } else if (cur & kLeafFlag) {
+            const std::uint32_t lf = (cur >> 3) & 0x0fffffffU;
+            if (!inBlas) {
+                // Item of the instance tree: enter its BVH.
+                st.push(kInstanceExit);
+                inBlas = true;
+                if (lf >= s.instanceCount) {
+                    inst = -1;
+                    base = 0;
+                    first = 0;
+                    cur = s.drawnRoot;
+                } else {
+                    inst = static_cast<int>(lf);
+                    const PrototypeRecord pr = s.prototypes[s.instances[lf].prototype];
+                    base = s.instanceBase[lf];
+                    first = pr.firstTriangle;
+                    box = objectBoxRay(s, lf, r, best.t);
+                    cur = pr.root;
+                }
+                continue;
+            }
+            const std::uint32_t count = (cur & 7U) + 1U;
+            for (std::uint32_t j = 0; j < count; ++j) {
+                float4 a, b, c;
+                slotWorld(s, lf + j, inst, a, b, c);
+                const std::uint32_t slotPrim = __float_as_uint(s.tris[static_cast<std::size_t>(lf + j) * 3].w);
+                const std::uint32_t prim = inst >= 0 ? base + (slotPrim - first) : slotPrim;
+                if (kCount) ++tris;
+                float t, u, v;
+                if (intersectTriangle(r, a, b, c, tmin, best.t, t, u, v)) {
+                    const bool aboveLo = loPrim == 0xffffffffU || t > loT || (t == loT && prim > loPrim);
+                    const bool better = !found || t < best.t || (t == best.t && prim < best.prim);
+                    if (aboveLo && better) {
+                        found = true;
+                        best.t = t;
+                        best.prim = prim;
+                        best.u = u;
+                        best.v = v;
+                    }
+                }
+            }
+        } else {
+            if (kCount) ++nodes;
+            float tn[4];
+            std::uint32_t ref[4];
+            testNodeBox(s.nodes + cur, box, tmin, best.t, tn, ref);
+            HDCX_SWAP_IF2(0, 1) HDCX_SWAP_IF2(2, 3) HDCX_SWAP_IF2(0, 2) HDCX_SWAP_IF2(1, 3) HDCX_SWAP_IF2(1, 2)
+            if (tn[0] != INFINITY) {
+                if (tn[3] != INFINITY) st.push(ref[3]);
+                if (tn[2] != INFINITY) st.push(ref[2]);
+                if (tn[1] != INFINITY) st.push(ref[1]);
+                cur = ref[0];
+                continue;
+            }
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    if (kCount && ctr) {
+        atomicAdd(&ctr->nodes, nodes);
+        atomicAdd(&ctr->triangles, tris);
+    }
+    return found;
+}
+
+// Any hit in [tmin, tmax] over the instance tree.
+template <bool kCount>
+__device__ bool traceAny2L(const DeviceScene& s, const RayPre& r, float tmin, float tmax, TraversalStack& st,
+                           TraversalCounters* ctr)
+{
+    if (s.nodeCount == 0) return false;
+    st.sp = 0;
+    std::uint32_t cur = s.tlasRoot;
+    bool inBlas = false, hitAny = false;
+    int inst = -1;
+    BoxRay box = worldBoxRay(r);
+    unsigned long long nodes = 0, tris = 0;
+    for (;;) {
+        if (cur == kInstanceExit) {
+            inBlas = false;
+            inst = -1;
+            box = worldBoxRay(r);
+        } else if (cur & kLeafFlag) {
+            const std::uint32_t lf = (cur >> 3) & 0x0fffffffU;
+            if (!inBlas) {
+                st.push(kInstanceExit);
+                inBlas = true;
+                if (lf >= s.instanceCount) {
+                    inst = -1;
+                    cur = s.drawnRoot;
+                } else {
+                    inst = static_cast<int>(lf);
+                    box = objectBoxRay(s, lf, r, tmax);
+                    cur = s.prototypes[s.instances[lf].prototype].root;
+                }
+                continue;
+            }
+            const std::uint32_t count = (cur & 7U) + 1U;
+            for (std::uint32_t j = 0; j < count; ++j) {
+                float4 a, b, c;
+                slotWorld(s, lf + j, inst, a, b, c);
+                if (kCount) ++tris;
+                float t, u, v;
+                if (intersectTriangle(r, a, b, c, tmin, tmax, t, u, v)) {
+                    hitAny = true;
+                    break;
+                }
+            }
+            if (hitAny) break;
+        } else {
+            if (kCount) ++nodes;
+            float tn[4];
+            std::uint32_t ref[4];
+            testNodeBox(s.nodes + cur, box, tmin, tmax, tn, ref);
+            std::uint32_t next = 0xffffffffU;
+            bool have = false;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (tn[i] != INFINITY) {
+                    if (!have) { next = ref[i]; have = true; }
+                    else st.push(ref[i]);
+                }
+            }
+            if (have) { cur = next; continue; }
+        }
+        if (st.sp == 0) break;
+        cur = st.pop();
+    }
+    if (kCount && ctr) {
+        atomicAdd(&ctr->nodes, nodes);
+        atomicAdd(&ctr->triangles, tris);
+    }
+    return hitAny;
+}
+
+#undef HDCX_SWAP_IF2
+
+// One- or two-level dispatch, fixed at compile time per kernel instance.
+template <bool kCount, bool kTwoLevel>
+__device__ __forceinline__ bool traceClosestS(const DeviceScene& s, const RayPre& r, float tmin, float tmax, float loT,
+                                              std::uint32_t loPrim, HitRec& best, TraversalStack& st,
+                                              TraversalCounters* ctr)
+{
+    if constexpr (kTwoLevel) return traceClosest2L<kCount>(s, r, tmin, tmax, loT, loPrim, best, st, ctr);
+    else return traceClosest<kCount>(s, r, tmin, tmax, loT, loPrim, best, st, ctr);
+}
+template <bool kCount, bool kTwoLevel>
+__device__ __forceinline__ bool traceAnyS(const DeviceScene& s, const RayPre& r, float tmin, float tmax,
+                                          TraversalStack& st, TraversalCounters* ctr)
+{
+    if constexpr (kTwoLevel) return traceAny2L<kCount>(s, r, tmin, tmax, st, ctr);
+    else return traceAny<kCount>(s, r, tmin, tmax, st, ctr);
+}
+
+// Flattened primitive id -> the scene triangle it reads and its instance
+// (-1 for a drawn triangle).
+struct TriRef {
+    std::uint32_t tri;
+    int inst;
+};
+
+template <bool kTwoLevel>
+__device__ __forceinline__ TriRef resolvePrim(const DeviceScene& s, std::uint32_t prim)
+{
+    if (!kTwoLevel || prim < s.drawnTriangles) return TriRef{prim, -1};
+    std::uint32_t lo = 0, hi = s.instanceCount;  // last instance with base <= prim
+    while (hi - lo > 1U) {
+        const std::uint32_t mid = (lo + hi) >> 1;
+        if (s.instanceBase[mid] <= prim) lo = mid;
+        else hi = mid;
+    }
+    const PrototypeRecord pr = s.prototypes[s.instances[lo].prototype];
+    return TriRef{pr.firstTriangle + (prim - s.instanceBase[lo]), static_cast<int>(lo)};
+}
+
+__device__ __forceinline__ void triangleWorld(const DeviceScene& s, TriRef t, V3& p0, V3& p1, V3& p2)
+{
+    const std::uint32_t i0 = s.indices[t.tri * 3], i1 = s.indices[t.tri * 3 + 1], i2 = s.indices[t.tri * 3 + 2];
+    const float4 a = s.positions[i0], b = s.positions[i1], c = s.positions[i2];
+    if (t.inst < 0) {
+        p0 = v3(a);
+        p1 = v3(b);
+        p2 = v3(c);
+        return;
+    }
+    const float* m = s.instances[t.inst].toWorld;
+    float w[3];
+    TransformPoint(m, a.x, a.y, a.z, w);
+    p0 = v3(w[0], w[1], w[2]);
+    TransformPoint(m, b.x, b.y, b.z, w);
+    p1 = v3(w[0], w[1], w[2]);
+    TransformPoint(m, c.x, c.y, c.z, w);
+    p2 = v3(w[0], w[1], w[2]);
+}
+
+__device__ __forceinline__ float4 cornerNormal(const DeviceScene& s, TriRef t, int corner)
+{
+    const float4 n = s.normals[t.tri * 3 + corner];
+    if (t.inst < 0) return n;
+    float w[3];
+    TransformNormal(s.instances[t.inst].normal, n.x, n.y, n.z, w);
+    return make_float4(w[0], w[1], w[2], 0.0f);
+}
+
+__device__ __forceinline__ std::uint32_t triangleMaterial(const DeviceScene& s, TriRef t)
+{
+    if (t.inst >= 0) {
+        const std::uint32_t m = s.instances[t.inst].material;
+        if (m != 0xffffffffU) return m;
+    }
+    return s.triMaterial[t.tri];
+}
+
+} // namespace hdcx

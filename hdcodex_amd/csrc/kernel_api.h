@@ -16,15 +16,15 @@ hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st);
 // Batch renderer: the same stages over a table of scenes (scene of each
-// path in pb.pathScene), the waves' raygen / film reading the device wave
+// path in pb.pathScene; twoLevel when any scene has instances), the waves' raygen / film reading the device wave
 // cursor, and the cursor step. maxWork sizes the grid (as a wave's paths or
 // pixels can only be known on the device).
-hipError_t LaunchExtendBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
+                             std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
+hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
+                            int last, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                              unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
-hipError_t LaunchShadeBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                            std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
-hipError_t LaunchShadowBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
-                             TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
                              hipStream_t st);
 hipError_t LaunchFilmBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,

@@ -8,7 +8,7 @@
 // wave ballot + prefix popcount and one atomic per wave.
 #include "kernel_api.h"
 #include "dev_shading.h"
-#include "dev_traverse.h"
+#include "dev_traverse2.h"
 #include "rcache.h"
 
 namespace hdcx {
@@ -150,10 +150,13 @@ __device__ __forceinline__ std::uint32_t waveSlot(bool want, std::uint32_t* coun
     return want ? base + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL))) : 0xffffffffU;
 }
 
+template <bool k2L>
 __device__ __forceinline__ float materialOpacity(const DeviceScene& s, std::uint32_t prim, float bu, float bv)
 {
-    const GpuMaterialRecord& m = s.materials[s.triMaterial[prim]];
-    const float2 t0 = s.texcoords[prim * 3], t1 = s.texcoords[prim * 3 + 1], t2 = s.texcoords[prim * 3 + 2];
+    const TriRef tr = resolvePrim<k2L>(s, prim);
+    const GpuMaterialRecord& m = s.materials[triangleMaterial(s, tr)];
+    const std::uint32_t tri = tr.tri;
+    const float2 t0 = s.texcoords[tri * 3], t1 = s.texcoords[tri * 3 + 1], t2 = s.texcoords[tri * 3 + 2];
     const float w0 = 1.0f - bu - bv;
     const float u = t0.x * w0 + t1.x * bu + t2.x * bv, v = t0.y * w0 + t1.y * bu + t2.y * bv;
     return clampf(matScalar(s, m.textureIndices1.x, u, v, m.transmissionOpacityIor.y), 0.0f, 1.0f);
@@ -162,7 +165,7 @@ __device__ __forceinline__ float materialOpacity(const DeviceScene& s, std::uint
 // Path-vertex hit of a non-opaque scene: candidates in increasing
 // (t, primitive) order, each confirmed with randomFloat <= opacity
 // (the reference's any-hit loop :1233-1243 with a defined order).
-template <bool kCount>
+template <bool kCount, bool k2L>
 __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint32_t& rng, HitRec& hit,
                                 TraversalStack& st, TraversalCounters* ctr)
 {
@@ -170,8 +173,8 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
     std::uint32_t loPrim = 0xffffffffU;
     for (;;) {
         HitRec h;
-        if (!traceClosest<kCount>(s, r, 0.001f, 10000.0f, loT, loPrim, h, st, ctr)) return false;
-        if (randomFloat(rng) <= materialOpacity(s, h.prim, h.u, h.v)) {
+        if (!traceClosestS<kCount, k2L>(s, r, 0.001f, 10000.0f, loT, loPrim, h, st, ctr)) return false;
+        if (randomFloat(rng) <= materialOpacity<k2L>(s, h.prim, h.u, h.v)) {
             hit = h;
             return true;
         }
@@ -183,7 +186,7 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // ---------------------------------------------------------------------------
 // extend: closest hit for every queued path (:1228-1243).
 // ---------------------------------------------------------------------------
-template <bool kCount, class Scenes>
+template <bool kCount, class Scenes, bool k2L>
 __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in,
                                                       TraversalCounters* ctr)
 {
@@ -206,10 +209,10 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
         HitRec h;
         bool found;
         if (s.opaque) {
-            found = traceClosest<kCount>(s, r, 0.001f, 10000.0f, 0.0f, 0xffffffffU, h, st, ctr);
+            found = traceClosestS<kCount, k2L>(s, r, 0.001f, 10000.0f, 0.0f, 0xffffffffU, h, st, ctr);
         } else {
             std::uint32_t rng = pb.rng[pid];
-            found = traceStochastic<kCount>(s, r, rng, h, st, ctr);
+            found = traceStochastic<kCount, k2L>(s, r, rng, h, st, ctr);
             pb.rng[pid] = rng;
         }
         pb.hit[pid] = make_float4(h.t, __uint_as_float(found ? h.prim : 0xffffffffU), h.u, h.v);
@@ -217,21 +220,23 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
 }
 
 // Closest opaque hit used by the subsurface walk (:1130-1151).
+template <bool k2L>
 __device__ bool opaqueBoundary(const DeviceScene& s, V3 o, V3 d, TraversalStack& st, V3& boundary, V3& bn, float& dist)
 {
     const RayPre r = prepareRay(o.x, o.y, o.z, d.x, d.y, d.z);
     HitRec h;
-    if (!traceClosest<false>(s, r, 0.001f, 10000.0f, 0.0f, 0xffffffffU, h, st, nullptr)) return false;
+    if (!traceClosestS<false, k2L>(s, r, 0.001f, 10000.0f, 0.0f, 0xffffffffU, h, st, nullptr)) return false;
     dist = h.t;
     boundary = o + d * dist;
-    const std::uint32_t i0 = s.indices[h.prim * 3], i1 = s.indices[h.prim * 3 + 1], i2 = s.indices[h.prim * 3 + 2];
-    const V3 p0 = v3(s.positions[i0]), p1 = v3(s.positions[i1]), p2 = v3(s.positions[i2]);
+    V3 p0, p1, p2;
+    triangleWorld(s, resolvePrim<k2L>(s, h.prim), p0, p1, p2);
     bn = normalize(cross(p1 - p0, p2 - p0));
     if (dot(bn, d) > 0.0f) bn = -bn;
     return true;
 }
 
 // randomWalkSubsurface (:1153-1186)
+template <bool k2L>
 __device__ bool randomWalk(const DeviceScene& s, V3 entry, V3 outN, float mfp, float albedo, float aniso,
                            std::uint32_t& rng, TraversalStack& st, V3& exitO, V3& exitD, float& atten)
 {
@@ -244,7 +249,7 @@ __device__ bool randomWalk(const DeviceScene& s, V3 entry, V3 outN, float mfp, f
     for (int step = 0; step < 8; ++step) {
         V3 boundary, bn;
         float bd;
-        if (!opaqueBoundary(s, o, d, st, boundary, bn, bd)) return false;
+        if (!opaqueBoundary<k2L>(s, o, d, st, boundary, bn, bd)) return false;
         const float scatter = -logf(fmaxf(1.0f - randomFloat(rng), 1e-6f)) / sigmaS;
         const bool force = step == 7;
         const float travel = force ? bd : fminf(scatter, bd);
@@ -276,7 +281,7 @@ __device__ __forceinline__ void queueShadow(const PathBuffers& pb, std::uint32_t
 // shadow stage adds it in slot order, which keeps the reference's order of
 // additions (emission, analytic NEE, mesh-light NEE).
 // ---------------------------------------------------------------------------
-template <class Scenes>
+template <class Scenes, bool k2L>
 __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int bounce,
                                                      int lastBounce, CacheParams cp)
 {
@@ -338,15 +343,16 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                     if (!mediumScattered) throughput *= expf(-med.x * distance);
                 }
                 if (!mediumScattered) {
-                    const std::uint32_t i0 = s.indices[prim * 3], i1 = s.indices[prim * 3 + 1], i2 = s.indices[prim * 3 + 2];
-                    const V3 p0 = v3(s.positions[i0]), p1 = v3(s.positions[i1]), p2 = v3(s.positions[i2]);
+                    const TriRef tr = resolvePrim<k2L>(s, prim);
+                    V3 p0, p1, p2;
+                    triangleWorld(s, tr, p0, p1, p2);
                     const V3 gn = normalize(cross(p1 - p0, p2 - p0));
                     const bool front = dot(gn, rd) < 0.0f;
                     const V3 ogn = front ? gn : -gn;
                     const float w0 = 1.0f - bu - bv;
                     V3 normal;
                     {
-                        const float4 n0 = s.normals[prim * 3], n1 = s.normals[prim * 3 + 1], n2 = s.normals[prim * 3 + 2];
+                        const float4 n0 = cornerNormal(s, tr, 0), n1 = cornerNormal(s, tr, 1), n2 = cornerNormal(s, tr, 2);
                         V3 an = v3(n0) * w0 + v3(n1) * bu + v3(n2) * bv;
                         if (dot(an, an) < 1e-12f) an = gn;
                         else {
@@ -355,10 +361,10 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                         }
                         normal = front ? an : -an;
                     }
-                    const float2 t0 = s.texcoords[prim * 3], t1 = s.texcoords[prim * 3 + 1], t2 = s.texcoords[prim * 3 + 2];
+                    const float2 t0 = s.texcoords[tr.tri * 3], t1 = s.texcoords[tr.tri * 3 + 1], t2 = s.texcoords[tr.tri * 3 + 2];
                     const float su = t0.x * w0 + t1.x * bu + t2.x * bv, sv = t0.y * w0 + t1.y * bu + t2.y * bv;
                     const V3 hp = ro + rd * distance;
-                    const GpuMaterialRecord& m = s.materials[s.triMaterial[prim]];
+                    const GpuMaterialRecord& m = s.materials[triangleMaterial(s, tr)];
 
                     // Material fetch (:1296-1437).
                     V3 baseRgb = v3(m.baseColorMetalness);
@@ -428,7 +434,9 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                     float emissionMis = 1.0f;
                     if (!(flags & kPrevDelta) && (emissionRgb.x > 0.0f || emissionRgb.y > 0.0f || emissionRgb.z > 0.0f)) {
                         const float ec = fabsf(dot(gn, -rd));
-                        const float lp = ec <= 0.0f ? 0.0f : s.emissive[prim].y * (distance * distance) / ec;
+                        // Instanced triangles are never sampled as lights (pdf 0).
+                        const float epdf = (!k2L || prim < s.drawnTriangles) ? s.emissive[prim].y : 0.0f;
+                        const float lp = ec <= 0.0f ? 0.0f : epdf * (distance * distance) / ec;
                         emissionMis = powerHeuristic(mis.x, lp);
                     }
                     if (emissionRgb.x != 0.0f || emissionRgb.y != 0.0f || emissionRgb.z != 0.0f)
@@ -694,7 +702,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                                         const float4 e = sampleMaterialTexture(s, lm.textureIndices0.w, lu, lvv, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
                                         le = v3(e.x, e.y, e.z) * weight;
                                     }
-                                    le = le * materialOpacity(s, lprim, b1, b2);
+                                    le = le * materialOpacity<k2L>(s, lprim, b1, b2);
                                     const float w = powerHeuristic(lightPdf, surfacePdf(c, ld));
                                     const float contribution = throughput * direct * illuminantSpectrum(le, wl) * (w / lightPdf);
                                     queueShadow(pb, pid, slot, offsetOrigin(hp, ogn, ld), ld, fmaxf(dist - 0.003f, 0.001f),
@@ -781,7 +789,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                                             const float albedo = reflectanceSpectrum(sssColorRgb, wl);
                                             V3 eo, ed;
                                             float att;
-                                            if (randomWalk(s, hp, ogn, radius, albedo, m.subsurfaceWeightScale.z, rng, st, eo, ed, att)) {
+                                            if (randomWalk<k2L>(s, hp, ogn, radius, albedo, m.subsurfaceWeightScale.z, rng, st, eo, ed, att)) {
                                                 throughput *= att;
                                                 ro = eo;
                                                 rd = ed;
@@ -847,7 +855,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
 // radiance in slot order (shadowVisibility :855-889, meshLightVisible
 // :891-912).
 // ---------------------------------------------------------------------------
-template <bool kCount, class Scenes>
+template <bool kCount, class Scenes, bool k2L>
 __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffers pb, TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -891,7 +899,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
                 bool occluded;
                 float hitDistance = 0.0f;
                 if (s.opaque && !falloff) {
-                    occluded = traceAny<kCount>(s, r, 0.001f, tmax, st, ctr);
+                    occluded = traceAnyS<kCount, k2L>(s, r, 0.001f, tmax, st, ctr);
                 } else {
                     // Closest confirmed occluder: opacity >= 0.5 in a
                     // non-opaque scene; its distance drives the falloff.
@@ -900,8 +908,8 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
                     occluded = false;
                     for (;;) {
                         HitRec h;
-                        if (!traceClosest<kCount>(s, r, 0.001f, tmax, loT, loPrim, h, st, ctr)) break;
-                        if (s.opaque || materialOpacity(s, h.prim, h.u, h.v) >= 0.5f) {
+                        if (!traceClosestS<kCount, k2L>(s, r, 0.001f, tmax, loT, loPrim, h, st, ctr)) break;
+                        if (s.opaque || materialOpacity<k2L>(s, h.prim, h.u, h.v) >= 0.5f) {
                             occluded = true;
                             hitDistance = h.t;
                             break;
@@ -1070,6 +1078,7 @@ __global__ void zeroDoubleKernel(double4* p, std::uint32_t n)
 }
 
 // Ray-query kernels behind hdcx_trace_closest / hdcx_trace_occluded.
+template <bool k2L>
 __global__ __launch_bounds__(kBlock) void queryClosestKernel(DeviceScene s, const float4* rays, std::uint32_t n, float4* hits)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -1080,12 +1089,13 @@ __global__ __launch_bounds__(kBlock) void queryClosestKernel(DeviceScene s, cons
         const float4 a = rays[2 * i], b = rays[2 * i + 1];
         const RayPre r = prepareRay(a.x, a.y, a.z, b.x, b.y, b.z);
         HitRec h;
-        const bool found = traceClosest<false>(s, r, a.w, b.w, 0.0f, 0xffffffffU, h, st, nullptr);
+        const bool found = traceClosestS<false, k2L>(s, r, a.w, b.w, 0.0f, 0xffffffffU, h, st, nullptr);
         hits[i] = found ? make_float4(h.t, __uint_as_float(h.prim), h.u, h.v)
                         : make_float4(0.0f, __uint_as_float(0xffffffffU), 0.0f, 0.0f);
     }
 }
 
+template <bool k2L>
 __global__ __launch_bounds__(kBlock) void queryAnyKernel(DeviceScene s, const float4* rays, std::uint32_t n, unsigned char* out)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -1095,7 +1105,7 @@ __global__ __launch_bounds__(kBlock) void queryAnyKernel(DeviceScene s, const fl
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 a = rays[2 * i], b = rays[2 * i + 1];
         const RayPre r = prepareRay(a.x, a.y, a.z, b.x, b.y, b.z);
-        out[i] = traceAny<false>(s, r, a.w, b.w, st, nullptr) ? 1 : 0;
+        out[i] = traceAnyS<false, k2L>(s, r, a.w, b.w, st, nullptr) ? 1 : 0;
     }
 }
 
@@ -1120,67 +1130,77 @@ hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, st
     hipLaunchKernelGGL(gatherTrainingKernel, dim3(GridFor(records, maxBlocks)), dim3(kBlock), 0, st, pb, cp, records);
     return hipGetLastError();
 }
-template <class Scenes>
+template <class Scenes, bool k2L>
 hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                          unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
-                           st, s, pb, in, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock),
+                           0, st, s, pb, in, ctr);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
-                           st, s, pb, in, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)),
+                           dim3(kBlock), 0, st, s, pb, in, ctr);
     return hipGetLastError();
 }
-template <class Scenes>
+template <class Scenes, bool k2L>
 hipError_t LaunchShadeT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
                         const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb,
-                       in, bounce, last, cp);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
+                       pb, in, bounce, last, cp);
     return hipGetLastError();
 }
-template <class Scenes>
+template <class Scenes, bool k2L>
 hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                          TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
-                           st, s, pb, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock),
+                           0, st, s, pb, ctr);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0,
-                           st, s, pb, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)),
+                           dim3(kBlock), 0, st, s, pb, ctr);
     return hipGetLastError();
 }
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
-    return LaunchExtendT(SingleScene{s}, pb, in, maxWork, maxBlocks, ctr, st);
+    return s.instanceCount ? LaunchExtendT<SingleScene, true>(SingleScene{s}, pb, in, maxWork, maxBlocks, ctr, st)
+                           : LaunchExtendT<SingleScene, false>(SingleScene{s}, pb, in, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
                        const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
-    return LaunchShadeT(SingleScene{s}, pb, in, bounce, last, cp, maxWork, maxBlocks, st);
+    return s.instanceCount
+        ? LaunchShadeT<SingleScene, true>(SingleScene{s}, pb, in, bounce, last, cp, maxWork, maxBlocks, st)
+        : LaunchShadeT<SingleScene, false>(SingleScene{s}, pb, in, bounce, last, cp, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st)
 {
-    return LaunchShadowT(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st);
+    return s.instanceCount ? LaunchShadowT<SingleScene, true>(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st)
+                           : LaunchShadowT<SingleScene, false>(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st);
 }
-hipError_t LaunchExtendBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
+                             std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+{
+    const SceneTable t{table, pb.pathScene};
+    return twoLevel ? LaunchExtendT<SceneTable, true>(t, pb, in, maxWork, maxBlocks, ctr, st)
+                    : LaunchExtendT<SceneTable, false>(t, pb, in, maxWork, maxBlocks, ctr, st);
+}
+hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
+                            int last, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+{
+    const SceneTable t{table, pb.pathScene};
+    return twoLevel ? LaunchShadeT<SceneTable, true>(t, pb, in, bounce, last, CacheParams{}, maxWork, maxBlocks, st)
+                    : LaunchShadeT<SceneTable, false>(t, pb, in, bounce, last, CacheParams{}, maxWork, maxBlocks, st);
+}
+hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                              unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
-    return LaunchExtendT(SceneTable{table, pb.pathScene}, pb, in, maxWork, maxBlocks, ctr, st);
-}
-hipError_t LaunchShadeBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                            std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
-{
-    return LaunchShadeT(SceneTable{table, pb.pathScene}, pb, in, bounce, last, CacheParams{}, maxWork, maxBlocks, st);
-}
-hipError_t LaunchShadowBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
-                             TraversalCounters* ctr, hipStream_t st)
-{
-    return LaunchShadowT(SceneTable{table, pb.pathScene}, pb, maxWork, maxBlocks, ctr, st);
+    const SceneTable t{table, pb.pathScene};
+    return twoLevel ? LaunchShadowT<SceneTable, true>(t, pb, maxWork, maxBlocks, ctr, st)
+                    : LaunchShadowT<SceneTable, false>(t, pb, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
                              hipStream_t st)
@@ -1224,13 +1244,19 @@ hipError_t LaunchZero(double4* p, std::uint32_t n, unsigned maxBlocks, hipStream
 hipError_t LaunchQueryClosest(const DeviceScene& s, const float4* rays, std::uint32_t n, float4* hits, unsigned maxBlocks,
                               hipStream_t st)
 {
-    hipLaunchKernelGGL(queryClosestKernel, dim3(GridFor(n, maxBlocks)), dim3(kBlock), 0, st, s, rays, n, hits);
+    if (s.instanceCount)
+        hipLaunchKernelGGL(queryClosestKernel<true>, dim3(GridFor(n, maxBlocks)), dim3(kBlock), 0, st, s, rays, n, hits);
+    else
+        hipLaunchKernelGGL(queryClosestKernel<false>, dim3(GridFor(n, maxBlocks)), dim3(kBlock), 0, st, s, rays, n, hits);
     return hipGetLastError();
 }
 hipError_t LaunchQueryAny(const DeviceScene& s, const float4* rays, std::uint32_t n, unsigned char* out, unsigned maxBlocks,
                           hipStream_t st)
 {
-    hipLaunchKernelGGL(queryAnyKernel, dim3(GridFor(n, maxBlocks)), dim3(kBlock), 0, st, s, rays, n, out);
+    if (s.instanceCount)
+        hipLaunchKernelGGL(queryAnyKernel<true>, dim3(GridFor(n, maxBlocks)), dim3(kBlock), 0, st, s, rays, n, out);
+    else
+        hipLaunchKernelGGL(queryAnyKernel<false>, dim3(GridFor(n, maxBlocks)), dim3(kBlock), 0, st, s, rays, n, out);
     return hipGetLastError();
 }
 

@@ -142,6 +142,58 @@ std::uint32_t BuildSahTop(std::vector<FrontierItem>& items, std::uint32_t firstI
     return root;
 }
 
+Bvh4 BuildBvh4OverBoxes(const std::vector<Box4>& boxes)
+{
+    if (boxes.empty()) throw std::invalid_argument("no boxes");
+    Bvh4 out;
+    const auto n = static_cast<std::uint32_t>(boxes.size());
+    if (n == 1) {
+        Bvh4Node node{};
+        for (int i = 0; i < 4; ++i) {
+            node.lox[i] = node.loy[i] = node.loz[i] = INFINITY;
+            node.hix[i] = node.hiy[i] = node.hiz[i] = -INFINITY;
+            node.child[i] = kEmptyChild;
+        }
+        node.lox[0] = lbvh::PadDown(boxes[0].lo[0]);
+        node.loy[0] = lbvh::PadDown(boxes[0].lo[1]);
+        node.loz[0] = lbvh::PadDown(boxes[0].lo[2]);
+        node.hix[0] = lbvh::PadUp(boxes[0].hi[0]);
+        node.hiy[0] = lbvh::PadUp(boxes[0].hi[1]);
+        node.hiz[0] = lbvh::PadUp(boxes[0].hi[2]);
+        node.child[0] = EncodeLeaf(0, 1);
+        out.nodes.push_back(node);
+        out.depth = 1;
+        return out;
+    }
+    std::vector<FrontierItem> items(n);
+    for (std::uint32_t k = 0; k < n; ++k) items[k] = FrontierItem{k | lbvh::kLeafRef, 1U, 0U, 0U, boxes[k]};
+    std::vector<std::uint32_t> child2, range2;
+    std::vector<Box4> nodeBox;
+    const std::uint32_t root = BuildSahTop(items, 0, child2, range2, nodeBox);
+    std::vector<std::uint32_t> ids(n);
+    for (std::uint32_t k = 0; k < n; ++k) ids[k] = k;
+    const std::uint32_t inner = static_cast<std::uint32_t>(nodeBox.size());
+    out.nodes.resize(inner);
+    std::vector<std::uint32_t> source(inner, 0U), nodeStack(inner, 0U);
+    std::uint32_t counters[4] = {1U, 0U, 0U, 0U};
+    source[0] = root;
+    std::uint32_t begin = 0, end = 1, level = 0;
+    while (begin < end) {
+        for (std::uint32_t id = begin; id < end; ++id)
+            lbvh::CollapseNode<SerialAtomics>(id, level, source.data(), child2.data(), range2.data(), nodeBox.data(),
+                                              boxes.data(), ids.data(), nodeStack.data(), counters, inner,
+                                              out.nodes.data());
+        if (counters[3]) throw std::runtime_error("instance tree collapse overflowed its node pool");
+        begin = end;
+        end = counters[0];
+        ++level;
+    }
+    out.nodes.resize(counters[0]);
+    out.maxStack = counters[1];
+    out.depth = counters[2];
+    return out;
+}
+
 void VertexBounds(const float* positions, std::uint32_t vertexCount, float lo[3], float hi[3])
 {
     for (int k = 0; k < 3; ++k) {

@@ -30,6 +30,10 @@ struct LbvhBuildInfo {
 Bvh4 BuildBvh4LbvhSerial(const float* positions, const std::uint32_t* indices, std::uint32_t n,
                          std::uint32_t targetClusters, LbvhBuildInfo* info = nullptr);
 
+// BVH4 over boxes (binned SAH, one item per leaf: leaf k = item k), for
+// the instance tree. maxStack / depth filled in. items >= 1.
+Bvh4 BuildBvh4OverBoxes(const std::vector<lbvh::Box4>& boxes);
+
 // Morton quantisation bounds of a vertex set (the same on both paths).
 void VertexBounds(const float* positions, std::uint32_t vertexCount, float lo[3], float hi[3]);
 

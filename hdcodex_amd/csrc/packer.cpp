@@ -2,10 +2,12 @@
 // (src/gpu/vulkan_path_tracer.cpp) whose ordering and layout it keeps; the
 // primitive ids the integrator reports are defined by this flattening.
 #include "packer.hpp"
+#include "instance_math.h"
 
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <stdexcept>
 #include <string>
 
 namespace hdcx {
@@ -154,6 +156,68 @@ SceneTexture BuildLatLongImportanceTexture(const SceneTexture& source)
     return result;
 }
 
+PackedScene FlattenInstances(const hdcx_scene_view& v)
+{
+    PackedScene out;
+    const std::uint32_t drawn = v.prototype_count ? v.prototypes[0].first_triangle : v.triangle_count;
+    out.positions.assign(v.positions, v.positions + static_cast<std::size_t>(v.vertex_count) * 4);
+    out.indices.assign(v.indices, v.indices + static_cast<std::size_t>(drawn) * 3);
+    out.triangleMaterials.assign(v.triangle_materials, v.triangle_materials + drawn);
+    out.texcoords.assign(v.texcoords, v.texcoords + static_cast<std::size_t>(drawn) * 6);
+    out.normals.assign(v.normals, v.normals + static_cast<std::size_t>(drawn) * 12);
+    out.emissive.assign(v.emissive, v.emissive + static_cast<std::size_t>(drawn) * 4);
+    for (std::uint32_t i = 0; i < v.instance_count; ++i) {
+        const hdcx_instance& inst = v.instances[i];
+        const hdcx_prototype& proto = v.prototypes[inst.prototype];
+        float normal[9], toObject[12];
+        if (!InvertInstance(inst.object_to_world, toObject, normal))
+            throw std::invalid_argument("singular instance transform");
+        for (std::uint32_t k = 0; k < proto.triangle_count; ++k) {
+            const std::uint32_t t = proto.first_triangle + k;
+            const auto base = static_cast<std::uint32_t>(out.positions.size() / 4);
+            for (int c = 0; c < 3; ++c) {
+                const float* p = v.positions + static_cast<std::size_t>(v.indices[3 * t + c]) * 4;
+                float w[3];
+                TransformPoint(inst.object_to_world, p[0], p[1], p[2], w);
+                out.positions.insert(out.positions.end(), {w[0], w[1], w[2], 1.0F});
+                out.indices.push_back(base + static_cast<std::uint32_t>(c));
+                const float* n = v.normals + static_cast<std::size_t>(t) * 12 + 4 * c;
+                float wn[3];
+                TransformNormal(normal, n[0], n[1], n[2], wn);
+                out.normals.insert(out.normals.end(), {wn[0], wn[1], wn[2], 0.0F});
+            }
+            out.triangleMaterials.push_back(inst.material == 0xffffffffU ? v.triangle_materials[t] : inst.material);
+            out.texcoords.insert(out.texcoords.end(), v.texcoords + static_cast<std::size_t>(t) * 6,
+                                 v.texcoords + static_cast<std::size_t>(t) * 6 + 6);
+            // Never sampled as a light (as on the device): the final CDF value, zero pdf.
+            out.emissive.insert(out.emissive.end(), {v.emissive[static_cast<std::size_t>(t) * 4], 0.0F, 0.0F, 0.0F});
+        }
+    }
+    const auto* mats = static_cast<const GpuMaterialRecord*>(v.materials);
+    out.materials.assign(mats, mats + v.material_count);
+    const auto* lights = static_cast<const GpuLightRecord*>(v.lights);
+    out.lights.assign(lights, lights + std::max<std::uint32_t>(v.light_count, 1U));
+    out.lightCount = v.light_count;
+    for (std::uint32_t i = 0; i < v.texture_count; ++i) {
+        const hdcx_texture_view& tv = v.textures[i];
+        PackedTexture t;
+        t.width = tv.width;
+        t.height = tv.height;
+        t.format = tv.format;
+        const std::size_t n = static_cast<std::size_t>(tv.width) * tv.height * 4;
+        if (tv.format == HDCX_TEXTURE_RGBA32F) {
+            const auto* f = static_cast<const float*>(tv.texels);
+            t.rgba32f.assign(f, f + n);
+        } else {
+            const auto* b = static_cast<const std::uint8_t*>(tv.texels);
+            t.rgba8.assign(b, b + n);
+        }
+        out.textures.push_back(std::move(t));
+    }
+    out.opaque = v.opaque != 0U;
+    return out;
+}
+
 hdcx_scene_view PackedScene::View()
 {
     textureViews.resize(textures.size());
@@ -181,6 +245,10 @@ hdcx_scene_view PackedScene::View()
     v.texture_count = static_cast<std::uint32_t>(textures.size());
     v.opaque = opaque ? 1U : 0U;
     v.textures = textureViews.data();
+    v.prototype_count = static_cast<std::uint32_t>(prototypes.size());
+    v.prototypes = prototypes.data();
+    v.instance_count = static_cast<std::uint32_t>(instances.size());
+    v.instances = instances.data();
     return v;
 }
 
@@ -372,7 +440,18 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
     std::size_t vertexTotal = 0;
     for (const SceneMesh& mesh : snapshot.meshes) vertexTotal += mesh.positions.size() / 3;
     out.positions.reserve(vertexTotal * 4);
-    for (const SceneMesh& mesh : snapshot.meshes) {
+    // Drawn meshes first, then prototype meshes (each a range of triangles
+    // after the drawn ones), both in snapshot order.
+    std::vector<const SceneMesh*> order;
+    for (const SceneMesh& mesh : snapshot.meshes)
+        if (!mesh.prototype) order.push_back(&mesh);
+    const std::size_t drawnMeshes = order.size();
+    for (const SceneMesh& mesh : snapshot.meshes)
+        if (mesh.prototype) order.push_back(&mesh);
+    std::map<std::string, std::uint32_t> prototypeIndex;
+    for (std::size_t oi = 0; oi < order.size(); ++oi) {
+        const SceneMesh& mesh = *order[oi];
+        const auto firstTriangle = static_cast<std::uint32_t>(out.triangleMaterials.size());
         const std::uint32_t base = static_cast<std::uint32_t>(out.positions.size() / 4);
         for (std::size_t i = 0; i + 2 < mesh.positions.size(); i += 3) {
             out.positions.insert(out.positions.end(),
@@ -424,6 +503,38 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
                 }
             }
         }
+        if (oi >= drawnMeshes) {
+            const auto count = static_cast<std::uint32_t>(out.triangleMaterials.size()) - firstTriangle;
+            if (count > 0) {
+                prototypeIndex.emplace(mesh.id, static_cast<std::uint32_t>(out.prototypes.size()));
+                out.prototypes.push_back(hdcx_prototype{firstTriangle, count});
+            }
+        }
+    }
+    // Instances in snapshot order; unknown prototypes are dropped, unknown
+    // materials fall back to the prototype's own.
+    for (const SceneInstance& inst : snapshot.instances) {
+        const auto proto = prototypeIndex.find(inst.prototypeMeshId);
+        if (proto == prototypeIndex.end()) continue;
+        bool finite = true;
+        for (float x : inst.objectToWorld) finite = finite && std::isfinite(x);
+        if (!finite) continue;
+        hdcx_instance r{};
+        std::copy(inst.objectToWorld.begin(), inst.objectToWorld.end(), r.object_to_world);
+        r.prototype = proto->second;
+        const auto mat = inst.materialId.empty() ? materialIndex.end() : materialIndex.find(inst.materialId);
+        r.material = mat == materialIndex.end() ? 0xffffffffU : mat->second;
+        out.instances.push_back(r);
+    }
+    if (out.instances.empty()) {
+        // Prototypes that no instance draws are not part of the scene.
+        const std::uint32_t drawn = out.prototypes.empty() ? static_cast<std::uint32_t>(out.triangleMaterials.size())
+                                                          : out.prototypes.front().first_triangle;
+        out.triangleMaterials.resize(drawn);
+        out.indices.resize(static_cast<std::size_t>(drawn) * 3);
+        out.texcoords.resize(static_cast<std::size_t>(drawn) * 6);
+        out.normals.resize(static_cast<std::size_t>(drawn) * 12);
+        out.prototypes.clear();
     }
     if (out.positions.empty() || out.indices.empty()) {
         out.positions.clear();
@@ -431,13 +542,18 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
         out.triangleMaterials.clear();
         out.texcoords.clear();
         out.normals.clear();
+        out.prototypes.clear();
+        out.instances.clear();
         return out;
     }
 
-    // Emissive-triangle CDF over all triangles, weight = area x emission
-    // power, accumulated in FP64; last CDF entry forced to 1 (:2944-2993).
-    const std::size_t triangles = out.triangleMaterials.size();
-    out.emissive.assign(triangles * 4, 0.0F);
+    // Emissive-triangle CDF over all drawn triangles, weight = area x
+    // emission power, accumulated in FP64; last CDF entry forced to 1
+    // (:2944-2993). Prototype triangles are never sampled as lights: their
+    // entries repeat the final CDF value with zero pdf.
+    const std::size_t allTriangles = out.triangleMaterials.size();
+    const std::size_t triangles = out.prototypes.empty() ? allTriangles : out.prototypes.front().first_triangle;
+    out.emissive.assign(allTriangles * 4, 0.0F);
     std::vector<double> weights(triangles, 0.0);
     std::vector<float> areas(triangles, 0.0F);
     double total = 0.0;
@@ -471,6 +587,7 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
             out.emissive[p * 4 + 1] = areas[p] > 0.0F ? static_cast<float>(probability) / areas[p] : 0.0F;
             out.emissive[p * 4 + 2] = areas[p];
         }
+        for (std::size_t p = triangles; p < allTriangles; ++p) out.emissive[p * 4 + 0] = 1.0F;
     }
     return out;
 }

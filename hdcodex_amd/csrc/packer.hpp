@@ -32,6 +32,8 @@ struct PackedScene {
     std::vector<PackedTexture> textures;
     std::vector<hdcx_texture_view> textureViews;
     bool opaque{true};
+    std::vector<hdcx_prototype> prototypes;
+    std::vector<hdcx_instance> instances;
 
     [[nodiscard]] bool HasGeometry() const noexcept
     {
@@ -40,6 +42,11 @@ struct PackedScene {
     // The view points into this object; refreshes textureViews.
     hdcx_scene_view View();
 };
+
+// The scene with every instance written out as ordinary triangles, in
+// primitive-id order (what the reference would flatten): world vertices and
+// normals from the same FP32 formulas the device uses on the fly.
+PackedScene FlattenInstances(const hdcx_scene_view& view);
 
 // Restates the reference's BuildScene host flattening and packing
 // (src/gpu/vulkan_path_tracer.cpp:2604-2993).

@@ -479,6 +479,8 @@ ProceduralScene MakeProceduralScene(const std::string& name, std::uint64_t seed,
     if (name == "materials") return MakeMaterialShowcase(seed);
     if (name == "furnace") return MakeFurnace();
     if (name == "mixed") return MakeMixedScene(seed, param == 0 ? 20000 : param, domeWidth == 0 ? 256 : domeWidth, 8);
+    if (name == "instanced") return MakeInstancedScene(seed, param == 0 ? 10000 : param, 5, domeWidth == 0 ? 256 : domeWidth, 64);
+    if (name == "instanced_small") return MakeInstancedScene(seed, param == 0 ? 50 : param, 3, domeWidth == 0 ? 64 : domeWidth, 8);
     if (name == "mixed_lights") return MakeMixedScene(seed, param == 0 ? 20000 : param, domeWidth == 0 ? 256 : domeWidth, 63);
     throw std::invalid_argument("unknown procedural scene '" + name + "'");
 }

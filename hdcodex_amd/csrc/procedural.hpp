@@ -53,7 +53,9 @@ struct ProceduralScene {
 };
 
 // "cornell", "heightfield", "triangle", "materials", "furnace", "mixed"
-// and "mixed_lights" (param: triangle target of the mixed scenes).
+// and "mixed_lights" (param: triangle target of the mixed scenes),
+// "instanced" (param: instances of a 20,480-triangle sphere, 64 lights) and
+// "instanced_small" (param: instances of a 1,280-triangle sphere, 8 lights).
 ProceduralScene MakeProceduralScene(const std::string& name, std::uint64_t seed,
                                     std::uint32_t param, std::uint32_t domeWidth);
 
@@ -68,5 +70,11 @@ ProceduralScene MakeMaterialShowcase(std::uint64_t seed);
 // dome of domeWidth x domeWidth/2. About triangleTarget triangles.
 ProceduralScene MakeMixedScene(std::uint64_t seed, std::uint32_t triangleTarget, std::uint32_t domeWidth,
                                std::uint32_t lights);
+
+// `instances` instances (random TRS, materials from the same mix, one per
+// instance) of one level-`level` icosphere prototype (20 x 4^level
+// triangles) over a drawn ground quad, with `lights` lights and the dome.
+ProceduralScene MakeInstancedScene(std::uint64_t seed, std::uint32_t instances, int level, std::uint32_t domeWidth,
+                                   std::uint32_t lights);
 
 } // namespace hdcx

@@ -86,6 +86,111 @@ void Sky(SceneTexture& t, std::uint32_t w, std::uint32_t h, SplitMix64& rng)
         }
 }
 
+SceneMaterial MixMaterial(const std::string& id, SplitMix64& rng)
+{
+    SceneMaterial mat;
+    mat.id = id;
+    mat.baseColor = {rng.Range(0.1, 0.9), rng.Range(0.1, 0.9), rng.Range(0.1, 0.9)};
+    switch (rng.Next() % 6U) {
+    case 0:  // Lambert
+        mat.specularWeight = 0.0F;
+        break;
+    case 1:  // Oren-Nayar
+        mat.diffuseModel = SceneMaterial::DiffuseModel::OrenNayar;
+        mat.diffuseRoughness = rng.Range(0.3, 1.0);
+        mat.specularWeight = 0.0F;
+        break;
+    case 2:  // GGX conductor
+        mat.metalness = 1.0F;
+        mat.baseColor = {rng.Range(0.5, 1.0), rng.Range(0.5, 1.0), rng.Range(0.5, 1.0)};
+        mat.roughness = rng.Range(0.05, 0.5);
+        mat.roughnessV = mat.roughness;
+        break;
+    case 3:  // dispersive dielectric
+        mat.baseColor = {1.0F, 1.0F, 1.0F};
+        mat.transmission = 1.0F;
+        mat.transmissionColor = {1.0F, 1.0F, 1.0F};
+        mat.transmissionDispersionScale = 1.0F;
+        mat.transmissionDispersionAbbeNumber = rng.Range(20.0, 60.0);
+        mat.indexOfRefraction = 1.5F;
+        mat.roughness = rng.Range(0.02, 0.1);
+        mat.roughnessV = mat.roughness;
+        break;
+    case 4:  // diffuse translucent
+        mat.translucentWeight = 1.0F;
+        mat.translucentColor = {rng.Range(0.3, 0.9), rng.Range(0.3, 0.9), rng.Range(0.3, 0.9)};
+        mat.specularWeight = 0.2F;
+        break;
+    default:  // subsurface random walk
+        mat.subsurface = 1.0F;
+        mat.subsurfaceColor = {rng.Range(0.4, 0.95), rng.Range(0.4, 0.95), rng.Range(0.4, 0.95)};
+        mat.subsurfaceRadius = {rng.Range(0.01, 0.5), rng.Range(0.01, 0.5), rng.Range(0.01, 0.5)};
+        mat.subsurfaceScale = 1.0F;
+        mat.roughness = 0.4F;
+        mat.roughnessV = 0.4F;
+        break;
+    }
+    return mat;
+}
+
+void AddSkyAndLights(SceneSnapshot& s, const std::string& prefix, std::uint32_t domeWidth, std::uint32_t lights,
+                     double extent, double spacing, SplitMix64& rng)
+{
+    if (domeWidth > 0) {
+        SceneTexture sky;
+        sky.id = prefix + "/tex/sky";
+        Sky(sky, domeWidth, std::max(domeWidth / 2, 1U), rng);
+        s.textures.push_back(sky);
+        SceneLight dome;
+        dome.id = prefix + "/light/a_dome";
+        dome.type = SceneLightType::Dome;
+        dome.texture = prefix + "/tex/sky";
+        dome.intensity = 0.5F;
+        s.lights.push_back(dome);
+    }
+    for (std::uint32_t i = 0; i < lights; ++i) {
+        char name[48];
+        std::snprintf(name, sizeof(name), "/light/b_%04u", i);
+        SceneLight l;
+        l.id = prefix + name;
+        l.color = {rng.Range(0.6, 1.0), rng.Range(0.6, 1.0), rng.Range(0.6, 1.0)};
+        const float x = static_cast<float>((rng.Uniform() - 0.5) * extent);
+        const float z = static_cast<float>((rng.Uniform() - 0.5) * extent);
+        const float y = static_cast<float>(spacing * (2.0 + 2.0 * rng.Uniform()));
+        l.position = {x, y, z};
+        l.basisX = {1.0F, 0.0F, 0.0F};
+        l.basisY = {0.0F, 0.0F, -1.0F};
+        l.basisZ = {0.0F, 1.0F, 0.0F};  // emits along -basisZ (down)
+        switch (i % 3U) {
+        case 0: {
+            l.type = SceneLightType::Rect;
+            const float a = rng.Range(0.3, 1.0), b = rng.Range(0.3, 1.0);
+            l.axisU = {a, 0.0F, 0.0F};
+            l.axisV = {0.0F, 0.0F, -b};
+            l.area = a * b;
+            l.intensity = rng.Range(5.0, 20.0);
+            break;
+        }
+        case 1:
+            l.type = SceneLightType::Sphere;
+            l.radius = rng.Range(0.1, 0.4);
+            l.area = static_cast<float>(4.0 * kPi * l.radius * l.radius);
+            l.intensity = rng.Range(5.0, 20.0);
+            break;
+        default: {
+            l.type = SceneLightType::Disk;
+            l.radius = rng.Range(0.2, 0.6);
+            l.axisU = {l.radius, 0.0F, 0.0F};
+            l.axisV = {0.0F, 0.0F, -l.radius};
+            l.area = static_cast<float>(kPi * l.radius * l.radius);
+            l.intensity = rng.Range(5.0, 20.0);
+            break;
+        }
+        }
+        s.lights.push_back(l);
+    }
+}
+
 } // namespace
 
 const IcoMesh& UnitIcosphere(int level)
@@ -165,104 +270,10 @@ ProceduralScene MakeMixedScene(std::uint64_t seed, std::uint32_t triangleTarget,
         m.materialId = std::string("/mixed/mat/") + name;
         s.meshes.push_back(std::move(m));
 
-        SceneMaterial mat;
-        mat.id = std::string("/mixed/mat/") + name;
-        mat.baseColor = {rng.Range(0.1, 0.9), rng.Range(0.1, 0.9), rng.Range(0.1, 0.9)};
-        switch (rng.Next() % 6U) {
-        case 0:  // Lambert
-            mat.specularWeight = 0.0F;
-            break;
-        case 1:  // Oren-Nayar
-            mat.diffuseModel = SceneMaterial::DiffuseModel::OrenNayar;
-            mat.diffuseRoughness = rng.Range(0.3, 1.0);
-            mat.specularWeight = 0.0F;
-            break;
-        case 2:  // GGX conductor
-            mat.metalness = 1.0F;
-            mat.baseColor = {rng.Range(0.5, 1.0), rng.Range(0.5, 1.0), rng.Range(0.5, 1.0)};
-            mat.roughness = rng.Range(0.05, 0.5);
-            mat.roughnessV = mat.roughness;
-            break;
-        case 3:  // dispersive dielectric
-            mat.baseColor = {1.0F, 1.0F, 1.0F};
-            mat.transmission = 1.0F;
-            mat.transmissionColor = {1.0F, 1.0F, 1.0F};
-            mat.transmissionDispersionScale = 1.0F;
-            mat.transmissionDispersionAbbeNumber = rng.Range(20.0, 60.0);
-            mat.indexOfRefraction = 1.5F;
-            mat.roughness = rng.Range(0.02, 0.1);
-            mat.roughnessV = mat.roughness;
-            break;
-        case 4:  // diffuse translucent
-            mat.translucentWeight = 1.0F;
-            mat.translucentColor = {rng.Range(0.3, 0.9), rng.Range(0.3, 0.9), rng.Range(0.3, 0.9)};
-            mat.specularWeight = 0.2F;
-            break;
-        default:  // subsurface random walk
-            mat.subsurface = 1.0F;
-            mat.subsurfaceColor = {rng.Range(0.4, 0.95), rng.Range(0.4, 0.95), rng.Range(0.4, 0.95)};
-            mat.subsurfaceRadius = {rng.Range(0.01, 0.5), rng.Range(0.01, 0.5), rng.Range(0.01, 0.5)};
-            mat.subsurfaceScale = 1.0F;
-            mat.roughness = 0.4F;
-            mat.roughnessV = 0.4F;
-            break;
-        }
-        s.materials.push_back(mat);
+        s.materials.push_back(MixMaterial(std::string("/mixed/mat/") + name, rng));
     }
 
-    if (domeWidth > 0) {
-        SceneTexture sky;
-        sky.id = "/mixed/tex/sky";
-        Sky(sky, domeWidth, std::max(domeWidth / 2, 1U), rng);
-        s.textures.push_back(sky);
-        SceneLight dome;
-        dome.id = "/mixed/light/a_dome";
-        dome.type = SceneLightType::Dome;
-        dome.texture = "/mixed/tex/sky";
-        dome.intensity = 0.5F;
-        s.lights.push_back(dome);
-    }
-    for (std::uint32_t i = 0; i < lights; ++i) {
-        char name[48];
-        std::snprintf(name, sizeof(name), "/mixed/light/b_%04u", i);
-        SceneLight l;
-        l.id = name;
-        l.color = {rng.Range(0.6, 1.0), rng.Range(0.6, 1.0), rng.Range(0.6, 1.0)};
-        const float x = static_cast<float>((rng.Uniform() - 0.5) * extent);
-        const float z = static_cast<float>((rng.Uniform() - 0.5) * extent);
-        const float y = static_cast<float>(spacing * (2.0 + 2.0 * rng.Uniform()));
-        l.position = {x, y, z};
-        l.basisX = {1.0F, 0.0F, 0.0F};
-        l.basisY = {0.0F, 0.0F, -1.0F};
-        l.basisZ = {0.0F, 1.0F, 0.0F};  // emits along -basisZ (down)
-        switch (i % 3U) {
-        case 0: {
-            l.type = SceneLightType::Rect;
-            const float a = rng.Range(0.3, 1.0), b = rng.Range(0.3, 1.0);
-            l.axisU = {a, 0.0F, 0.0F};
-            l.axisV = {0.0F, 0.0F, -b};
-            l.area = a * b;
-            l.intensity = rng.Range(5.0, 20.0);
-            break;
-        }
-        case 1:
-            l.type = SceneLightType::Sphere;
-            l.radius = rng.Range(0.1, 0.4);
-            l.area = static_cast<float>(4.0 * kPi * l.radius * l.radius);
-            l.intensity = rng.Range(5.0, 20.0);
-            break;
-        default: {
-            l.type = SceneLightType::Disk;
-            l.radius = rng.Range(0.2, 0.6);
-            l.axisU = {l.radius, 0.0F, 0.0F};
-            l.axisV = {0.0F, 0.0F, -l.radius};
-            l.area = static_cast<float>(kPi * l.radius * l.radius);
-            l.intensity = rng.Range(5.0, 20.0);
-            break;
-        }
-        }
-        s.lights.push_back(l);
-    }
+    AddSkyAndLights(s, "/mixed", domeWidth, lights, extent, spacing, rng);
 
     std::sort(s.meshes.begin(), s.meshes.end(), [](const SceneMesh& a, const SceneMesh& b) { return a.id < b.id; });
     std::sort(s.materials.begin(), s.materials.end(),
@@ -270,6 +281,82 @@ ProceduralScene MakeMixedScene(std::uint64_t seed, std::uint32_t triangleTarget,
     std::sort(s.lights.begin(), s.lights.end(), [](const SceneLight& a, const SceneLight& b) { return a.id < b.id; });
     out.eye = {0.0, extent * 0.55, extent * 0.95};
     out.target = {0.0, spacing * 0.3, 0.0};
+    out.up = {0.0, 1.0, 0.0};
+    out.verticalFovDegrees = 45.0;
+    return out;
+}
+
+ProceduralScene MakeInstancedScene(std::uint64_t seed, std::uint32_t instances, int level, std::uint32_t domeWidth,
+                                   std::uint32_t lights)
+{
+    ProceduralScene out;
+    SceneSnapshot& s = out.snapshot;
+    s.revision = 1;
+    SplitMix64 rng{StreamSeed(seed, 400)};
+    const std::uint32_t n = std::max<std::uint32_t>(instances, 1U);
+    const auto side = static_cast<std::uint32_t>(std::ceil(std::cbrt(static_cast<double>(n) * 4.0)));
+    const double spacing = 2.0, extent = spacing * side;
+    {
+        SceneMesh ground;
+        ground.id = "/inst/a_ground";
+        const auto g = static_cast<float>(extent * 2.0);
+        ground.positions = {-g, 0, g, g, 0, g, g, 0, -g, -g, 0, -g};
+        ground.indices = {0, 1, 2, 0, 2, 3};
+        ground.materialId = "/inst/mat/a_ground";
+        s.meshes.push_back(ground);
+        SceneMaterial gm;
+        gm.id = "/inst/mat/a_ground";
+        gm.baseColor = {0.45F, 0.42F, 0.4F};
+        gm.specularWeight = 0.0F;
+        s.materials.push_back(gm);
+    }
+    {
+        SceneMesh proto;
+        proto.id = "/inst/b_sphere";
+        const double c[3] = {0.0, 0.0, 0.0};
+        AppendIcosphere(proto, level, c, 1.0);
+        proto.materialId = "/inst/mat/a_ground";
+        proto.prototype = true;
+        s.meshes.push_back(std::move(proto));
+    }
+    // Random TRS: uniform scale, uniform rotation (a random unit
+    // quaternion), positions in a slab above the ground.
+    for (std::uint32_t i = 0; i < n; ++i) {
+        char name[48];
+        std::snprintf(name, sizeof(name), "%07u", i);
+        const double scale = spacing * (0.15 + 0.25 * rng.Uniform());
+        double q[4] = {0, 0, 0, 0}, len = 0.0;
+        do {
+            for (double& x : q) x = 2.0 * rng.Uniform() - 1.0;
+            len = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+        } while (len > 1.0 || len < 1e-6);
+        for (double& x : q) x /= std::sqrt(len);
+        const double w = q[0], x = q[1], y = q[2], z = q[3];
+        const double r[9] = {1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                             2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                             2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)};
+        const double t[3] = {(rng.Uniform() - 0.5) * extent, scale + rng.Uniform() * extent * 0.25,
+                             (rng.Uniform() - 0.5) * extent};
+        SceneInstance inst;
+        inst.id = std::string("/inst/i_") + name;
+        inst.prototypeMeshId = "/inst/b_sphere";
+        for (int row = 0; row < 3; ++row) {
+            for (int col = 0; col < 3; ++col) inst.objectToWorld[4 * row + col] = static_cast<float>(scale * r[3 * row + col]);
+            inst.objectToWorld[4 * row + 3] = static_cast<float>(t[row]);
+        }
+        inst.materialId = std::string("/inst/mat/i_") + name;
+        s.materials.push_back(MixMaterial(inst.materialId, rng));
+        s.instances.push_back(std::move(inst));
+    }
+    AddSkyAndLights(s, "/inst", domeWidth, lights, extent, spacing, rng);
+    std::sort(s.meshes.begin(), s.meshes.end(), [](const SceneMesh& a, const SceneMesh& b) { return a.id < b.id; });
+    std::sort(s.materials.begin(), s.materials.end(),
+              [](const SceneMaterial& a, const SceneMaterial& b) { return a.id < b.id; });
+    std::sort(s.lights.begin(), s.lights.end(), [](const SceneLight& a, const SceneLight& b) { return a.id < b.id; });
+    std::sort(s.instances.begin(), s.instances.end(),
+              [](const SceneInstance& a, const SceneInstance& b) { return a.id < b.id; });
+    out.eye = {0.0, extent * 0.45, extent * 0.9};
+    out.target = {0.0, spacing, 0.0};
     out.up = {0.0, 1.0, 0.0};
     out.verticalFovDegrees = 45.0;
     return out;

@@ -2,6 +2,7 @@
 // importance tables) and wavefront path storage; see scene_store.h.
 #include "scene_store.h"
 #include "dev_traverse.h"
+#include "instance_math.h"
 #include "lbvh_host.hpp"
 #include "records.h"
 
@@ -60,6 +61,34 @@ void ValidateScene(const hdcx_scene_view& v)
             ValidateTextureRef(u->z, v.texture_count, "material");
             ValidateTextureRef(u->w, v.texture_count, "material");
         }
+    }
+    if (v.prototype_count) {
+        if (!v.prototypes) throw Failure(HDCX_ERR_INVALID_SCENE, "prototype table missing");
+        std::uint64_t next = v.prototypes[0].first_triangle;
+        for (std::uint32_t k = 0; k < v.prototype_count; ++k) {
+            const hdcx_prototype& p = v.prototypes[k];
+            if (p.first_triangle != next || p.triangle_count == 0)
+                throw Failure(HDCX_ERR_INVALID_SCENE, "prototypes must tile the triangles after the drawn ones");
+            next += p.triangle_count;
+        }
+        if (next != v.triangle_count) throw Failure(HDCX_ERR_INVALID_SCENE, "prototypes must end at the last triangle");
+    }
+    if (v.instance_count) {
+        if (!v.instances || !v.prototype_count) throw Failure(HDCX_ERR_INVALID_SCENE, "instances without prototypes");
+        std::uint64_t flat = v.prototypes[0].first_triangle;
+        for (std::uint32_t i = 0; i < v.instance_count; ++i) {
+            const hdcx_instance& inst = v.instances[i];
+            if (inst.prototype >= v.prototype_count) throw Failure(HDCX_ERR_INVALID_SCENE, "instance prototype out of range");
+            if (inst.material != 0xffffffffU && inst.material >= v.material_count)
+                throw Failure(HDCX_ERR_INVALID_SCENE, "instance material out of range");
+            float toObject[12], normal[9];
+            for (float x : inst.object_to_world)
+                if (!std::isfinite(x)) throw Failure(HDCX_ERR_INVALID_SCENE, "non-finite instance transform");
+            if (!InvertInstance(inst.object_to_world, toObject, normal))
+                throw Failure(HDCX_ERR_INVALID_SCENE, "singular instance transform");
+            flat += v.prototypes[inst.prototype].triangle_count;
+        }
+        if (flat >= 0xffffffffULL) throw Failure(HDCX_ERR_INVALID_SCENE, "more than 2^32 - 1 flattened triangles");
     }
     const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
     for (std::uint32_t l = 0; l < v.light_count; ++l) {
@@ -128,6 +157,164 @@ void BuildSceneBvh(SceneStore* t, const hdcx_scene_view& v, const float lo[3], c
     UploadHostBvh(t, bvh, HDCX_BVH_HOST_SAH, st);
 }
 
+// Instanced scene: a host SAH BVH per prototype and for the drawn
+// triangles, and an instance tree over their world boxes; one node array
+// (instance tree first, root 0) and one slot array.
+void BuildTwoLevel(SceneStore* t, const hdcx_scene_view& v, hipStream_t st)
+{
+    const std::uint32_t drawn = v.prototypes[0].first_triangle;
+    struct Blas {
+        Bvh4 bvh;
+        std::uint32_t nodeBase = 0, slotBase = 0;
+        lbvh::Box4 box{};
+    };
+    auto rootBox = [](const Bvh4& b) {
+        lbvh::Box4 box{};
+        for (int k = 0; k < 3; ++k) {
+            box.lo[k] = INFINITY;
+            box.hi[k] = -INFINITY;
+        }
+        const Bvh4Node& r = b.nodes[0];
+        for (int i = 0; i < 4; ++i) {
+            if (r.child[i] == kEmptyChild) continue;
+            box.lo[0] = std::min(box.lo[0], r.lox[i]);
+            box.lo[1] = std::min(box.lo[1], r.loy[i]);
+            box.lo[2] = std::min(box.lo[2], r.loz[i]);
+            box.hi[0] = std::max(box.hi[0], r.hix[i]);
+            box.hi[1] = std::max(box.hi[1], r.hiy[i]);
+            box.hi[2] = std::max(box.hi[2], r.hiz[i]);
+        }
+        return box;
+    };
+    std::vector<Blas> blas(v.prototype_count + (drawn > 0 ? 1 : 0));
+    for (std::uint32_t k = 0; k < v.prototype_count; ++k) {
+        const hdcx_prototype& p = v.prototypes[k];
+        Bvh4 b = BuildBvh4Host(v.positions, v.indices + static_cast<std::size_t>(p.first_triangle) * 3, p.triangle_count);
+        ValidateBvh4(b, p.triangle_count);
+        for (std::size_t slot = 0; slot < b.triangles.size() / 12; ++slot) {
+            std::uint32_t id;
+            std::memcpy(&id, &b.triangles[slot * 12 + 3], 4);
+            id += p.first_triangle;
+            std::memcpy(&b.triangles[slot * 12 + 3], &id, 4);
+        }
+        blas[k].box = rootBox(b);
+        blas[k].bvh = std::move(b);
+    }
+    if (drawn > 0) {
+        Bvh4 b = BuildBvh4Host(v.positions, v.indices, drawn);
+        ValidateBvh4(b, drawn);
+        blas.back().box = rootBox(b);
+        blas.back().bvh = std::move(b);
+    }
+    // Instance boxes: the prototype box's corners through the transform
+    // (FP64), widened by a bound on the FP32 vertex transform's rounding.
+    std::vector<lbvh::Box4> items;
+    std::vector<InstanceRecord> records(v.instance_count);
+    std::vector<std::uint32_t> bases(v.instance_count);
+    std::vector<float> pads(static_cast<std::size_t>(v.instance_count) * 4);
+    std::uint32_t base = drawn;
+    for (std::uint32_t i = 0; i < v.instance_count; ++i) {
+        const hdcx_instance& in = v.instances[i];
+        const lbvh::Box4& pb = blas[in.prototype].box;
+        const float* m = in.object_to_world;
+        InstanceRecord& r = records[i];
+        std::memcpy(r.toWorld, m, sizeof(r.toWorld));
+        if (!InvertInstance(m, r.toObject, r.normal)) throw Failure(HDCX_ERR_INVALID_SCENE, "singular instance transform");
+        r.prototype = in.prototype;
+        r.material = in.material;
+        r.base = base;
+        bases[i] = base;
+        base += v.prototypes[in.prototype].triangle_count;
+        double ext[3], objExt = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            ext[k] = std::max(std::fabs(static_cast<double>(pb.lo[k])), std::fabs(static_cast<double>(pb.hi[k])));
+            objExt = std::max(objExt, ext[k]);
+        }
+        lbvh::Box4 box{};
+        double normM = 0.0, normInv = 0.0, tInf = 0.0;
+        for (int row = 0; row < 3; ++row) {
+            double lo = 0.0, hi = 0.0, err = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double a = m[4 * row + k];
+                lo += std::min(a * pb.lo[k], a * pb.hi[k]);
+                hi += std::max(a * pb.lo[k], a * pb.hi[k]);
+                err += std::fabs(a) * ext[k];
+            }
+            const double tr = m[4 * row + 3];
+            err = 8.0 * 5.9604644775390625e-8 * (err + std::fabs(tr));
+            box.lo[row] = static_cast<float>(lo + tr - err);
+            box.hi[row] = static_cast<float>(hi + tr + err);
+            normM = std::max(normM, std::fabs(static_cast<double>(m[4 * row])) + std::fabs(static_cast<double>(m[4 * row + 1])) +
+                                        std::fabs(static_cast<double>(m[4 * row + 2])));
+            normInv = std::max(normInv, std::fabs(static_cast<double>(r.toObject[4 * row])) +
+                                            std::fabs(static_cast<double>(r.toObject[4 * row + 1])) +
+                                            std::fabs(static_cast<double>(r.toObject[4 * row + 2])));
+            tInf = std::max(tInf, std::fabs(tr));
+        }
+        items.push_back(box);
+        pads[4 * i + 0] = static_cast<float>(normInv);
+        pads[4 * i + 1] = static_cast<float>(normM);
+        pads[4 * i + 2] = static_cast<float>(tInf);
+        pads[4 * i + 3] = static_cast<float>(objExt);
+    }
+    if (drawn > 0) items.push_back(blas.back().box);
+    const Bvh4 tlas = BuildBvh4OverBoxes(items);
+    // Layout: instance tree, then each BLAS; BLAS references relocated.
+    std::uint32_t nodeCount = static_cast<std::uint32_t>(tlas.nodes.size()), slotCount = 0, blasStack = 0, blasDepth = 0;
+    for (Blas& b : blas) {
+        b.nodeBase = nodeCount;
+        b.slotBase = slotCount;
+        nodeCount += static_cast<std::uint32_t>(b.bvh.nodes.size());
+        slotCount += static_cast<std::uint32_t>(b.bvh.triangles.size() / 12);
+        blasStack = std::max(blasStack, b.bvh.maxStack);
+        blasDepth = std::max(blasDepth, b.bvh.depth);
+    }
+    if (tlas.maxStack + 1 + blasStack > static_cast<std::uint32_t>(kMaxStack))
+        throw Failure(HDCX_ERR_INVALID_SCENE, "instance tree plus BVH deeper than the traversal stack");
+    std::vector<Bvh4Node> nodes(tlas.nodes);
+    nodes.reserve(nodeCount);
+    std::vector<float> slots;
+    slots.reserve(static_cast<std::size_t>(slotCount) * 12);
+    for (const Blas& b : blas) {
+        for (Bvh4Node n : b.bvh.nodes) {
+            for (std::uint32_t& c : n.child) {
+                if (c == kEmptyChild) continue;
+                if (c & kLeafFlag) c = EncodeLeaf(((c >> 3) & 0x0fffffffU) + b.slotBase, (c & 7U) + 1U);
+                else c += b.nodeBase;
+            }
+            nodes.push_back(n);
+        }
+        slots.insert(slots.end(), b.bvh.triangles.begin(), b.bvh.triangles.end());
+    }
+    std::vector<PrototypeRecord> protos(v.prototype_count);
+    for (std::uint32_t k = 0; k < v.prototype_count; ++k)
+        protos[k] = PrototypeRecord{v.prototypes[k].first_triangle, v.prototypes[k].triangle_count, blas[k].nodeBase, 0U};
+    t->nodes.Upload(nodes.data(), nodes.size() * sizeof(Bvh4Node), st);
+    t->tris.Upload(slots.data(), slots.size() * sizeof(float), st);
+    t->instanceRecords.Upload(records.data(), records.size() * sizeof(InstanceRecord), st);
+    t->prototypeRecords.Upload(protos.data(), protos.size() * sizeof(PrototypeRecord), st);
+    t->instanceBase.Upload(bases.data(), bases.size() * 4, st);
+    t->instancePad.Upload(pads.data(), pads.size() * 4, st);
+    Check(hipStreamSynchronize(st), "instance upload");
+    t->refitPlan.order.Reset();
+    t->refitPlan.begin.assign(1, 0U);
+    t->bvhInfo = hdcx_bvh_info{};
+    t->bvhInfo.node_count = nodeCount;
+    t->bvhInfo.slot_count = slotCount;
+    t->bvhInfo.max_stack = tlas.maxStack + 1 + blasStack;
+    t->bvhInfo.depth = tlas.depth + blasDepth;
+    t->bvhInfo.builder = HDCX_BVH_HOST_SAH;
+    DeviceScene& d = t->scene;
+    d.instanceCount = v.instance_count;
+    d.drawnTriangles = drawn;
+    d.tlasRoot = 0;
+    d.drawnRoot = drawn > 0 ? blas.back().nodeBase : kEmptyChild;
+    d.instances = t->instanceRecords.as<InstanceRecord>();
+    d.prototypes = t->prototypeRecords.as<PrototypeRecord>();
+    d.instanceBase = t->instanceBase.as<std::uint32_t>();
+    d.instancePad = t->instancePad.as<float>();
+}
+
 } // namespace
 
 void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
@@ -148,9 +335,21 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
         t->sceneLo[k] = lo[k];
         t->sceneInvExtent[k] = hi[k] > lo[k] ? 1.0F / (hi[k] - lo[k]) : 1.0F;
     }
-    const bool refit = t->bvhOptions.refit != 0U && hadGeometry && t->builtVertexCount == v.vertex_count &&
-                       t->builtIndices.size() == static_cast<std::size_t>(v.triangle_count) * 3 &&
+    // Drawn triangles; prototype triangles only through instances.
+    const std::uint32_t drawn = v.prototype_count ? v.prototypes[0].first_triangle : v.triangle_count;
+    const bool instanced = v.instance_count > 0;
+    if (!instanced && drawn == 0) {
+        t->builtIndices.clear();
+        t->builtVertexCount = 0;
+        return;
+    }
+    const bool refit = !instanced && t->bvhOptions.refit != 0U && hadGeometry && t->builtVertexCount == v.vertex_count &&
+                       t->builtIndices.size() == static_cast<std::size_t>(drawn) * 3 &&
                        std::memcmp(t->builtIndices.data(), v.indices, t->builtIndices.size() * 4) == 0;
+    t->scene.instanceCount = 0;
+    t->scene.drawnTriangles = drawn;
+    t->scene.tlasRoot = 0;
+    t->scene.drawnRoot = 0;
     t->positions.Upload(v.positions, static_cast<std::size_t>(v.vertex_count) * 16, st);
     t->indices.Upload(v.indices, static_cast<std::size_t>(v.triangle_count) * 12, st);
     const auto buildStart = std::chrono::steady_clock::now();
@@ -158,9 +357,15 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
         RefitBvh4(t->positions.as<float4>(), t->indices.as<std::uint32_t>(), t->bvhInfo.slot_count,
                   t->nodes.as<Bvh4Node>(), t->tris.as<float4>(), t->refitPlan, st);
         ++t->bvhInfo.refits;
+    } else if (instanced) {
+        BuildTwoLevel(t, v, st);
+        t->builtIndices.clear();
+        t->builtVertexCount = 0;
     } else {
-        BuildSceneBvh(t, v, lo, hi, st);
-        t->builtIndices.assign(v.indices, v.indices + static_cast<std::size_t>(v.triangle_count) * 3);
+        hdcx_scene_view dv = v;
+        dv.triangle_count = drawn;
+        BuildSceneBvh(t, dv, lo, hi, st);
+        t->builtIndices.assign(v.indices, v.indices + static_cast<std::size_t>(drawn) * 3);
         t->builtVertexCount = v.vertex_count;
     }
     t->bvhInfo.build_ms =
@@ -232,10 +437,10 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     s.texels = t->texels.as<unsigned char>();
     s.importance = t->importance.as<ImportanceDesc>();
     s.importancePool = t->importancePool.as<float>();
-    s.primitiveCount = v.triangle_count;
+    s.primitiveCount = drawn;  // emissive-triangle sampling covers the drawn triangles
     s.lightCount = v.light_count;
     s.opaque = v.opaque ? 1U : 0U;
-    s.hasEmissive = v.emissive[(static_cast<std::size_t>(v.triangle_count) - 1) * 4] > 0.0F ? 1U : 0U;
+    s.hasEmissive = drawn > 0 && v.emissive[(static_cast<std::size_t>(drawn) - 1) * 4] > 0.0F ? 1U : 0U;
     s.textureCount = v.texture_count;
     s.nodeCount = t->bvhInfo.node_count;
     t->geometry = true;

@@ -29,6 +29,9 @@ struct SceneStore {
     RefitPlan refitPlan;
     std::vector<std::uint32_t> builtIndices;
     std::uint32_t builtVertexCount = 0;
+    // Two-level scenes: instance records, prototypes, flattened bases and
+    // the object-space padding terms of each instance.
+    DeviceBuffer instanceRecords, prototypeRecords, instanceBase, instancePad;
     // Vertex bounds (radiance-cache position normalisation).
     float sceneLo[3] = {0, 0, 0}, sceneInvExtent[3] = {1, 1, 1};
 };

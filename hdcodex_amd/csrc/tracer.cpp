@@ -570,6 +570,22 @@ int hdcx_scene_view_get(const hdcx_scene* scene, hdcx_scene_view* out_view)
     });
 }
 
+int hdcx_scene_flatten(const hdcx_scene* scene, hdcx_scene** out_scene)
+{
+    return Guard([&] {
+        if (!scene || !out_scene) throw std::invalid_argument("null argument");
+        auto s = std::make_unique<hdcx_scene>();
+        s->procedural.eye = scene->procedural.eye;
+        s->procedural.target = scene->procedural.target;
+        s->procedural.up = scene->procedural.up;
+        s->procedural.verticalFovDegrees = scene->procedural.verticalFovDegrees;
+        s->procedural.explicitCamera = scene->procedural.explicitCamera;
+        s->procedural.camera = scene->procedural.camera;
+        s->packed = FlattenInstances(const_cast<hdcx_scene*>(scene)->packed.View());
+        *out_scene = s.release();
+    });
+}
+
 int hdcx_scene_camera_get(const hdcx_scene* scene, uint32_t width, uint32_t height, hdcx_camera* out_camera)
 {
     return Guard([&] {

@@ -75,6 +75,27 @@ typedef struct hdcx_texture_view {
  * BuildScene (src/gpu/vulkan_path_tracer.cpp:3004-3030), in its order and
  * byte layout. All pointers are host memory owned by the caller for the
  * duration of hdcx_set_scene. */
+/* Instancing (no counterpart in the reference, which flattens every mesh
+ * into one BLAS under a one-instance TLAS, vulkan_path_tracer.cpp:3032-3125).
+ * A prototype is a range of the view's triangles that is not drawn itself;
+ * prototypes follow the drawn triangles and tile [drawn, triangle_count) in
+ * order. An instance draws one prototype with a row-major 3x4
+ * object-to-world transform and, unless material == 0xffffffff, one
+ * material for all its triangles. Primitive ids are those of the scene the
+ * reference would flatten: drawn triangles first, then instance i's
+ * triangles at base_i + local, base_0 = drawn, base_{i+1} = base_i +
+ * triangles of instance i's prototype. */
+typedef struct hdcx_prototype {
+    uint32_t first_triangle;
+    uint32_t triangle_count;
+} hdcx_prototype;
+
+typedef struct hdcx_instance {
+    float object_to_world[12]; /* rows: x' = m0 x + m1 y + m2 z + m3, ... */
+    uint32_t prototype;
+    uint32_t material;         /* 0xffffffff: the prototype triangles' own */
+} hdcx_instance;
+
 typedef struct hdcx_scene_view {
     uint32_t vertex_count;
     uint32_t triangle_count;
@@ -91,6 +112,10 @@ typedef struct hdcx_scene_view {
     uint32_t texture_count;
     uint32_t opaque;                   /* 1 iff every material is opaque */
     const hdcx_texture_view* textures;
+    uint32_t prototype_count;          /* 0: no instancing */
+    const hdcx_prototype* prototypes;
+    uint32_t instance_count;
+    const hdcx_instance* instances;
 } hdcx_scene_view;
 
 /* One query ray: origin, tmin, direction, tmax (32 bytes). */
@@ -101,9 +126,9 @@ typedef struct hdcx_ray {
     float tmax;
 } hdcx_ray;
 
-/* Closest-hit record (16 bytes). primitive = flattened triangle index, or
- * 0xffffffff on a miss; instance is always 0 for a flattened scene;
- * (u, v) are the barycentrics of vertices 1 and 2. */
+/* Closest-hit record (16 bytes). primitive = flattened triangle index (for
+ * instanced triangles the id in the flattened scene, see hdcx_instance),
+ * or 0xffffffff on a miss; (u, v) are the barycentrics of vertices 1 and 2. */
 typedef struct hdcx_hit {
     float t;
     uint32_t primitive;
@@ -185,6 +210,10 @@ int hdcx_scene_create_procedural(const char* name, uint64_t seed,
 int hdcx_scene_view_get(const hdcx_scene* scene, hdcx_scene_view* out_view);
 int hdcx_scene_camera_get(const hdcx_scene* scene, uint32_t width,
                           uint32_t height, hdcx_camera* out_camera);
+/* A copy of a procedural scene with every instance written out as ordinary
+ * triangles in primitive-id order (the scene the reference would flatten;
+ * test input for the CPU oracle). Same camera. */
+int hdcx_scene_flatten(const hdcx_scene* scene, hdcx_scene** out_scene);
 int hdcx_scene_destroy(hdcx_scene* scene);
 
 /* ---- BVH (no counterpart in the reference: its acceleration structure is

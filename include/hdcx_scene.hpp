@@ -32,6 +32,7 @@ struct SceneMesh {
     std::string materialId;
     std::vector<std::string> triangleMaterialIds;
     std::array<float, 3> displayColor{0.5F, 0.5F, 0.5F};
+    bool prototype{false};  // drawn only through SceneSnapshot::instances
 };
 
 struct SceneMaterial {
@@ -177,12 +178,23 @@ struct SceneLight {
 // Meshes, materials, textures and lights in the order the publisher emits
 // them: lexicographic by id (versioned_scene.h:297-326). Primitive ids are the
 // triangle indices after flattening meshes in this order.
+// Instancing (an extension; the reference has none): a mesh marked
+// `prototype` is not drawn itself, only through instances, each with a
+// row-major 3x4 object-to-world transform and an optional material.
+struct SceneInstance {
+    std::string id;
+    std::string prototypeMeshId;
+    std::array<float, 12> objectToWorld{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    std::string materialId;  // empty: the prototype's own materials
+};
+
 struct SceneSnapshot {
     std::uint64_t revision{0};
     std::vector<SceneMesh> meshes;
     std::vector<SceneMaterial> materials;
     std::vector<SceneTexture> textures;
     std::vector<SceneLight> lights;
+    std::vector<SceneInstance> instances;
 };
 
 struct PathTracerCamera {
