@@ -240,6 +240,49 @@ def batch_workload(hx, threads: int, frames: int = 1, graph: bool = True) -> dic
                                        "sample": "32 pixels per scene (4 rows x 8 columns), same seeds"}}
 
 
+def instanced_workload(hx, frames: int = 2) -> dict:
+    """10,000 instances (seeded uniform scale, uniform rotation, positions in
+    a slab) of a level-5 icosphere (20,480 triangles) over a ground quad:
+    204.8 M instanced triangles, mixed materials per instance, 64 analytic
+    lights (the 64-light cap drops the last in id order) plus a 2048 x 1024
+    dome; 1920 x 1080, 32 spp, 8 bounces, FP64 film. Two-level BVH: host SAH
+    per prototype, SAH instance tree over widened world boxes. Instanced hits
+    and images are bit-identical to the flattened scene's (GPU tests at 40
+    instances), so parity follows from the flattened oracle there."""
+    w, h, spp = 1920, 1080, 32
+    scene = hx.ProceduralScene("instanced", 0, 10000, DOME)
+    cam = scene.camera(w, h)
+    t = hx.PathTracer(0)
+    try:
+        t0 = time.perf_counter()
+        t.set_scene(scene)
+        set_scene_s = time.perf_counter() - t0
+        info = t.bvh_info()
+        t.render(cam, w, h, 0, BOUNCES, spp, readback=False)
+        t.reset_kernel_stats()
+        t.set_instrumentation(1)
+        times = []
+        for f in range(frames):
+            t0 = time.perf_counter()
+            t.render(cam, w, h, 0, BOUNCES, spp, readback=False)
+            times.append(time.perf_counter() - t0)
+        timed = t.kernel_stats()
+        t.set_instrumentation(0)
+        frame = t.read_film(w, h)
+    finally:
+        t.close()
+    ms = sum(times) / frames * 1e3
+    paths = w * h * spp
+    return {"workload": "10,000 instances of a 20,480-triangle icosphere (204.8 M instanced triangles) + ground, mixed "
+                        "materials, 63 lights + dome, 1920x1080, 32 spp x 3 lanes, 8 bounces, two-level BVH (seeded)",
+            "instances": scene.instance_count, "instanced_triangles": 10000 * 20480,
+            "stored_triangles": scene.triangle_count, "ms_per_frame": ms, "mpaths_per_s": paths / (ms * 1e-3) / 1e6,
+            "mrays_per_s": (timed["extend"]["items"] + timed["shadow"]["items"]) / (sum(times)) / 1e6,
+            "kernels_ms_per_frame": {k: timed[k]["ms"] / frames for k in ("extend", "shade", "walk", "shadow")},
+            "set_scene_s": set_scene_s, "bvh": {k: info[k] for k in ("node_count", "max_stack", "depth", "build_ms")},
+            "frame_mean_rgb": [float(x) for x in frame[..., :3].reshape(-1, 3).mean(axis=0)]}
+
+
 def copy_calibration() -> int:
     """A plain copy of known size: 256 MiB read + 256 MiB written per launch.
     Its FETCH_SIZE / WRITE_SIZE readings give the scale that turns those
@@ -270,6 +313,8 @@ def main() -> int:
     ap.add_argument("--batch", action="store_true",
                     help="also render the 64-scene batch workload (adds about a minute)")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-scene batch workload")
+    ap.add_argument("--no-instanced", action="store_true", help="skip the 10,000-instance two-level BVH workload")
+    ap.add_argument("--instanced-only", action="store_true", help="only the 10,000-instance workload")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch workload with plain launches instead of graph replay, no autotuning")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -282,6 +327,11 @@ def main() -> int:
 
         print(json.dumps({"batch": batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph)}),
               flush=True)
+        return 0
+    if args.instanced_only:
+        import hdcodex_amd as hx
+
+        print(json.dumps({"instanced": instanced_workload(hx)}), flush=True)
         return 0
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (every box has one GPU); "
@@ -345,6 +395,12 @@ def main() -> int:
     kernels = {}
     for k in hx.KERNELS:
         kernels[k] = {"ms_per_step": timed[k]["ms"] / args.steps, "launches_per_step": timed[k]["launches"] / args.steps}
+        traffic = measured_traffic(k)
+        if traffic and timed[k]["launches"]:
+            # Measured HBM bytes per launch (rocprofv3 counters) over this run's average launch time.
+            avg_s = timed[k]["ms"] * 1e-3 / timed[k]["launches"]
+            kernels[k]["hbm_GBs"] = traffic / avg_s / 1e9
+            kernels[k]["hbm_frac"] = kernels[k]["hbm_GBs"] / HBM_PEAK_GBS
     dominant = max(("extend", "shade", "shadow"), key=lambda k: timed[k]["ms"])
     roof = {}
     for k in ("extend", "shadow"):
@@ -400,6 +456,9 @@ def main() -> int:
         log("parity vs FP64 oracle (subset)")
         result["parity"] = parity(scene, cam, frame)
         result["parity_rel_mse_vs_fp64_oracle"] = result["parity"]["rel_mse_vs_fp64_oracle"]
+    if not args.no_instanced:
+        log("instanced workload")
+        result["instanced"] = instanced_workload(hx)
     if args.batch:
         log("batch workload")
         result["batch"] = batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph)

@@ -45,7 +45,7 @@ struct Plan {
 struct GraphKey {
     const void* pool = nullptr;
     const void* tables[7] = {};
-    std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0, twoLevel = 0;
+    std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0, twoLevel = 0, walk = 0;
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 
@@ -195,8 +195,12 @@ struct hdcx_batch : PathPool {
         bw.wavePixels = wavePixels.as<std::uint32_t>();
         bw.cursor = cursor.as<std::uint32_t>();
         const DeviceScene* table = sceneTable.as<DeviceScene>();
-        bool twoLevel = false;
-        for (const DeviceScene& d : sceneHost) twoLevel = twoLevel || d.instanceCount > 0;
+        bool twoLevel = false, subsurface = false;
+        for (const DeviceScene& d : sceneHost) {
+            twoLevel = twoLevel || d.instanceCount > 0;
+            subsurface = subsurface || d.hasSubsurface;
+        }
+        const int walk = subsurface ? kShadeWalkDefer : kShadeWalkInline;
         const PathBuffers pb = Paths();
         const std::uint32_t maxPaths = plan.maxPaths, maxPixels = plan.maxPixels;
         auto enqueueWave = [&] {
@@ -204,9 +208,15 @@ struct hdcx_batch : PathPool {
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
                 Check(LaunchExtendBatch(table, twoLevel, pb, in, maxPaths, maxBlocks, nullptr, st), "batch extend");
-                Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, maxPaths,
-                                       maxBlocks, st),
+                const int last = b + 1 == bounces ? 1 : 0;
+                Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, walk, maxPaths, maxBlocks, st),
                       "batch shade");
+                if (walk == kShadeWalkDefer && !last) {
+                    Check(LaunchWalkBatch(table, twoLevel, pb, maxPaths, maxBlocks, st), "batch walk");
+                    Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, kShadeWalkResume, maxPaths,
+                                           maxBlocks, st),
+                          "batch shade (resume)");
+                }
                 Check(LaunchShadowBatch(table, twoLevel, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
             }
             Check(LaunchFilmBatch(pb, bw, maxPixels, maxBlocks, st), "batch film");
@@ -224,6 +234,7 @@ struct hdcx_batch : PathPool {
             key.maxPaths = maxPaths;
             key.maxPixels = maxPixels;
             key.twoLevel = twoLevel ? 1U : 0U;
+            key.walk = static_cast<std::uint32_t>(walk);
             if (!graphExec || !(key == graphKey)) {
                 DropGraph();
                 Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");

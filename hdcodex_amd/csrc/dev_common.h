@@ -50,6 +50,7 @@ struct DeviceScene {
     std::uint32_t hasEmissive;
     std::uint32_t textureCount;
     std::uint32_t nodeCount;
+    std::uint32_t hasSubsurface;       // some material can take the subsurface random walk
     // Two-level scenes (instancing); instanceCount == 0 for one level.
     std::uint32_t instanceCount;
     std::uint32_t drawnTriangles;      // primitive ids below this are drawn triangles
@@ -89,11 +90,17 @@ struct PathBuffers {
     std::uint32_t* shadowLight;  // light index, kShadowMeshLight, or ~0 = empty
     std::uint32_t* queue[2];
     std::uint32_t* shadowQueue;
-    std::uint32_t* counters; // [0],[1] extend queue sizes; [2] shadow queue size
+    std::uint32_t* counters; // [0],[1] extend queue sizes; [2] shadow queue size; [3] walk queue size
     unsigned long long* totals;  // [0] extend rays, [1] shadow-queued paths, [2] shadow rays traced
     float* lambda0;          // wavelength of lane 0 of the path's hero triple
     float2* spread;          // radiance cache: primary footprint a0, accumulated sqrt spread
     std::uint32_t* pathScene;  // batch renderer: scene of each path
+    // Subsurface walks deferred by shade, indexed by walk-queue slot: three
+    // float4 per walk. In: (entry xyz, radius), (outward normal xyz, albedo),
+    // (anisotropy, rng bits, -, -). Out: (exit origin xyz, attenuation),
+    // (exit direction xyz, 1 = exited / 0 = no exit), (-, rng bits, -, -).
+    float4* walk;
+    std::uint32_t* walkQueue;  // path id of each walk slot
     std::uint32_t capacity;
 };
 

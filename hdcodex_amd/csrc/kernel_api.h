@@ -11,8 +11,15 @@ hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, st
                                 hipStream_t st);
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
+// walk: kShadeWalkInline runs subsurface walks inside shade;
+// kShadeWalkDefer queues them (then LaunchWalk, then shade again with
+// kShadeWalkResume over the walk queue: same results, the walks' traversal
+// runs in a kernel of its own).
+constexpr int kShadeWalkInline = 0, kShadeWalkDefer = 1, kShadeWalkResume = 2;
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                       const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+                       const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                      hipStream_t st);
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st);
 // Batch renderer: the same stages over a table of scenes (scene of each
@@ -22,7 +29,9 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
                              std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
-                            int last, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+                            int last, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
+                           unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                              unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,

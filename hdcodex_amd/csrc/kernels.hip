@@ -6,6 +6,12 @@
 // length the previous stage wrote; the host never reads a count back.
 // shade compacts surviving paths and paths with pending shadow rays with a
 // wave ballot + prefix popcount and one atomic per wave.
+//
+// FP contraction is off for the whole integrator: every instantiation of a
+// stage (one scene or a scene table, one or two levels, walks inline or
+// deferred) then rounds the same operations the same way, and the FP32
+// arithmetic follows the oracle's (built with -ffp-contract=off).
+#pragma clang fp contract(off)
 #include "kernel_api.h"
 #include "dev_shading.h"
 #include "dev_traverse2.h"
@@ -194,6 +200,7 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         pb.counters[in ^ 1U] = 0U;
         pb.counters[2] = 0U;
+        pb.counters[3] = 0U;
     }
     const std::uint32_t count = pb.counters[in];
     if (blockIdx.x == 0 && threadIdx.x == 0) pb.totals[0] += count;
@@ -280,8 +287,17 @@ __device__ __forceinline__ void queueShadow(const PathBuffers& pb, std::uint32_t
 // test is queued as a shadow ray carrying its unoccluded contribution; the
 // shadow stage adds it in slot order, which keeps the reference's order of
 // additions (emission, analytic NEE, mesh-light NEE).
+//
+// Subsurface walks (kWalk): kShadeWalkInline runs each walk here.
+// kShadeWalkDefer stops a path at the walk, stores the walk's inputs in a
+// walk-queue slot and commits none of the path's state (what it did write,
+// the shadow-ray slots and training records, is rewritten with the same
+// values later); walkKernel then runs the walks without shade's register
+// load, and kShadeWalkResume shades the queued paths again from their
+// unchanged state, taking the walk's outcome and random state from the
+// slot. The three steps give the same results as kShadeWalkInline.
 // ---------------------------------------------------------------------------
-template <class Scenes, bool k2L>
+template <class Scenes, bool k2L, int kWalk>
 __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int bounce,
                                                      int lastBounce, CacheParams cp)
 {
@@ -289,8 +305,8 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
     TraversalStack st;
     st.lds = stackMem + threadIdx.x;
     st.stride = kBlock;
-    const std::uint32_t count = pb.counters[in];
-    const std::uint32_t* queue = pb.queue[in];
+    const std::uint32_t count = kWalk == kShadeWalkResume ? pb.counters[3] : pb.counters[in];
+    const std::uint32_t* queue = kWalk == kShadeWalkResume ? pb.walkQueue : pb.queue[in];
     std::uint32_t* outQueue = pb.queue[in ^ 1U];
     std::uint32_t* outCounter = pb.counters + (in ^ 1U);
     // Every lane of a wave must reach the ballots: iterate over the padded range.
@@ -302,6 +318,8 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
         std::uint32_t pid = 0;
         bool wantQuery = false;
         RcQuery query{};
+        bool deferred = false;
+        float4 walkIn0{}, walkIn1{}, walkIn2{};
         if (i < count) {
             pid = queue[i];
             const DeviceScene& s = scenes.Get(pid);
@@ -316,6 +334,8 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
             std::uint32_t flags = pb.flags[pid];
             float2 mis = pb.mis[pid];
             float4 med = pb.medium[pid];
+            float2 spreadOut = make_float2(0.0f, 0.0f);
+            bool spreadSet = false;
             for (int k = 0; k < kShadowSlots; ++k) pb.shadowLight[k * pb.capacity + pid] = 0xffffffffU;
 
             if (prim == 0xffffffffU) {
@@ -549,7 +569,8 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                             } else if (!(flags & kPrevDelta) && mis.x > 0.0f) {
                                 sp.y += sqrtf(distance * distance / (mis.x * cosIn));
                             }
-                            pb.spread[pid] = sp;
+                            spreadOut = sp;
+                            spreadSet = true;
                             if (bounce >= static_cast<int>(cp.terminateBounce) &&
                                 (cp.spreadThreshold <= 0.0f || sp.y * sp.y > cp.spreadThreshold * sp.x)) {
                                 cacheStop = true;
@@ -788,8 +809,24 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                                             const float radius = fmaxf(reconstructRgb(sssRadiusRgb, wl) * fmaxf(m.subsurfaceWeightScale.y, 0.0f), 0.001f);
                                             const float albedo = reflectanceSpectrum(sssColorRgb, wl);
                                             V3 eo, ed;
-                                            float att;
-                                            if (randomWalk<k2L>(s, hp, ogn, radius, albedo, m.subsurfaceWeightScale.z, rng, st, eo, ed, att)) {
+                                            float att = 1.0f;
+                                            bool exited = false;
+                                            if constexpr (kWalk == kShadeWalkDefer) {
+                                                deferred = true;
+                                                walkIn0 = make_float4(hp.x, hp.y, hp.z, radius);
+                                                walkIn1 = make_float4(ogn.x, ogn.y, ogn.z, albedo);
+                                                walkIn2 = make_float4(m.subsurfaceWeightScale.z, __uint_as_float(rng), 0.0f, 0.0f);
+                                            } else if constexpr (kWalk == kShadeWalkResume) {
+                                                const float4 w0 = pb.walk[3U * i], w1 = pb.walk[3U * i + 1U];
+                                                rng = __float_as_uint(pb.walk[3U * i + 2U].y);
+                                                exited = w1.w != 0.0f;
+                                                eo = v3(w0);
+                                                ed = v3(w1);
+                                                att = w0.w;
+                                            } else {
+                                                exited = randomWalk<k2L>(s, hp, ogn, radius, albedo, m.subsurfaceWeightScale.z, rng, st, eo, ed, att);
+                                            }
+                                            if (exited) {
                                                 throughput *= att;
                                                 ro = eo;
                                                 rd = ed;
@@ -798,7 +835,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                                                 walked = true;
                                             }
                                         }
-                                        if (!walked) {
+                                        if (!walked && !deferred) {
                                             rd = cosineHemisphere(normal, rng);
                                             const float lambert = fmaxf(dot(normal, rd), 0.0f) / kPi;
                                             throughput *= evalDiffuse(normal, c.view, rd, c.diffuseColor, c.diffuseRoughness, c.diffuseModel) /
@@ -809,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                             }
                             if (walked) {
                                 keep = true;
-                            } else {
+                            } else if (!deferred) {
                                 if (s.lightCount > 0U || s.hasEmissive) {
                                     mis.x = surfacePdf(c, rd);
                                     flags &= ~kPrevDelta;
@@ -829,24 +866,63 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                         }
                     }
                 }
-                if (keep) {
-                    pb.rayO[pid] = make_float4(ro.x, ro.y, ro.z, 0.0f);
-                    pb.rayD[pid] = make_float4(rd.x, rd.y, rd.z, 0.0f);
-                    pb.flags[pid] = flags;
-                    pb.mis[pid] = mis;
-                    pb.medium[pid] = med;
+                if (!deferred) {
+                    if (keep) {
+                        pb.rayO[pid] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+                        pb.rayD[pid] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+                        pb.flags[pid] = flags;
+                        pb.mis[pid] = mis;
+                        pb.medium[pid] = med;
+                    }
+                    pb.rng[pid] = rng;
+                    pb.throughput[pid] = throughput;
+                    pb.radiance[pid] = radiance;
+                    if (spreadSet) pb.spread[pid] = spreadOut;
                 }
-                pb.rng[pid] = rng;
-                pb.throughput[pid] = throughput;
-                pb.radiance[pid] = radiance;
             }
         }
         waveAppend(keep, pid, outQueue, outCounter);
-        waveAppend(slot > 0, pid, pb.shadowQueue, pb.counters + 2);
-        if (cp.enabled) {
-            const std::uint32_t qslot = waveSlot(wantQuery, cp.queryCount);
-            if (wantQuery && qslot < cp.queryCapacity) cp.queries[qslot] = query;
+        if constexpr (kWalk != kShadeWalkResume) {
+            // A resumed path queued its shadow rays when it was deferred.
+            waveAppend(slot > 0, pid, pb.shadowQueue, pb.counters + 2);
+            if (cp.enabled) {
+                const std::uint32_t qslot = waveSlot(wantQuery, cp.queryCount);
+                if (wantQuery && qslot < cp.queryCapacity) cp.queries[qslot] = query;
+            }
         }
+        if constexpr (kWalk == kShadeWalkDefer) {
+            const std::uint32_t wslot = waveSlot(deferred, pb.counters + 3);
+            if (deferred) {
+                pb.walk[3U * wslot] = walkIn0;
+                pb.walk[3U * wslot + 1U] = walkIn1;
+                pb.walk[3U * wslot + 2U] = walkIn2;
+                pb.walkQueue[wslot] = pid;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// walk: the subsurface random walks that shade deferred, one per lane.
+// ---------------------------------------------------------------------------
+template <class Scenes, bool k2L>
+__global__ __launch_bounds__(kBlock) void walkKernel(Scenes scenes, PathBuffers pb)
+{
+    __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
+    TraversalStack st;
+    st.lds = stackMem + threadIdx.x;
+    st.stride = kBlock;
+    const std::uint32_t count = pb.counters[3];
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const DeviceScene& s = scenes.Get(pb.walkQueue[i]);
+        const float4 w0 = pb.walk[3U * i], w1 = pb.walk[3U * i + 1U], w2 = pb.walk[3U * i + 2U];
+        std::uint32_t rng = __float_as_uint(w2.y);
+        V3 eo = v3(0.0f), ed = v3(0.0f);
+        float att = 1.0f;
+        const bool exited = randomWalk<k2L>(s, v3(w0), v3(w1), w0.w, w1.w, w2.x, rng, st, eo, ed, att);
+        pb.walk[3U * i] = make_float4(eo.x, eo.y, eo.z, att);
+        pb.walk[3U * i + 1U] = make_float4(ed.x, ed.y, ed.z, exited ? 1.0f : 0.0f);
+        pb.walk[3U * i + 2U] = make_float4(w2.x, __uint_as_float(rng), 0.0f, 0.0f);
     }
 }
 
@@ -1144,10 +1220,25 @@ hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t i
 }
 template <class Scenes, bool k2L>
 hipError_t LaunchShadeT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                        const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+                        const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
-                       pb, in, bounce, last, cp);
+    const dim3 grid(GridFor(maxWork, maxBlocks));
+    if (walk == kShadeWalkDefer)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkDefer>), grid, dim3(kBlock), 0, st, s, pb, in,
+                           bounce, last, cp);
+    else if (walk == kShadeWalkResume)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkResume>), grid, dim3(kBlock), 0, st, s, pb, in,
+                           bounce, last, cp);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkInline>), grid, dim3(kBlock), 0, st, s, pb, in,
+                           bounce, last, cp);
+    return hipGetLastError();
+}
+template <class Scenes, bool k2L>
+hipError_t LaunchWalkT(const Scenes& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(walkKernel<Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
+                       pb);
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
@@ -1169,11 +1260,17 @@ hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32
                            : LaunchExtendT<SingleScene, false>(SingleScene{s}, pb, in, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                       const CacheParams& cp, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+                       const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
     return s.instanceCount
-        ? LaunchShadeT<SingleScene, true>(SingleScene{s}, pb, in, bounce, last, cp, maxWork, maxBlocks, st)
-        : LaunchShadeT<SingleScene, false>(SingleScene{s}, pb, in, bounce, last, cp, maxWork, maxBlocks, st);
+        ? LaunchShadeT<SingleScene, true>(SingleScene{s}, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st)
+        : LaunchShadeT<SingleScene, false>(SingleScene{s}, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st);
+}
+hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                      hipStream_t st)
+{
+    return s.instanceCount ? LaunchWalkT<SingleScene, true>(SingleScene{s}, pb, maxWork, maxBlocks, st)
+                           : LaunchWalkT<SingleScene, false>(SingleScene{s}, pb, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st)
@@ -1189,11 +1286,18 @@ hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const Path
                     : LaunchExtendT<SceneTable, false>(t, pb, in, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
-                            int last, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+                            int last, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchShadeT<SceneTable, true>(t, pb, in, bounce, last, CacheParams{}, maxWork, maxBlocks, st)
-                    : LaunchShadeT<SceneTable, false>(t, pb, in, bounce, last, CacheParams{}, maxWork, maxBlocks, st);
+    return twoLevel ? LaunchShadeT<SceneTable, true>(t, pb, in, bounce, last, CacheParams{}, walk, maxWork, maxBlocks, st)
+                    : LaunchShadeT<SceneTable, false>(t, pb, in, bounce, last, CacheParams{}, walk, maxWork, maxBlocks, st);
+}
+hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
+                           unsigned maxBlocks, hipStream_t st)
+{
+    const SceneTable t{table, pb.pathScene};
+    return twoLevel ? LaunchWalkT<SceneTable, true>(t, pb, maxWork, maxBlocks, st)
+                    : LaunchWalkT<SceneTable, false>(t, pb, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                              unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)

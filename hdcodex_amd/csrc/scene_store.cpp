@@ -441,6 +441,10 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     s.lightCount = v.light_count;
     s.opaque = v.opaque ? 1U : 0U;
     s.hasEmissive = drawn > 0 && v.emissive[(static_cast<std::size_t>(drawn) - 1) * 4] > 0.0F ? 1U : 0U;
+    s.hasSubsurface = 0U;
+    const auto* mats = static_cast<const GpuMaterialRecord*>(v.materials);
+    for (std::uint32_t m = 0; m < v.material_count; ++m)
+        if (mats[m].subsurfaceWeightScale.x > 0.0F || mats[m].textureIndices2.x != kMissingTexture) s.hasSubsurface = 1U;
     s.textureCount = v.texture_count;
     s.nodeCount = t->bvhInfo.node_count;
     t->geometry = true;
@@ -474,6 +478,8 @@ void PathPool::EnsurePaths(std::uint32_t paths)
     lambda0.Allocate(n * 4);
     spread.Allocate(n * 8);
     pathScene.Allocate(n * 4);
+    walk.Allocate(n * 48);
+    walkQueue.Allocate(n * 4);
     capacity = paths;
 }
 
@@ -501,6 +507,8 @@ PathBuffers PathPool::Paths()
     pb.lambda0 = lambda0.as<float>();
     pb.spread = spread.as<float2>();
     pb.pathScene = pathScene.as<std::uint32_t>();
+    pb.walk = walk.as<float4>();
+    pb.walkQueue = walkQueue.as<std::uint32_t>();
     pb.capacity = capacity;
     return pb;
 }

@@ -305,6 +305,7 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
         Check(hipMemsetAsync(t->rcCounters.as<void>(), 0, 8, st), "rc counters");
     }
     TraversalCounters* ctr = (t->instrumentation & 2U) ? t->traversalCounters.as<TraversalCounters>() : nullptr;
+    const int walk = !t->scene.hasSubsurface || (t->instrumentation & 4U) ? kShadeWalkInline : kShadeWalkDefer;
     TraversalCounters* shadowCtr = ctr ? ctr + 1 : nullptr;
     ChunkDesc c{};
     c.width = w;
@@ -330,9 +331,16 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
                 t->Timed(HDCX_KERNEL_EXTEND, [&] { return LaunchExtend(t->scene, pb, in, paths, t->maxBlocks, ctr, st); });
+                const int last = b + 1 == bounces ? 1 : 0;
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
-                    return LaunchShade(t->scene, pb, in, static_cast<int>(b), b + 1 == bounces ? 1 : 0, cp, paths, t->maxBlocks, st);
+                    return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, walk, paths, t->maxBlocks, st);
                 });
+                if (walk == kShadeWalkDefer && !last) {
+                    t->Timed(HDCX_KERNEL_WALK, [&] { return LaunchWalk(t->scene, pb, paths, t->maxBlocks, st); });
+                    t->Timed(HDCX_KERNEL_SHADE, [&] {
+                        return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, kShadeWalkResume, paths, t->maxBlocks, st);
+                    });
+                }
                 // Queries come from the terminate vertex, or from any later
                 // one when the spread rule is on.
                 if (cache && (b == cp.terminateBounce || (b > cp.terminateBounce && cp.spreadThreshold > 0.0F))) {

@@ -396,7 +396,8 @@ int hdcx_rc_get_stats(hdcx_tracer* tracer, hdcx_rc_stats* out_stats);
 #define HDCX_KERNEL_SHADOW 3
 #define HDCX_KERNEL_FILM 4
 #define HDCX_KERNEL_CACHE 5 /* radiance-cache inference */
-#define HDCX_KERNEL_COUNT 6
+#define HDCX_KERNEL_WALK 6  /* subsurface random walks deferred by shade */
+#define HDCX_KERNEL_COUNT 7
 
 typedef struct hdcx_kernel_stats {
     uint64_t launches[HDCX_KERNEL_COUNT];
@@ -417,7 +418,9 @@ typedef struct hdcx_kernel_stats {
 /* flags bit 0: time every kernel launch with HIP events on the launch
  * stream; bit 1: run the traversal kernels compiled with node/triangle
  * counters (a separate instantiation; the counting build is slower and is
- * used only to measure algorithmic bytes). */
+ * used only to measure algorithmic bytes); bit 2: run subsurface walks
+ * inside the shade kernel instead of deferring them to the walk kernel
+ * (the same results; kept to check the split against). */
 int hdcx_set_instrumentation(hdcx_tracer* tracer, uint32_t flags);
 int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats);
 int hdcx_reset_kernel_stats(hdcx_tracer* tracer);
