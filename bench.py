@@ -267,10 +267,23 @@ def instanced_workload(hx, frames: int = 2) -> dict:
             t.render(cam, w, h, 0, BOUNCES, spp, readback=False)
             times.append(time.perf_counter() - t0)
         timed = t.kernel_stats()
-        t.set_instrumentation(0)
         frame = t.read_film(w, h)
+        # Counting pass: nodes (instance tree and prototype trees) and
+        # triangles per ray.
+        t.reset_kernel_stats()
+        t.set_instrumentation(2)
+        t.render(cam, w, h, 0, BOUNCES, spp, readback=False)
+        counted = t.kernel_stats()
+        t.set_instrumentation(0)
     finally:
         t.close()
+    alg = algorithmic_bytes(counted)
+    traversal = {}
+    for k in ("extend", "shadow"):
+        if k in alg and timed[k]["ms"] > 0:
+            rays = timed[k]["items"]
+            achieved = rays * alg[k]["bytes_per_ray"] / (timed[k]["ms"] * 1e-3) / 1e9
+            traversal[k] = {**alg[k], "achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS}
     ms = sum(times) / frames * 1e3
     paths = w * h * spp
     return {"workload": "10,000 instances of a 20,480-triangle icosphere (204.8 M instanced triangles) + ground, mixed "
@@ -279,6 +292,7 @@ def instanced_workload(hx, frames: int = 2) -> dict:
             "stored_triangles": scene.triangle_count, "ms_per_frame": ms, "mpaths_per_s": paths / (ms * 1e-3) / 1e6,
             "mrays_per_s": (timed["extend"]["items"] + timed["shadow"]["items"]) / (sum(times)) / 1e6,
             "kernels_ms_per_frame": {k: timed[k]["ms"] / frames for k in ("extend", "shade", "walk", "shadow")},
+            "traversal": traversal,
             "set_scene_s": set_scene_s, "bvh": {k: info[k] for k in ("node_count", "max_stack", "depth", "build_ms")},
             "frame_mean_rgb": [float(x) for x in frame[..., :3].reshape(-1, 3).mean(axis=0)]}
 

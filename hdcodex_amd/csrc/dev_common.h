@@ -59,7 +59,7 @@ struct DeviceScene {
     const InstanceRecord* instances;
     const PrototypeRecord* prototypes;
     const std::uint32_t* instanceBase; // flattened id of each instance's first triangle
-    const float* instancePad;          // per instance: 4 floats bounding the object-space slab padding
+    const float* instancePad;          // per instance: 12 floats: padding bound terms, world box lo, hi
 };
 
 // ---------------------------------------------------------------------------

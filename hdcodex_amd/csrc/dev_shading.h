@@ -546,10 +546,11 @@ __device__ inline float totalLightWeight(const DeviceScene& s, V3 pos, V3 n, flo
     for (std::uint32_t i = 0; i < s.lightCount; ++i) total += lightSelectionWeight(s.lights[i], pos, n, de, se, te);
     return total;
 }
-__device__ inline void sampleAnalyticLight(const DeviceScene& s, V3 pos, V3 n, float de, float se, float te,
+// total: totalLightWeight at the same point and lobe energies (one sum per
+// shading point, shared by light selection and both MIS pdfs).
+__device__ inline void sampleAnalyticLight(const DeviceScene& s, V3 pos, V3 n, float de, float se, float te, float total,
                                            std::uint32_t& rng, std::uint32_t& index, float& pdf)
 {
-    const float total = totalLightWeight(s, pos, n, de, se, te);
     if (total <= 0.0f) {
         index = min(static_cast<std::uint32_t>(randomFloat(rng) * static_cast<float>(s.lightCount)), s.lightCount - 1U);
         pdf = 1.0f / static_cast<float>(s.lightCount);
@@ -564,10 +565,10 @@ __device__ inline void sampleAnalyticLight(const DeviceScene& s, V3 pos, V3 n, f
     }
     pdf = lightSelectionWeight(s.lights[index], pos, n, de, se, te) / total;
 }
-__device__ inline float environmentLightPdf(const DeviceScene& s, V3 pos, V3 n, V3 dir, float de, float se, float te)
+__device__ inline float environmentLightPdf(const DeviceScene& s, V3 pos, V3 n, V3 dir, float de, float se, float te,
+                                           float total)
 {
     if (s.lightCount == 0U) return 0.0f;
-    const float total = totalLightWeight(s, pos, n, de, se, te);
     if (total <= 0.0f) return 0.0f;
     float pdf = 0.0f;
     for (std::uint32_t i = 0; i < s.lightCount; ++i) {

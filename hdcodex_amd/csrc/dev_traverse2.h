@@ -24,6 +24,9 @@ struct BoxRay {
 __device__ __forceinline__ BoxRay worldBoxRay(const RayPre& r) { return BoxRay{r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, 0.0f}; }
 
 // Object-space ray of instance i and the box widening for hits up to tmax.
+// The rounding of the transformed ray grows with the ray parameter t, and a
+// hit on the instance lies inside its world box, so t·|d|∞ is bounded by
+// the largest coordinate distance from the origin to that box.
 __device__ __forceinline__ BoxRay objectBoxRay(const DeviceScene& s, std::uint32_t i, const RayPre& r, float tmax)
 {
     const InstanceRecord& in = s.instances[i];
@@ -33,11 +36,15 @@ __device__ __forceinline__ BoxRay objectBoxRay(const DeviceScene& s, std::uint32
 #pragma clang fp contract(off)
         for (int k = 0; k < 3; ++k) d[k] = (in.toObject[4 * k] * r.dx + in.toObject[4 * k + 1] * r.dy) + in.toObject[4 * k + 2] * r.dz;
     }
-    const float4 p = reinterpret_cast<const float4*>(s.instancePad)[i];  // invNorm, norm, |T|, object extent
+    // invNorm, norm, |T|, object extent; world box lo; world box hi.
+    const float4* q = reinterpret_cast<const float4*>(s.instancePad) + 3U * i;
+    const float4 p = q[0], lo = q[1], hi = q[2];
     const float oInf = fmaxf(fmaxf(fabsf(r.ox), fabsf(r.oy)), fabsf(r.oz));
     const float dInf = fmaxf(fmaxf(fabsf(r.dx), fabsf(r.dy)), fabsf(r.dz));
-    const float tm = fminf(tmax, 1e30f);
-    const float pad = 16.0f * 5.9604645e-8f * (p.x * (oInf + 2.0f * p.z + tm * dInf + p.y * p.w) + p.w) + 1e-6f * p.w;
+    const float reach = fmaxf(fmaxf(fmaxf(fabsf(lo.x - r.ox), fabsf(hi.x - r.ox)), fmaxf(fabsf(lo.y - r.oy), fabsf(hi.y - r.oy))),
+                              fmaxf(fabsf(lo.z - r.oz), fabsf(hi.z - r.oz)));
+    const float td = fminf(fminf(tmax, 1e30f) * dInf, reach * 1.00001f);
+    const float pad = 16.0f * 5.9604645e-8f * (p.x * (oInf + 2.0f * p.z + td + p.y * p.w) + p.w) + 1e-6f * p.w;
     return BoxRay{o[0], o[1], o[2], safeRcp(d[0]), safeRcp(d[1]), safeRcp(d[2]), pad};
 }
 

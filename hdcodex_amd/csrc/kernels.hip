@@ -580,12 +580,14 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                         }
                     }
 
+                    float lightTotal = 0.0f;  // set with the light sample; every later use has lightCount > 0
                     if (cacheStop) {
                     } else if (s.lightCount > 0U) {
                         // Analytic NEE (:1469-1592).
                         std::uint32_t li;
                         float selPdf;
-                        sampleAnalyticLight(s, hp, normal, c.diffE, dSpecE, c.translucentE, rng, li, selPdf);
+                        lightTotal = totalLightWeight(s, hp, normal, c.diffE, dSpecE, c.translucentE);
+                        sampleAnalyticLight(s, hp, normal, c.diffE, dSpecE, c.translucentE, lightTotal, rng, li, selPdf);
                         const GpuLightRecord& L = s.lights[li];
                         V3 ld = v3(0.0f, 1.0f, 0.0f);
                         V3 lrad = v3(0.0f);
@@ -653,7 +655,8 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                             const float direct = evalDirect(c, ld, L.controls.x, L.controls.y);
                             if (direct > 0.0f) {
                                 const float w = L.textureInfo.z == 0U
-                                    ? powerHeuristic(environmentLightPdf(s, hp, normal, ld, c.diffE, dSpecE, c.translucentE), surfacePdf(c, ld))
+                                    ? powerHeuristic(environmentLightPdf(s, hp, normal, ld, c.diffE, dSpecE, c.translucentE, lightTotal),
+                                                     surfacePdf(c, ld))
                                     : 1.0f;
                                 const float contribution = throughput * direct * illuminantSpectrum(lrad, wl) * invPdf * w / fmaxf(selPdf, 1e-8f);
                                 queueShadow(pb, pid, slot, offsetOrigin(hp, ogn, ld), ld, maxShadow, contribution, li);
@@ -852,7 +855,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                                     flags &= ~kPrevDelta;
                                     flags |= kEnvOnMiss;
                                     mis.y = s.lightCount > 0U
-                                        ? environmentLightPdf(s, hp, normal, rd, c.diffE, c.specE + lumGrey(c.coatContribution) + c.pSheen, c.translucentE)
+                                        ? environmentLightPdf(s, hp, normal, rd, c.diffE, dSpecE, c.translucentE, lightTotal)
                                         : 0.0f;
                                 }
                                 keep = true;

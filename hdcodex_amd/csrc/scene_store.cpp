@@ -211,7 +211,7 @@ void BuildTwoLevel(SceneStore* t, const hdcx_scene_view& v, hipStream_t st)
     std::vector<lbvh::Box4> items;
     std::vector<InstanceRecord> records(v.instance_count);
     std::vector<std::uint32_t> bases(v.instance_count);
-    std::vector<float> pads(static_cast<std::size_t>(v.instance_count) * 4);
+    std::vector<float> pads(static_cast<std::size_t>(v.instance_count) * 12);
     std::uint32_t base = drawn;
     for (std::uint32_t i = 0; i < v.instance_count; ++i) {
         const hdcx_instance& in = v.instances[i];
@@ -252,10 +252,15 @@ void BuildTwoLevel(SceneStore* t, const hdcx_scene_view& v, hipStream_t st)
             tInf = std::max(tInf, std::fabs(tr));
         }
         items.push_back(box);
-        pads[4 * i + 0] = static_cast<float>(normInv);
-        pads[4 * i + 1] = static_cast<float>(normM);
-        pads[4 * i + 2] = static_cast<float>(tInf);
-        pads[4 * i + 3] = static_cast<float>(objExt);
+        float* pad = &pads[12 * static_cast<std::size_t>(i)];
+        pad[0] = static_cast<float>(normInv);
+        pad[1] = static_cast<float>(normM);
+        pad[2] = static_cast<float>(tInf);
+        pad[3] = static_cast<float>(objExt);
+        for (int k = 0; k < 3; ++k) {
+            pad[4 + k] = box.lo[k];
+            pad[8 + k] = box.hi[k];
+        }
     }
     if (drawn > 0) items.push_back(blas.back().box);
     const Bvh4 tlas = BuildBvh4OverBoxes(items);
