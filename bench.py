@@ -178,13 +178,15 @@ def equal_time(hx, tracer, cam, plain_frame: np.ndarray, plain_ms: float, warmup
     return out
 
 
-def batch_workload(hx, threads: int, frames: int = 1, graph: bool = True) -> dict:
+def batch_workload(hx, threads: int, frames: int = 1, graph: bool = True, cache: bool = True) -> dict:
     """64 independent seeded mixed-material scenes (triangle targets
     log-uniform in [1e3, 5e5], square images of 128..1024 pixels, 8..64 spp,
     8 bounces) rendered together by the batch renderer: autotuned wave /
     tile / grid sizes, per-wave hipGraph replay. Reports aggregate Mpaths/s
     and each scene's rel-MSE against the FP64 oracle on 32 pixels (four rows
-    of eight) at the same seeds."""
+    of eight) at the same seeds. With `cache`, also per-scene radiance caches
+    against the plain batch at equal wall time, both scored per scene
+    against FP64 oracle references of the same pixels (2 x 256 spp)."""
     import oracle
 
     rng = np.random.default_rng(2024)
@@ -203,6 +205,7 @@ def batch_workload(hx, threads: int, frames: int = 1, graph: bool = True) -> dic
         b.add_scene(scene, hx.BVH_DEVICE_LBVH, 2048)
         b.set_view(i, scene.camera(res, res), res, res, spp)
     setup_s = time.perf_counter() - t0
+    log(f"batch: 64 scenes set up in {setup_s:.1f} s")
     t0 = time.perf_counter()
     if graph:
         tuning = b.autotune(BOUNCES, None)
@@ -219,25 +222,140 @@ def batch_workload(hx, threads: int, frames: int = 1, graph: bool = True) -> dic
         b.render(0, BOUNCES)
         times.append(time.perf_counter() - t0)
     stats = b.stats()
-    errs = []
+    log(f"batch: plain frame {min(times) * 1e3:.0f} ms")
+    plain_films = [b.read_film(i) for i in range(len(specs))]
+    errs, refs, subsets = [], [], []
+    ref_half = 256
     for (i, tris, res, spp), scene in zip(specs, scenes):
-        film = b.read_film(i)
         rows = np.linspace(res // 8, res - res // 8, 4).astype(np.uint32)
         col_step = res // 8
-        ref = oracle.OracleScene(scene.view).render(scene.camera(res, res), res, res, 0, BOUNCES, spp, precision=64,
-                                                    rows=rows, col_step=col_step, threads=threads) / spp
-        x = film[rows.astype(np.int64)][:, ::col_step, :3].astype(np.float64)
+        subsets.append((rows.astype(np.int64), col_step))
+        o = oracle.OracleScene(scene.view)
+        cam = scene.camera(res, res)
+        ref = o.render(cam, res, res, 0, BOUNCES, spp, precision=64, rows=rows, col_step=col_step, threads=threads) / spp
+        x = plain_films[i][rows.astype(np.int64)][:, ::col_step, :3].astype(np.float64)
         errs.append(float(np.mean((x - ref) ** 2 / (ref * ref + 0.01))))
-    b.close()
+        if i % 16 == 15:
+            log(f"batch: oracle scenes {i + 1}/64")
+        if cache:
+            halves = [o.render(cam, res, res, 100000 + h * ref_half, BOUNCES, ref_half, precision=64, rows=rows,
+                               col_step=col_step, threads=threads) / ref_half for h in range(2)]
+            refs.append(halves)
     ms = min(times) * 1e3
-    return {"workload": "64 mixed-material scenes, 1e3..5e5 triangles (log-uniform), 128^2..1024^2, 8..64 spp, "
-                        "8 bounces, one batch (seeded)",
-            "scenes": 64, "triangles": int(sum(sc.triangle_count for sc in scenes)),
-            "pixel_samples": stats["pixel_samples"], "ms_per_frame": ms,
-            "mpaths_per_s": stats["pixel_samples"] / (ms * 1e-3) / 1e6, "waves": stats["waves"],
-            "graph_launches": stats["graph_launches"], "tuning": tuning, "autotune_s": tune_s, "setup_s": setup_s,
-            "rel_mse_vs_fp64_oracle": {"max": max(errs), "median": float(np.median(errs)), "per_scene": errs,
-                                       "sample": "32 pixels per scene (4 rows x 8 columns), same seeds"}}
+    out = {"workload": "64 mixed-material scenes, 1e3..5e5 triangles (log-uniform), 128^2..1024^2, 8..64 spp, "
+                       "8 bounces, one batch (seeded)",
+           "scenes": 64, "triangles": int(sum(sc.triangle_count for sc in scenes)),
+           "pixel_samples": stats["pixel_samples"], "ms_per_frame": ms,
+           "mpaths_per_s": stats["pixel_samples"] / (ms * 1e-3) / 1e6, "waves": stats["waves"],
+           "graph_launches": stats["graph_launches"], "tuning": tuning, "autotune_s": tune_s, "setup_s": setup_s,
+           "rel_mse_vs_fp64_oracle": {"max": max(errs), "median": float(np.median(errs)), "per_scene": errs,
+                                      "sample": "32 pixels per scene (4 rows x 8 columns), same seeds"}}
+    if cache:
+        out["equal_time"] = batch_equal_time(b, specs, scenes, plain_films, ms, refs, subsets)
+    b.close()
+    return out
+
+
+def batch_equal_time(b, specs, scenes, plain_films, plain_ms: float, refs, subsets, warm_frames: int = 20) -> dict:
+    """Per-scene radiance caches vs the plain batch at equal wall time. The
+    caches are warm: warm_frames renders of 2 spp per scene train them (not
+    timed); then every view's spp is scaled so that one render (its training
+    included) takes about the plain render's time."""
+    def score(film, halves, sub):
+        rows, step = sub
+        a, c = halves
+        y = 0.5 * (a + c)
+        floor = float(np.mean((a - c) ** 2 / (y * y + 0.01))) / 4.0
+        x = film[rows][:, ::step, :3].astype(np.float64)
+        e = float(np.mean((x - y) ** 2 / (y * y + 0.01)))
+        return e, e - floor
+
+    def set_spp(scale: float):
+        out = []
+        for (i, _, res, spp), scene in zip(specs, scenes):
+            n = max(1, int(round(spp * scale)))
+            b.set_view(i, scene.camera(res, res), res, res, n)
+            out.append(n)
+        return out
+
+    config = dict(train_samples=16384, terminate_bounce=1, train_steps=4, learning_rate=1e-2, seed=1, ema_decay=0.99)
+    b.set_radiance_cache(True, **config)
+    for (i, _, res, _), scene in zip(specs, scenes):
+        b.set_view(i, scene.camera(res, res), res, res, 2)
+    t0 = time.perf_counter()
+    for f in range(warm_frames):
+        b.render(1000 + 2 * f, BOUNCES)
+    warm_s = time.perf_counter() - t0
+    log(f"batch cache: {warm_frames} warm frames in {warm_s:.1f} s")
+    set_spp(1.0)
+    t0 = time.perf_counter()
+    b.render(5000, BOUNCES)
+    unit_s = time.perf_counter() - t0
+    spps = set_spp(plain_ms * 1e-3 / unit_s)
+    t0 = time.perf_counter()
+    b.render(0, BOUNCES)
+    cache_ms = (time.perf_counter() - t0) * 1e3
+    log(f"batch cache: equal-time frame {cache_ms:.0f} ms")
+    rc = b.rc_stats()
+    plain, cached = [], []
+    for i in range(len(specs)):
+        plain.append(score(plain_films[i], refs[i], subsets[i]))
+        cached.append(score(b.read_film(i), refs[i], subsets[i]))
+    b.set_radiance_cache(False)
+    for (i, _, res, spp), scene in zip(specs, scenes):
+        b.set_view(i, scene.camera(res, res), res, res, spp)
+    ratio = [p[0] / max(c[0], 1e-30) for p, c in zip(plain, cached)]
+    return {"config": {**config, "train_samples_per_scene": config["train_samples"]}, "warm_frames": warm_frames,
+            "warm_spp": 2, "warm_s": warm_s, "plain_ms": plain_ms, "cache_ms": cache_ms,
+            "cache_spp_scale": spps[0] / specs[0][3], "cache_spp": spps,
+            "plain_rel_mse": [p[0] for p in plain], "cache_rel_mse": [c[0] for c in cached],
+            "plain_rel_mse_debiased": [p[1] for p in plain], "cache_rel_mse_debiased": [c[1] for c in cached],
+            "ratio_plain_over_cache": ratio, "median_ratio": float(np.median(ratio)),
+            "scenes_improved": int(sum(r > 1.0 for r in ratio)),
+            "mean_plain_rel_mse": float(np.mean([p[0] for p in plain])),
+            "mean_cache_rel_mse": float(np.mean([c[0] for c in cached])),
+            "cache_queries": rc["queries"], "mean_loss": rc["mean_loss"],
+            "reference": "FP64 oracle, same 32 pixels per scene, two independent halves of 256 spp"}
+
+
+def cornell_workload(hx, threads: int, frames: int = 3) -> dict:
+    """Seeded Cornell box (5 wall quads + 2 rotated boxes = 34 triangles, one
+    0.25 x 0.25 rect light, Lambert albedos), 256 x 256, 64 spp, 8 bounces:
+    GPU throughput, the frame's rel-MSE against the FP64 oracle over every
+    pixel (same seeds), and the FP32 restatement timed over the whole frame
+    on the host's cores."""
+    import oracle
+
+    w = h = 256
+    spp = 64
+    scene = hx.ProceduralScene("cornell", 0)
+    cam = scene.camera(w, h)
+    t = hx.PathTracer(0)
+    try:
+        t.set_scene(scene)
+        t.render(cam, w, h, 0, BOUNCES, spp, readback=False)
+        times = []
+        for _ in range(frames):
+            t0 = time.perf_counter()
+            t.render(cam, w, h, 0, BOUNCES, spp, readback=False)
+            times.append(time.perf_counter() - t0)
+        frame = t.read_film(w, h)[..., :3].astype(np.float64)
+    finally:
+        t.close()
+    o = oracle.OracleScene(scene.view)
+    t0 = time.perf_counter()
+    ref32 = o.render(cam, w, h, 0, BOUNCES, spp, precision=32, threads=threads) / spp
+    cpu_s = time.perf_counter() - t0
+    ref64 = o.render(cam, w, h, 0, BOUNCES, spp, precision=64, threads=threads) / spp
+    paths = w * h * spp
+    gpu_s = min(times)
+    return {"workload": "Cornell box, 34 triangles, 256x256, 64 spp x 3 lanes, 8 bounces (seed 0)",
+            "ms_per_frame": gpu_s * 1e3, "mpaths_per_s": paths / gpu_s / 1e6,
+            "rel_mse_vs_fp64_oracle": float(np.mean((frame - ref64) ** 2 / (ref64 * ref64 + 0.01))),
+            "rel_mse_vs_fp32_restatement": float(np.mean((frame - ref32) ** 2 / (ref32 * ref32 + 0.01))),
+            "pixels_compared": w * h,
+            "cpu_baseline": {"value": paths / cpu_s / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+                             "sample": f"FP32 restatement over the whole frame ({paths} paths) in {cpu_s:.2f} s"}}
 
 
 def instanced_workload(hx, frames: int = 2) -> dict:
@@ -339,7 +457,8 @@ def main() -> int:
     if args.batch_only:
         import hdcodex_amd as hx
 
-        print(json.dumps({"batch": batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph)}),
+        print(json.dumps({"batch": batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph,
+                                                  cache=not args.no_graph)}),
               flush=True)
         return 0
     if args.instanced_only:
@@ -470,12 +589,15 @@ def main() -> int:
         log("parity vs FP64 oracle (subset)")
         result["parity"] = parity(scene, cam, frame)
         result["parity_rel_mse_vs_fp64_oracle"] = result["parity"]["rel_mse_vs_fp64_oracle"]
+    log("cornell workload")
+    result["cornell"] = cornell_workload(hx, min(os.cpu_count() or 1, 16))
     if not args.no_instanced:
         log("instanced workload")
         result["instanced"] = instanced_workload(hx)
     if args.batch:
         log("batch workload")
-        result["batch"] = batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph)
+        result["batch"] = batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph,
+                                         cache=not args.no_graph)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         threads = min(os.cpu_count() or 1, 16)

@@ -490,6 +490,14 @@ class BatchStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchRcStats(ctypes.Structure):
+    _fields_ = [("queries", c_uint64), ("train_samples", c_uint64), ("adam_steps", c_uint64), ("train_ms", c_double),
+                ("mean_loss", c_float), ("scenes_trained", c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 def _bind_batch(lib: ctypes.CDLL) -> None:
     lib.hdcx_batch_create.argtypes = [c_int, POINTER(c_void_p)]
     lib.hdcx_batch_destroy.argtypes = [c_void_p]
@@ -501,6 +509,8 @@ def _bind_batch(lib: ctypes.CDLL) -> None:
     lib.hdcx_batch_render.argtypes = [c_void_p, c_uint32, c_uint32]
     lib.hdcx_batch_read_film.argtypes = [c_void_p, c_uint32, c_void_p]
     lib.hdcx_batch_get_stats.argtypes = [c_void_p, POINTER(BatchStats)]
+    lib.hdcx_batch_set_radiance_cache.argtypes = [c_void_p, POINTER(RadianceCacheConfig)]
+    lib.hdcx_batch_get_rc_stats.argtypes = [c_void_p, POINTER(BatchRcStats)]
 
 
 class BatchRenderer:
@@ -554,6 +564,20 @@ class BatchRenderer:
         _check(library().hdcx_batch_get_stats(self._h, ctypes.byref(s)))
         return s.as_dict()
 
+    def set_radiance_cache(self, enabled: bool = True, train_samples: int = 65536, terminate_bounce: int = 1,
+                           train_steps: int = 4, learning_rate: float = 1e-2, seed: int = 0,
+                           ema_decay: float = 0.99, spread_threshold: float = 0.0) -> None:
+        """One cache per scene (hdcx_batch_set_radiance_cache); train_samples
+        per scene and render."""
+        c = RadianceCacheConfig(int(enabled), train_samples, terminate_bounce, train_steps, learning_rate, 0.9, 0.99,
+                                1e-8, ema_decay, spread_threshold, seed)
+        _check(library().hdcx_batch_set_radiance_cache(self._h, ctypes.byref(c)))
+
+    def rc_stats(self) -> dict:
+        s = BatchRcStats()
+        _check(library().hdcx_batch_get_rc_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
     def close(self) -> None:
         if self._h:
             _check(library().hdcx_batch_destroy(self._h))
@@ -569,4 +593,4 @@ class BatchRenderer:
 EXPORTED_SYMBOLS = EXPORTED_SYMBOLS + (
     "hdcx_batch_create", "hdcx_batch_destroy", "hdcx_batch_add_scene", "hdcx_batch_set_view", "hdcx_batch_set_tuning",
     "hdcx_batch_get_tuning", "hdcx_batch_autotune", "hdcx_batch_render", "hdcx_batch_read_film",
-    "hdcx_batch_get_stats")
+    "hdcx_batch_get_stats", "hdcx_batch_set_radiance_cache", "hdcx_batch_get_rc_stats")

@@ -2,10 +2,13 @@
 // path pool; every stage is one launch over all scenes, and the per-wave
 // launch sequence is captured once in a hipGraph and replayed for every
 // wave (the wave index lives in device memory). Includes the autotuner of
-// the wave / tile / grid sizes. C ABI in include/hdcx.h.
+// the wave / tile / grid sizes and one radiance cache per scene. C ABI in
+// include/hdcx.h.
 #include "../../include/hdcx.h"
 #include "device_buffer.h"
 #include "kernel_api.h"
+#include "rc_config.h"
+#include "rcache.h"
 #include "scene_store.h"
 
 #include <hip/hip_runtime.h>
@@ -46,8 +49,13 @@ struct GraphKey {
     const void* pool = nullptr;
     const void* tables[7] = {};
     std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0, twoLevel = 0, walk = 0;
+    std::uint32_t scenes = 0;  // also keeps the struct free of implicit padding (keys compare with memcmp)
+    CacheParams cache{};
+    const void* rc[6] = {};    // per-scene weights, sorted queries, bucket arrays
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
+static_assert(sizeof(CacheParams) == 10 * 4 + 10 * 8, "CacheParams has implicit padding");
+static_assert(sizeof(GraphKey) == 8 * 8 + 8 * 4 + sizeof(CacheParams) + 6 * 8, "GraphKey has implicit padding");
 
 // Capacity-preserving upload (stable device addresses for a captured graph).
 void Put(DeviceBuffer& b, const void* host, std::size_t bytes, hipStream_t st)
@@ -74,6 +82,22 @@ struct hdcx_batch : PathPool {
     hipGraphExec_t graphExec = nullptr;
     GraphKey graphKey{};
 
+    // Radiance caches: the networks of all scenes side by side (scene s at
+    // s * kRcParams; transposes at s * 4 * 4096), trained after each render
+    // on the samples it gathered (sampleCapacity per scene).
+    static constexpr unsigned kRcGradBlocks = 256;
+    static constexpr std::uint32_t kRecordVertices = 4;
+    hdcx_radiance_cache_config rcConfig{};
+    bool rcEnabled = false;
+    std::uint64_t rcSeed = 0;
+    std::uint32_t rcScenes = 0, rcFrame = 0;
+    std::vector<int> rcStep;
+    DeviceBuffer rcMaster, rcM, rcV, rcEma, rcW, rcWt, rcWe, rcWte, rcGrad, rcLoss;
+    DeviceBuffer rcQueries, rcSorted, rcCounters, rcRecords, rcRecordThroughput, rcTrainSlot, rcSamples, rcSceneSamples,
+        rcSceneTrain, rcBucketCount, rcBucketStart, rcBucketBlocks, rcBucketCursor;
+    hdcx_batch_rc_stats rcStats{};
+    std::vector<uint2> rcTrainHost;  // host copy of the per-scene training table (alive until the render syncs)
+
     ~hdcx_batch()
     {
         DropGraph();
@@ -87,6 +111,175 @@ struct hdcx_batch : PathPool {
         graphExec = nullptr;
         graph = nullptr;
         graphKey = GraphKey{};
+    }
+
+    RcState RcSceneState(std::uint32_t k)
+    {
+        const std::size_t p = static_cast<std::size_t>(k) * kRcParams, t = static_cast<std::size_t>(k) * 4 * 4096;
+        RcState r{};
+        r.master = rcMaster.as<float>() + p;
+        r.m = rcM.as<float>() + p;
+        r.v = rcV.as<float>() + p;
+        r.grad = rcGrad.as<float>();
+        r.w = rcW.as<__bf16>() + p;
+        r.wt = rcWt.as<__bf16>() + t;
+        r.ema = rcEma.as<float>() + p;
+        r.we = rcWe.as<__bf16>() + p;
+        r.wte = rcWte.as<__bf16>() + t;
+        r.emaDecay = rcConfig.ema_decay;
+        r.loss = rcLoss.as<float>();
+        return r;
+    }
+
+    // Weights of every scene from seed + scene index, Adam state cleared.
+    void EnsureRc()
+    {
+        const auto n = static_cast<std::uint32_t>(scenes.size());
+        if (rcScenes == n && rcSeed == rcConfig.seed && rcMaster.bytes()) return;
+        const std::size_t params = static_cast<std::size_t>(n) * kRcParams;
+        rcMaster.Allocate(params * 4);
+        rcM.Allocate(params * 4);
+        rcV.Allocate(params * 4);
+        rcEma.Allocate(params * 4);
+        rcW.Allocate(params * 2);
+        rcWe.Allocate(params * 2);
+        rcWt.Allocate(static_cast<std::size_t>(n) * 4 * 4096 * 2);
+        rcWte.Allocate(static_cast<std::size_t>(n) * 4 * 4096 * 2);
+        rcGrad.Allocate(static_cast<std::size_t>(kRcGradBlocks) * kRcParams * 4);
+        rcLoss.Allocate(8);
+        std::vector<float> w(params);
+        for (std::uint32_t k = 0; k < n; ++k) InitRcWeights(rcConfig.seed + k, w.data() + static_cast<std::size_t>(k) * kRcParams);
+        Check(hipMemcpyAsync(rcMaster.as<void>(), w.data(), params * 4, hipMemcpyHostToDevice, stream), "batch rc init");
+        Check(hipMemsetAsync(rcM.as<void>(), 0, rcM.bytes(), stream), "batch rc init");
+        Check(hipMemsetAsync(rcV.as<void>(), 0, rcV.bytes(), stream), "batch rc init");
+        Check(hipMemsetAsync(rcGrad.as<void>(), 0, rcGrad.bytes(), stream), "batch rc init");
+        for (std::uint32_t k = 0; k < n; ++k) Check(LaunchRcPublish(RcSceneState(k), stream), "batch rc publish");
+        Check(hipStreamSynchronize(stream), "batch rc init");
+        rcStep.assign(n, 0);
+        rcScenes = n;
+        rcSeed = rcConfig.seed;
+        rcFrame = 0;
+    }
+
+    // Cache hooks of one render: per-scene training strides (about
+    // train_samples samples per scene: 3 lanes x ~2.5 recorded vertices per
+    // training pixel sample), record slots for the whole frame, query queue
+    // for a wave.
+    CacheParams PrepareRc(std::uint32_t maxPaths)
+    {
+        EnsureRc();
+        const auto n = static_cast<std::uint32_t>(scenes.size());
+        std::vector<uint2>& train = rcTrainHost;
+        train.assign(n, uint2{0U, 0U});
+        std::uint64_t slots = 0;
+        const std::uint64_t wanted = std::max<std::uint64_t>(1, rcConfig.train_samples * 2ULL / 15ULL);
+        for (std::uint32_t k = 0; k < n; ++k) {
+            const ViewState& v = views[k];
+            if (!v.set || v.samples == 0 || !scenes[k]->geometry) continue;
+            const std::uint64_t frame = static_cast<std::uint64_t>(v.width) * v.height * v.samples;
+            const auto stride = static_cast<std::uint32_t>(std::max<std::uint64_t>(1, frame / wanted));
+            train[k] = uint2{stride, static_cast<std::uint32_t>((rcFrame * 2654435761ULL) % stride)};
+            slots += (frame / stride + 2) * 3;
+        }
+        const std::uint32_t cap = std::max(rcConfig.train_samples, 1U);
+        const std::size_t records = static_cast<std::size_t>(slots) * kRecordVertices;
+        if (rcRecords.bytes() < records * sizeof(RcQuery)) {
+            rcRecords.Allocate(records * sizeof(RcQuery));
+            // Every record starts unused (lane ~0); the gather marks consumed ones so again.
+            Check(hipMemsetAsync(rcRecords.as<void>(), 0xff, rcRecords.bytes(), stream), "batch rc records");
+            rcRecordThroughput.Allocate(records * 4);
+        }
+        rcQueries.Reserve(static_cast<std::size_t>(maxPaths) * sizeof(RcQuery));
+        rcSorted.Reserve(static_cast<std::size_t>(maxPaths) * sizeof(RcQuery));
+        rcTrainSlot.Reserve(static_cast<std::size_t>(maxPaths) * 4);
+        rcSamples.Reserve(static_cast<std::size_t>(n) * cap * sizeof(RcQuery));
+        rcSceneSamples.Reserve(static_cast<std::size_t>(n) * 4);
+        rcBucketCount.Reserve(static_cast<std::size_t>(n) * 4);
+        rcBucketCursor.Reserve(static_cast<std::size_t>(n) * 4);
+        rcBucketStart.Reserve(static_cast<std::size_t>(n + 1) * 4);
+        rcBucketBlocks.Reserve(static_cast<std::size_t>(n + 1) * 4);
+        rcCounters.Reserve(64);
+        Check(hipMemsetAsync(rcCounters.as<void>(), 0, 64, stream), "batch rc counters");
+        Check(hipMemsetAsync(rcSceneSamples.as<void>(), 0, static_cast<std::size_t>(n) * 4, stream), "batch rc counters");
+        Check(hipMemsetAsync(rcBucketCount.as<void>(), 0, static_cast<std::size_t>(n) * 4, stream), "batch rc counters");
+        Put(rcSceneTrain, train.data(), train.size() * sizeof(uint2), stream);
+
+        CacheParams cp{};
+        cp.enabled = 1;
+        cp.terminateBounce = rcConfig.terminate_bounce;
+        cp.spreadThreshold = rcConfig.spread_threshold;
+        cp.trainStride = 1;
+        cp.recordVertices = kRecordVertices;
+        cp.queryCapacity = maxPaths;
+        cp.recordSlots = static_cast<std::uint32_t>(slots);
+        cp.sampleCapacity = cap;
+        auto* counters = rcCounters.as<std::uint32_t>();
+        cp.queries = rcQueries.as<RcQuery>();
+        cp.queryCount = counters;        // [0]; [2..3]: queries answered (64-bit)
+        cp.recordCount = counters + 4;
+        cp.records = rcRecords.as<RcQuery>();
+        cp.recordThroughput = rcRecordThroughput.as<float>();
+        cp.samples = rcSamples.as<RcQuery>();
+        cp.sampleCount = counters + 1;
+        cp.trainSlot = rcTrainSlot.as<std::uint32_t>();
+        cp.sceneTrain = rcSceneTrain.as<uint2>();
+        cp.sceneSampleCount = rcSceneSamples.as<std::uint32_t>();
+        return cp;
+    }
+
+    RcSceneBuckets Buckets()
+    {
+        RcSceneBuckets b{};
+        b.count = rcBucketCount.as<std::uint32_t>();
+        b.start = rcBucketStart.as<std::uint32_t>();
+        b.blockStart = rcBucketBlocks.as<std::uint32_t>();
+        b.cursor = rcBucketCursor.as<std::uint32_t>();
+        b.scenes = static_cast<std::uint32_t>(scenes.size());
+        return b;
+    }
+
+    // Adam steps per scene on the samples the render gathered.
+    void TrainRc()
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        const auto n = static_cast<std::uint32_t>(scenes.size());
+        std::vector<std::uint32_t> counts(n, 0U);
+        std::uint64_t answered = 0;
+        Check(hipMemcpyAsync(counts.data(), rcSceneSamples.as<void>(), n * 4, hipMemcpyDeviceToHost, stream), "batch rc counts");
+        Check(hipMemcpyAsync(&answered, rcCounters.as<std::uint32_t>() + 2, 8, hipMemcpyDeviceToHost, stream), "batch rc counts");
+        Check(hipStreamSynchronize(stream), "batch rc counts");
+        Check(hipMemsetAsync(rcLoss.as<void>(), 0, 8, stream), "batch rc loss");
+        const std::uint32_t cap = std::max(rcConfig.train_samples, 1U);
+        const std::uint32_t steps = std::max(rcConfig.train_steps, 1U);
+        std::uint32_t trained = 0;
+        for (std::uint32_t k = 0; k < n; ++k) {
+            const std::uint32_t count = std::min(counts[k], cap);
+            if (count == 0) continue;
+            ++trained;
+            const RcQuery* samples = rcSamples.as<RcQuery>() + static_cast<std::size_t>(k) * cap;
+            const RcState st = RcSceneState(k);
+            for (std::uint32_t i = 0; i < steps; ++i) {
+                const auto b0 = static_cast<std::uint32_t>(static_cast<std::uint64_t>(count) * i / steps);
+                const auto b1 = static_cast<std::uint32_t>(static_cast<std::uint64_t>(count) * (i + 1) / steps);
+                if (b1 <= b0) continue;
+                Check(LaunchRcGradient(RcNetwork{st.w, st.wt}, samples + b0, nullptr, b1 - b0, st.grad, st.loss,
+                                       kRcGradBlocks, stream),
+                      "batch rc gradient");
+                Check(LaunchRcAdamBlocks(st, static_cast<int>(kRcGradBlocks), ++rcStep[k], rcConfig.learning_rate,
+                                         rcConfig.beta1, rcConfig.beta2, rcConfig.epsilon, stream),
+                      "batch rc adam");
+                ++rcStats.adam_steps;
+                rcStats.train_samples += b1 - b0;
+            }
+        }
+        float loss[2] = {0.0F, 0.0F};
+        Check(hipMemcpyAsync(loss, rcLoss.as<void>(), 8, hipMemcpyDeviceToHost, stream), "batch rc loss");
+        Check(hipStreamSynchronize(stream), "batch rc train");
+        rcStats.queries = answered;
+        rcStats.mean_loss = loss[1] > 0.0F ? loss[0] / loss[1] : 0.0F;
+        rcStats.scenes_trained = trained;
+        rcStats.train_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        ++rcFrame;
     }
 
     void MakePlan()
@@ -203,24 +396,36 @@ struct hdcx_batch : PathPool {
         const int walk = subsurface ? kShadeWalkDefer : kShadeWalkInline;
         const PathBuffers pb = Paths();
         const std::uint32_t maxPaths = plan.maxPaths, maxPixels = plan.maxPixels;
+        const bool cache = rcEnabled;
+        const CacheParams cp = cache ? PrepareRc(maxPaths) : CacheParams{};
+        const RcSceneBuckets buckets = cache ? Buckets() : RcSceneBuckets{};
         auto enqueueWave = [&] {
-            Check(LaunchRaygenBatch(pb, bw, maxPaths, maxBlocks, st), "batch raygen");
+            Check(LaunchRaygenBatch(pb, bw, cp, maxPaths, maxBlocks, st), "batch raygen");
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
                 Check(LaunchExtendBatch(table, twoLevel, pb, in, maxPaths, maxBlocks, nullptr, st), "batch extend");
                 const int last = b + 1 == bounces ? 1 : 0;
-                Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, walk, maxPaths, maxBlocks, st),
+                Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, walk, maxPaths, maxBlocks, st),
                       "batch shade");
                 if (walk == kShadeWalkDefer && !last) {
                     Check(LaunchWalkBatch(table, twoLevel, pb, maxPaths, maxBlocks, st), "batch walk");
-                    Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, kShadeWalkResume, maxPaths,
+                    Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, kShadeWalkResume, maxPaths,
                                            maxBlocks, st),
                           "batch shade (resume)");
                 }
+                if (cache && (b == cp.terminateBounce || (b > cp.terminateBounce && cp.spreadThreshold > 0.0F))) {
+                    // Queries of this bounce, answered by their scenes' networks.
+                    Check(LaunchRcInferScenes(rcWe.as<__bf16>(), cp.queries, cp.queryCount, maxPaths, pb.pathScene, buckets,
+                                              rcSorted.as<RcQuery>(), pb.radiance, pb.throughput, maxBlocks, st),
+                          "batch rc infer");
+                    Check(LaunchConsumeQueries(cp, st), "batch rc queries");
+                }
                 Check(LaunchShadowBatch(table, twoLevel, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
             }
+            if (cache)
+                Check(LaunchGatherTraining(pb, cp, cp.recordSlots * kRecordVertices, maxBlocks, st), "batch rc samples");
             Check(LaunchFilmBatch(pb, bw, maxPixels, maxBlocks, st), "batch film");
-            Check(LaunchAdvanceWave(bw, st), "batch cursor");
+            Check(LaunchAdvanceWave(bw, cache ? cp.recordCount : nullptr, st), "batch cursor");
         };
         const auto waves = static_cast<std::uint32_t>(plan.wavePaths.size());
         if (tuning.use_graph) {
@@ -235,6 +440,13 @@ struct hdcx_batch : PathPool {
             key.maxPixels = maxPixels;
             key.twoLevel = twoLevel ? 1U : 0U;
             key.walk = static_cast<std::uint32_t>(walk);
+            key.scenes = static_cast<std::uint32_t>(scenes.size());
+            key.cache = cp;
+            if (cache) {
+                const void* r[6] = {rcWe.as<void>(), rcSorted.as<void>(), buckets.count, buckets.start, buckets.blockStart,
+                                    buckets.cursor};
+                std::memcpy(key.rc, r, sizeof(r));
+            }
             if (!graphExec || !(key == graphKey)) {
                 DropGraph();
                 Check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
@@ -263,6 +475,7 @@ struct hdcx_batch : PathPool {
                   "batch resolve");
         }
         Check(hipStreamSynchronize(st), "batch render");
+        if (cache) TrainRc();
         stats.pixel_samples = plan.pixelSamples;
         stats.lane_paths = plan.lanePaths;
         stats.waves = waves;
@@ -472,6 +685,29 @@ int hdcx_batch_get_stats(hdcx_batch* batch, hdcx_batch_stats* out_stats)
     return Guard([&] {
         if (!batch || !out_stats) throw std::invalid_argument("null argument");
         *out_stats = batch->stats;
+    });
+}
+
+int hdcx_batch_set_radiance_cache(hdcx_batch* batch, const hdcx_radiance_cache_config* config)
+{
+    return Guard([&] {
+        if (!batch) throw std::invalid_argument("batch is null");
+        if (!config || !config->enabled) {
+            batch->rcEnabled = false;
+            return;
+        }
+        batch->Activate();
+        batch->rcConfig = NormalizedRcConfig(*config);
+        batch->EnsureRc();
+        batch->rcEnabled = true;
+    });
+}
+
+int hdcx_batch_get_rc_stats(hdcx_batch* batch, hdcx_batch_rc_stats* out_stats)
+{
+    return Guard([&] {
+        if (!batch || !out_stats) throw std::invalid_argument("null argument");
+        *out_stats = batch->rcStats;
     });
 }
 

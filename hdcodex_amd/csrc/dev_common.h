@@ -26,6 +26,11 @@ struct TexDesc {
 struct ImportanceDesc {
     std::uint64_t marginal, conditional, pdf;  // float offsets into the pool
     std::uint32_t width, height;
+    // Guide tables (uint32 words in the pool), 0 when a CDF is not monotone:
+    // rowGuide has 2^rowShift + 1 entries, colGuide 2^colShift + 1 per row;
+    // entry g is the first index whose CDF value is >= g / 2^shift.
+    std::uint64_t rowGuide, colGuide;
+    std::uint32_t rowShift, colShift;
 };
 
 struct DeviceScene {
@@ -51,6 +56,8 @@ struct DeviceScene {
     std::uint32_t textureCount;
     std::uint32_t nodeCount;
     std::uint32_t hasSubsurface;       // some material can take the subsurface random walk
+    float boundsLo[3];                 // world box of the scene (radiance-cache position normalisation)
+    float boundsInvExtent[3];
     // Two-level scenes (instancing); instanceCount == 0 for one level.
     std::uint32_t instanceCount;
     std::uint32_t drawnTriangles;      // primitive ids below this are drawn triangles
@@ -136,15 +143,23 @@ struct CacheParams {
     std::uint32_t recordVertices;
     std::uint32_t queryCapacity;
     std::uint32_t recordSlots;     // training pixel-sample slots in the record buffer
-    std::uint32_t sampleCapacity;  // training samples kept per frame
-    float lo[3];                   // scene box used to normalise positions
-    float invExtent[3];
+    std::uint32_t sampleCapacity;  // training samples kept per frame (per scene in a batch)
+    std::uint32_t reserved;        // keeps the struct free of implicit padding
     RcQuery* queries;              // per chunk
     std::uint32_t* queryCount;
     RcQuery* records;              // [slot][lane][vertex]; target holds the radiance after emission
     float* recordThroughput;
     RcQuery* samples;              // per frame
     std::uint32_t* sampleCount;
+    // Batch renderer (null in a single render): record slots are handed out
+    // per lane path at raygen (trainSlot[path], ~0 = not training; the
+    // records are [slot][vertex]), training pixel samples are chosen per
+    // scene, and samples go to the path's scene: samples[scene *
+    // sampleCapacity + k], k counted in sceneSampleCount[scene].
+    std::uint32_t* trainSlot;
+    std::uint32_t* recordCount;    // slots handed out in the current wave
+    const uint2* sceneTrain;       // per scene: (trainStride, trainOffset); stride 0 = no training
+    std::uint32_t* sceneSampleCount;
 };
 
 // Pixel-sample chunk: samples [s0, s1) x pixels [p0, p1); path id =

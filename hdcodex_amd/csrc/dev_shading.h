@@ -470,6 +470,28 @@ __device__ inline float domeLightPdf(const DeviceScene& s, std::uint32_t li, con
     const int ty = min(max(static_cast<int>(v * static_cast<float>(d.height)), 0), static_cast<int>(d.height) - 1);
     return fmaxf(s.importancePool[d.pdf + static_cast<std::size_t>(ty) * d.width + tx], 0.0f);
 }
+// First index i of cdf[0, n) with u <= cdf[i] (n - 1 when there is none),
+// the binary search of sampleDomeLight. With a guide table the search runs
+// only between guide[g] and guide[g + 1], g = floor(u * 2^shift) (exact:
+// a power of two), which hold the answer when the CDF is monotone: every
+// index below guide[g] has a value < g / 2^shift <= u, and cdf[guide[g + 1]]
+// >= (g + 1) / 2^shift > u. Same index, a few probes instead of log2(n).
+__device__ __forceinline__ int cdfSearch(const float* cdf, int n, float u, const std::uint32_t* guide, std::uint32_t shift)
+{
+    int lo = 0, hi = n - 1;
+    if (guide) {
+        const std::uint32_t buckets = 1U << shift;
+        const std::uint32_t g = min(static_cast<std::uint32_t>(u * static_cast<float>(buckets)), buckets - 1U);
+        lo = static_cast<int>(guide[g]);
+        hi = static_cast<int>(guide[g + 1U]);
+    }
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (u <= cdf[mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
 // sampleDomeLight (:600-650): row by marginal CDF, column by conditional
 // CDF, cos(theta) stratified within the row.
 __device__ inline void sampleDomeLight(const DeviceScene& s, std::uint32_t li, const GpuLightRecord& L,
@@ -481,23 +503,15 @@ __device__ inline void sampleDomeLight(const DeviceScene& s, std::uint32_t li, c
         return;
     }
     const ImportanceDesc d = s.importance[li];
-    const float* marginal = s.importancePool + d.marginal;
+    const std::uint32_t* guides = reinterpret_cast<const std::uint32_t*>(s.importancePool);
     const float rowTarget = randomFloat(rng);
-    int lo = 0, hi = static_cast<int>(d.height) - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) / 2;
-        if (rowTarget <= marginal[mid]) hi = mid; else lo = mid + 1;
-    }
-    const int row = lo;
-    const float* cond = s.importancePool + d.conditional + static_cast<std::size_t>(row) * d.width;
+    const int row = cdfSearch(s.importancePool + d.marginal, static_cast<int>(d.height), rowTarget,
+                              d.rowGuide ? guides + d.rowGuide : nullptr, d.rowShift);
     const float colTarget = randomFloat(rng);
-    lo = 0;
-    hi = static_cast<int>(d.width) - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi) / 2;
-        if (colTarget <= cond[mid]) hi = mid; else lo = mid + 1;
-    }
-    const int col = lo;
+    const int col = cdfSearch(s.importancePool + d.conditional + static_cast<std::size_t>(row) * d.width,
+                              static_cast<int>(d.width), colTarget,
+                              d.colGuide ? guides + d.colGuide + static_cast<std::size_t>(row) * ((1U << d.colShift) + 1U) : nullptr,
+                              d.colShift);
     const float v0 = static_cast<float>(row) / static_cast<float>(d.height);
     const float v1 = static_cast<float>(row + 1) / static_cast<float>(d.height);
     const float c0 = cosf(kPi * (1.0f - v0)), c1 = cosf(kPi * (1.0f - v1));

@@ -29,16 +29,18 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
                              std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
-                            int last, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+                            int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,
+                            hipStream_t st);
 hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                              unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
-hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
-                             hipStream_t st);
+hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, const CacheParams& cp, std::uint32_t maxWork,
+                             unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchFilmBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
                            hipStream_t st);
-hipError_t LaunchAdvanceWave(const BatchWaves& bw, hipStream_t st);
+// Next wave; also zeroes *clear when given (the wave's record-slot count).
+hipError_t LaunchAdvanceWave(const BatchWaves& bw, std::uint32_t* clear, hipStream_t st);
 hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st);
 // Counts the answered cache queries of a bounce and empties the queue.
 hipError_t LaunchConsumeQueries(const CacheParams& cp, hipStream_t st);

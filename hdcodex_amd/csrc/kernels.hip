@@ -39,6 +39,21 @@ __device__ __forceinline__ void waveAppend(bool keep, std::uint32_t item, std::u
     if (keep) queue[base + before] = item;
 }
 
+// Index of this lane's entry in a compacted array (one atomic per wave), or
+// 0xffffffff when it does not want one.
+__device__ __forceinline__ std::uint32_t waveSlot(bool want, std::uint32_t* counter)
+{
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ULL) return 0xffffffffU;
+    const unsigned long long active = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(active)) - 1;
+    const std::uint32_t lane = laneId();
+    std::uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(counter, static_cast<std::uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    return want ? base + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL))) : 0xffffffffU;
+}
+
 // ---------------------------------------------------------------------------
 // raygen: seed, hero wavelength, jittered pinhole ray (:1198-1224).
 // ---------------------------------------------------------------------------
@@ -64,7 +79,10 @@ __device__ __forceinline__ void startPath(const PathBuffers& pb, const CachePara
     }
     pb.lambda0[pid] = wl0;
     if (cp.enabled) pb.spread[pid] = make_float2(0.0f, 0.0f);
-    if (cp.enabled && (ps + cp.trainOffset) % cp.trainStride == 0U) {
+    // A single render clears the training slot this path will write. (In a
+    // batch, slots are handed out per wave by raygenBatchKernel and the
+    // gather of the previous wave left every handed-out record empty.)
+    if (cp.enabled && !cp.trainSlot && (ps + cp.trainOffset) % cp.trainStride == 0U) {
         const std::uint32_t slot = (ps + cp.trainOffset) / cp.trainStride;
         if (slot < cp.recordSlots)
             for (std::uint32_t j = 0; j < cp.recordVertices; ++j)
@@ -122,38 +140,39 @@ __device__ __forceinline__ std::uint32_t findTile(const BatchTile* tiles, std::u
 
 // Batch raygen: the lane paths of the current wave (read from the device
 // cursor, so that the same launch serves every wave of a captured graph).
-__global__ __launch_bounds__(kBlock) void raygenBatchKernel(PathBuffers pb, BatchWaves bw)
+// With the radiance cache on, a pixel sample trains when its index within
+// the scene's frame (sample-major) plus the scene's offset is a multiple of
+// the scene's stride; each of its lane paths takes a record slot.
+__global__ __launch_bounds__(kBlock) void raygenBatchKernel(PathBuffers pb, BatchWaves bw, CacheParams cp)
 {
     const std::uint32_t w = *bw.cursor;
     const std::uint32_t ta = bw.waveTiles[w], tb = bw.waveTiles[w + 1];
     const std::uint32_t paths = bw.wavePaths[w];
-    const CacheParams none{};
-    for (std::uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x; pid < paths; pid += gridDim.x * blockDim.x) {
-        const BatchTile& t = bw.tiles[findTile(bw.tiles, ta, tb, pid, [](const BatchTile& x) { return x.pathOffset; })];
-        const BatchView& v = bw.views[t.scene];
-        const std::uint32_t local = (pid - t.pathOffset) / 3U, lane = pid - t.pathOffset - local * 3U;
-        const std::uint32_t tilePixels = t.p1 - t.p0;
-        const std::uint32_t so = local / tilePixels;
-        const std::uint32_t pixel = t.p0 + (local - so * tilePixels);
-        pb.pathScene[pid] = t.scene;
-        startPath(pb, none, pid, local, lane, pixel, v.sampleBase + so, v.width, v.height, v.cam);
+    const std::uint32_t stride = gridDim.x * blockDim.x;
+    const std::uint32_t padded = (paths + stride - 1U) / stride * stride;  // every lane reaches the ballot
+    for (std::uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x; pid < padded; pid += stride) {
+        bool train = false;
+        if (pid < paths) {
+            const BatchTile& t = bw.tiles[findTile(bw.tiles, ta, tb, pid, [](const BatchTile& x) { return x.pathOffset; })];
+            const BatchView& v = bw.views[t.scene];
+            const std::uint32_t local = (pid - t.pathOffset) / 3U, lane = pid - t.pathOffset - local * 3U;
+            const std::uint32_t tilePixels = t.p1 - t.p0;
+            const std::uint32_t so = local / tilePixels;
+            const std::uint32_t pixel = t.p0 + (local - so * tilePixels);
+            pb.pathScene[pid] = t.scene;
+            startPath(pb, cp, pid, local, lane, pixel, v.sampleBase + so, v.width, v.height, v.cam);
+            if (cp.enabled) {
+                const uint2 tr = cp.sceneTrain[t.scene];
+                const std::uint32_t index = so * (v.width * v.height) + pixel;
+                train = tr.x != 0U && (index + tr.y) % tr.x == 0U;
+            }
+        }
+        if (cp.enabled) {
+            const std::uint32_t slot = waveSlot(train, cp.recordCount);
+            if (pid < paths) cp.trainSlot[pid] = train && slot < cp.recordSlots ? slot : 0xffffffffU;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) pb.counters[0] = paths;
-}
-
-// Index of this lane's entry in a compacted array (one atomic per wave), or
-// 0xffffffff when it does not want one.
-__device__ __forceinline__ std::uint32_t waveSlot(bool want, std::uint32_t* counter)
-{
-    const unsigned long long mask = __ballot(want);
-    if (mask == 0ULL) return 0xffffffffU;
-    const unsigned long long active = __ballot(1);
-    const int leader = __ffsll(static_cast<long long>(active)) - 1;
-    const std::uint32_t lane = laneId();
-    std::uint32_t base = 0;
-    if (static_cast<int>(lane) == leader) base = atomicAdd(counter, static_cast<std::uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader);
-    return want ? base + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL))) : 0xffffffffU;
 }
 
 template <bool k2L>
@@ -541,9 +560,9 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                     if (cp.enabled) {
                         const std::uint32_t ps = pid / 3U, lane = pid - ps * 3U;
                         RcQuery rq;
-                        rq.px = (hp.x - cp.lo[0]) * cp.invExtent[0];
-                        rq.py = (hp.y - cp.lo[1]) * cp.invExtent[1];
-                        rq.pz = (hp.z - cp.lo[2]) * cp.invExtent[2];
+                        rq.px = (hp.x - s.boundsLo[0]) * s.boundsInvExtent[0];
+                        rq.py = (hp.y - s.boundsLo[1]) * s.boundsInvExtent[1];
+                        rq.pz = (hp.z - s.boundsLo[2]) * s.boundsInvExtent[2];
                         rq.dx = c.view.x;
                         rq.dy = c.view.y;
                         rq.dz = c.view.z;
@@ -553,10 +572,20 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                         rq.lane = lane;
                         rq.path = pid;
                         rq.target = radiance;
-                        if ((ps + cp.trainOffset) % cp.trainStride == 0U) {
+                        bool training;
+                        std::uint32_t recordBase = 0xffffffffU;
+                        if (cp.trainSlot) {
+                            const std::uint32_t ts = cp.trainSlot[pid];
+                            training = ts != 0xffffffffU;
+                            if (training) recordBase = ts * cp.recordVertices;
+                        } else {
+                            training = (ps + cp.trainOffset) % cp.trainStride == 0U;
                             const std::uint32_t tslot = (ps + cp.trainOffset) / cp.trainStride;
-                            if (tslot < cp.recordSlots && static_cast<std::uint32_t>(bounce) < cp.recordVertices) {
-                                const std::uint32_t r = (tslot * 3U + lane) * cp.recordVertices + static_cast<std::uint32_t>(bounce);
+                            if (training && tslot < cp.recordSlots) recordBase = (tslot * 3U + lane) * cp.recordVertices;
+                        }
+                        if (training) {
+                            if (recordBase != 0xffffffffU && static_cast<std::uint32_t>(bounce) < cp.recordVertices) {
+                                const std::uint32_t r = recordBase + static_cast<std::uint32_t>(bounce);
                                 cp.records[r] = rq;
                                 cp.recordThroughput[r] = throughput;
                             }
@@ -571,8 +600,12 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                             }
                             spreadOut = sp;
                             spreadSet = true;
-                            if (bounce >= static_cast<int>(cp.terminateBounce) &&
-                                (cp.spreadThreshold <= 0.0f || sp.y * sp.y > cp.spreadThreshold * sp.x)) {
+                            // Without the spread rule a path stops exactly at terminateBounce (a path
+                            // that scattered in a medium there goes on as a plain path); inference runs
+                            // after that bounce only, or after every later one with the rule.
+                            if (cp.spreadThreshold <= 0.0f ? bounce == static_cast<int>(cp.terminateBounce)
+                                                           : (bounce >= static_cast<int>(cp.terminateBounce) &&
+                                                              sp.y * sp.y > cp.spreadThreshold * sp.x)) {
                                 cacheStop = true;
                                 wantQuery = true;
                                 query = rq;
@@ -1036,6 +1069,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
 // scattered at j toward the previous vertex).
 __global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32_t records)
 {
+    if (cp.recordCount) records = min(*cp.recordCount, cp.recordSlots) * cp.recordVertices;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < records; i += gridDim.x * blockDim.x) {
         RcQuery r = cp.records[i];
         if (r.lane > 2U) continue;
@@ -1044,10 +1078,14 @@ __global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32
         if (!(t > 0.0f)) continue;
         const float target = (pb.radiance[r.path] - r.target) / t;
         if (!isfinite(target)) continue;
-        const std::uint32_t k = atomicAdd(cp.sampleCount, 1U);
-        if (k < cp.sampleCapacity) {
-            r.target = fmaxf(target, 0.0f);
-            cp.samples[k] = r;
+        r.target = fmaxf(target, 0.0f);
+        if (cp.sceneSampleCount) {
+            const std::uint32_t scene = pb.pathScene[r.path];
+            const std::uint32_t k = atomicAdd(cp.sceneSampleCount + scene, 1U);
+            if (k < cp.sampleCapacity) cp.samples[static_cast<std::size_t>(scene) * cp.sampleCapacity + k] = r;
+        } else {
+            const std::uint32_t k = atomicAdd(cp.sampleCount, 1U);
+            if (k < cp.sampleCapacity) cp.samples[k] = r;
         }
     }
 }
@@ -1129,7 +1167,11 @@ __global__ void filmBatchKernel(PathBuffers pb, BatchWaves bw)
 }
 
 // Moves the wave cursor on (last node of a wave's graph).
-__global__ void advanceWaveKernel(BatchWaves bw) { *bw.cursor += 1U; }
+__global__ void advanceWaveKernel(BatchWaves bw, std::uint32_t* clear)
+{
+    *bw.cursor += 1U;
+    if (clear) *clear = 0U;
+}
 
 // Progressive mean (:1886-1890) in FP64; alpha = 1.
 __global__ void resolveKernel(double4* mean, const double4* batch, float4* out, std::uint32_t pixels,
@@ -1289,11 +1331,12 @@ hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const Path
                     : LaunchExtendT<SceneTable, false>(t, pb, in, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
-                            int last, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+                            int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,
+                            hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchShadeT<SceneTable, true>(t, pb, in, bounce, last, CacheParams{}, walk, maxWork, maxBlocks, st)
-                    : LaunchShadeT<SceneTable, false>(t, pb, in, bounce, last, CacheParams{}, walk, maxWork, maxBlocks, st);
+    return twoLevel ? LaunchShadeT<SceneTable, true>(t, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st)
+                    : LaunchShadeT<SceneTable, false>(t, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st);
 }
 hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st)
@@ -1309,10 +1352,10 @@ hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const Path
     return twoLevel ? LaunchShadowT<SceneTable, true>(t, pb, maxWork, maxBlocks, ctr, st)
                     : LaunchShadowT<SceneTable, false>(t, pb, maxWork, maxBlocks, ctr, st);
 }
-hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
-                             hipStream_t st)
+hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, const CacheParams& cp, std::uint32_t maxWork,
+                             unsigned maxBlocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(raygenBatchKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, pb, bw);
+    hipLaunchKernelGGL(raygenBatchKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, pb, bw, cp);
     return hipGetLastError();
 }
 hipError_t LaunchFilmBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,
@@ -1321,9 +1364,9 @@ hipError_t LaunchFilmBatch(const PathBuffers& pb, const BatchWaves& bw, std::uin
     hipLaunchKernelGGL(filmBatchKernel, dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, pb, bw);
     return hipGetLastError();
 }
-hipError_t LaunchAdvanceWave(const BatchWaves& bw, hipStream_t st)
+hipError_t LaunchAdvanceWave(const BatchWaves& bw, std::uint32_t* clear, hipStream_t st)
 {
-    hipLaunchKernelGGL(advanceWaveKernel, dim3(1), dim3(1), 0, st, bw);
+    hipLaunchKernelGGL(advanceWaveKernel, dim3(1), dim3(1), 0, st, bw, clear);
     return hipGetLastError();
 }
 hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st)

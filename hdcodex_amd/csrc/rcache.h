@@ -62,6 +62,29 @@ struct RcState {
 hipError_t LaunchRcInfer(const RcNetwork& net, const RcQuery* queries, const std::uint32_t* count,
                          std::uint32_t maxCount, float* out, float* radiance, const float* throughput,
                          unsigned maxBlocks, hipStream_t st);
+// Buckets of cache queries by scene (batch renderer), device arrays:
+// count [scenes] (zero before the first use; the scan clears it), start and
+// blockStart [scenes + 1], cursor [scenes].
+struct RcSceneBuckets {
+    std::uint32_t* count;
+    std::uint32_t* start;
+    std::uint32_t* blockStart;
+    std::uint32_t* cursor;
+    std::uint32_t scenes;
+};
+
+// Inference with one network per scene: weights of scene s at
+// weights + s * kRcParams (BF16); the scene of a query is pathScene[path].
+// Queries are sorted by scene into `sorted` (maxCount entries), then
+// radiance[path] += throughput[path] * max(out[lane], 0) as LaunchRcInfer.
+hipError_t LaunchRcInferScenes(const __bf16* weights, const RcQuery* queries, const std::uint32_t* count,
+                               std::uint32_t maxCount, const std::uint32_t* pathScene, const RcSceneBuckets& b,
+                               RcQuery* sorted, float* radiance, const float* throughput, unsigned maxBlocks,
+                               hipStream_t st);
+// He-uniform initial weights for `seed` (bounds sqrt(6 / fan_in), output
+// layer scaled by 0.5), FP32, kRcParams values.
+void InitRcWeights(std::uint64_t seed, float* w);
+
 // Forward + backward over `count` samples. Block b adds its share of
 // d(mean loss)/dW into partial row b (rows are kRcParams floats; at most
 // maxBlocks rows are used); loss[0] += summed loss, loss[1] += samples.

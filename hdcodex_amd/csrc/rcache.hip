@@ -12,6 +12,8 @@
 // never leave registers between layers. swap23 exchanges bits 2 and 3.
 #include "rcache.h"
 
+#include <cmath>
+
 namespace hdcx {
 namespace {
 
@@ -151,6 +153,43 @@ __device__ f32x16 forward(const __bf16* w, const uint4* ldsTiles, const bf16x8 x
     return y;
 }
 
+// The block's copy of a network's MFMA-ready weight tiles (52 KB of LDS).
+__device__ void stageTiles(uint4* tiles, const __bf16* w)
+{
+    for (int c = threadIdx.x; c < kWeightTiles * 64; c += blockDim.x) {
+        const int b = c >> 6, l = c & 63;
+        int layer, t, s;
+        if (b < 16) { layer = 0; t = b >> 3; s = b & 7; }
+        else if (b < 48) { layer = 1 + ((b - 16) >> 3); t = ((b - 16) & 7) >> 2; s = (b - 16) & 3; }
+        else { layer = 5; t = 0; s = b - 48; }
+        tiles[c] = __builtin_bit_cast(uint4, weightTile(w, layer, t, s, l));
+    }
+}
+
+// One wave: queries [first, first + 32) below end.
+__device__ __forceinline__ void inferTile(const __bf16* w, const uint4* tiles, const RcQuery* queries, std::uint32_t first,
+                                          std::uint32_t end, int lane, float* out, float* radiance, const float* throughput)
+{
+    const std::uint32_t qi = first + static_cast<std::uint32_t>(lane & 31);
+    const bool valid = qi < end;
+    RcQuery q{};
+    if (valid) q = queries[qi];
+    bf16x8 x[8];
+    encode(q, lane >> 5, x);
+    const f32x16 y = forward<false>(w, tiles, x, lane, nullptr);
+    if (valid && lane < 32) {
+        if (out) {
+            out[qi * 3 + 0] = y[0];
+            out[qi * 3 + 1] = y[1];
+            out[qi * 3 + 2] = y[2];
+        }
+        if (radiance) {
+            const float v = q.lane == 0 ? y[0] : (q.lane == 1 ? y[1] : y[2]);
+            radiance[q.path] += throughput[q.path] * fmaxf(v, 0.0f);
+        }
+    }
+}
+
 __global__ __launch_bounds__(kInferBlock) void rcInferKernel(RcNetwork net, const RcQuery* queries,
                                                             const std::uint32_t* countPtr, std::uint32_t maxCount,
                                                             float* out, float* radiance, const float* throughput)
@@ -159,38 +198,83 @@ __global__ __launch_bounds__(kInferBlock) void rcInferKernel(RcNetwork net, cons
     const std::uint32_t count = countPtr ? min(*countPtr, maxCount) : maxCount;
     // Blocks without a tile of queries leave before staging the weights.
     if (blockIdx.x * (blockDim.x >> 6) >= (count + 31U) / 32U) return;
-    for (int c = threadIdx.x; c < kWeightTiles * 64; c += blockDim.x) {
-        const int b = c >> 6, l = c & 63;
-        int layer, t, s;
-        if (b < 16) { layer = 0; t = b >> 3; s = b & 7; }
-        else if (b < 48) { layer = 1 + ((b - 16) >> 3); t = ((b - 16) & 7) >> 2; s = (b - 16) & 3; }
-        else { layer = 5; t = 0; s = b - 48; }
-        tiles[c] = __builtin_bit_cast(uint4, weightTile(net.w, layer, t, s, l));
-    }
+    stageTiles(tiles, net.w);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const std::uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const std::uint32_t tileCount = (count + 31U) / 32U;
-    for (std::uint32_t tile = wave; tile < tileCount; tile += waves) {
-        const std::uint32_t qi = tile * 32U + static_cast<std::uint32_t>(lane & 31);
-        const bool valid = qi < count;
-        RcQuery q{};
-        if (valid) q = queries[qi];
-        bf16x8 x[8];
-        encode(q, lane >> 5, x);
-        const f32x16 y = forward<false>(net.w, tiles, x, lane, nullptr);
-        if (valid && lane < 32) {
-            if (out) {
-                out[qi * 3 + 0] = y[0];
-                out[qi * 3 + 1] = y[1];
-                out[qi * 3 + 2] = y[2];
-            }
-            if (radiance) {
-                const float v = q.lane == 0 ? y[0] : (q.lane == 1 ? y[1] : y[2]);
-                radiance[q.path] += throughput[q.path] * fmaxf(v, 0.0f);
-            }
+    for (std::uint32_t tile = wave; tile < tileCount; tile += waves)
+        inferTile(net.w, tiles, queries, tile * 32U, count, lane, out, radiance, throughput);
+}
+
+// ---- One network per scene (batch renderer) ----
+// Queries are bucketed by the scene of their path (histogram, scan,
+// scatter), then each inference block takes kSceneBlockQueries queries of
+// one scene and stages that scene's weights.
+constexpr std::uint32_t kSceneBlockQueries = 32U * (kInferBlock / 64);
+
+__global__ void rcSceneHistogramKernel(const RcQuery* queries, const std::uint32_t* countPtr, std::uint32_t maxCount,
+                                       const std::uint32_t* pathScene, RcSceneBuckets b)
+{
+    const std::uint32_t count = min(*countPtr, maxCount);
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        atomicAdd(&b.count[pathScene[queries[i].path]], 1U);
+}
+
+// One thread: bucket starts, inference-block starts, scatter cursors; the
+// histogram is cleared for the next use.
+__global__ void rcSceneScanKernel(RcSceneBuckets b)
+{
+    std::uint32_t q = 0, blocks = 0;
+    for (std::uint32_t s = 0; s < b.scenes; ++s) {
+        const std::uint32_t n = b.count[s];
+        b.start[s] = q;
+        b.cursor[s] = q;
+        b.blockStart[s] = blocks;
+        q += n;
+        blocks += (n + kSceneBlockQueries - 1U) / kSceneBlockQueries;
+        b.count[s] = 0U;
+    }
+    b.start[b.scenes] = q;
+    b.blockStart[b.scenes] = blocks;
+}
+
+__global__ void rcSceneScatterKernel(const RcQuery* queries, const std::uint32_t* countPtr, std::uint32_t maxCount,
+                                     const std::uint32_t* pathScene, RcSceneBuckets b, RcQuery* sorted)
+{
+    const std::uint32_t count = min(*countPtr, maxCount);
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const RcQuery q = queries[i];
+        sorted[atomicAdd(&b.cursor[pathScene[q.path]], 1U)] = q;
+    }
+}
+
+__global__ __launch_bounds__(kInferBlock) void rcInferScenesKernel(const __bf16* weights, const RcQuery* sorted,
+                                                                  RcSceneBuckets b, float* radiance,
+                                                                  const float* throughput)
+{
+    __shared__ uint4 tiles[kWeightTiles * 64];
+    const std::uint32_t total = b.blockStart[b.scenes];
+    const int lane = threadIdx.x & 63;
+    const std::uint32_t waveInBlock = threadIdx.x >> 6;
+    std::uint32_t staged = 0xffffffffU;
+    for (std::uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
+        // Scene of this block of queries: the last s with blockStart[s] <= vb.
+        std::uint32_t lo = 0, hi = b.scenes - 1U;
+        while (lo < hi) {
+            const std::uint32_t mid = (lo + hi + 1U) >> 1;
+            if (b.blockStart[mid] <= vb) lo = mid; else hi = mid - 1U;
         }
+        const __bf16* w = weights + static_cast<std::size_t>(lo) * kRcParams;
+        if (lo != staged) {
+            __syncthreads();
+            stageTiles(tiles, w);
+            __syncthreads();
+            staged = lo;
+        }
+        const std::uint32_t first = b.start[lo] + (vb - b.blockStart[lo]) * kSceneBlockQueries + 32U * waveInBlock;
+        if (first < b.start[lo + 1U]) inferTile(w, tiles, sorted, first, b.start[lo + 1U], lane, nullptr, radiance, throughput);
     }
 }
 
@@ -401,6 +485,22 @@ __global__ void rcPublishKernel(RcState s)
 
 } // namespace
 
+void InitRcWeights(std::uint64_t seed, float* w)
+{
+    std::uint64_t state = seed ^ 0x5deece66dULL;
+    auto next = [&]() {
+        std::uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+        return static_cast<double>((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int i = 0; i < kRcParams; ++i) {
+        const int fanIn = i < kRcW1 ? kRcIn : kRcHidden;
+        const double bound = std::sqrt(6.0 / fanIn) * (i >= kRcW5 ? 0.5 : 1.0);
+        w[i] = static_cast<float>((2.0 * next() - 1.0) * bound);
+    }
+}
+
 hipError_t LaunchRcInfer(const RcNetwork& net, const RcQuery* queries, const std::uint32_t* count,
                          std::uint32_t maxCount, float* out, float* radiance, const float* throughput,
                          unsigned maxBlocks, hipStream_t st)
@@ -410,6 +510,23 @@ hipError_t LaunchRcInfer(const RcNetwork& net, const RcQuery* queries, const std
     blocks = blocks == 0 ? 1U : (blocks < maxBlocks ? blocks : maxBlocks);
     hipLaunchKernelGGL(rcInferKernel, dim3(blocks), dim3(kInferBlock), 0, st, net, queries, count, maxCount, out,
                        radiance, throughput);
+    return hipGetLastError();
+}
+
+hipError_t LaunchRcInferScenes(const __bf16* weights, const RcQuery* queries, const std::uint32_t* count,
+                               std::uint32_t maxCount, const std::uint32_t* pathScene, const RcSceneBuckets& b,
+                               RcQuery* sorted, float* radiance, const float* throughput, unsigned maxBlocks,
+                               hipStream_t st)
+{
+    const unsigned spread = (maxCount + 255U) / 256U;
+    const unsigned grid = spread == 0 ? 1U : (spread < maxBlocks ? spread : maxBlocks);
+    hipLaunchKernelGGL(rcSceneHistogramKernel, dim3(grid), dim3(256), 0, st, queries, count, maxCount, pathScene, b);
+    hipLaunchKernelGGL(rcSceneScanKernel, dim3(1), dim3(1), 0, st, b);
+    hipLaunchKernelGGL(rcSceneScatterKernel, dim3(grid), dim3(256), 0, st, queries, count, maxCount, pathScene, b, sorted);
+    const unsigned blocks = (maxCount + kSceneBlockQueries - 1U) / kSceneBlockQueries + b.scenes;
+    const unsigned inferGrid = blocks < maxBlocks ? blocks : maxBlocks;
+    hipLaunchKernelGGL(rcInferScenesKernel, dim3(inferGrid == 0 ? 1U : inferGrid), dim3(kInferBlock), 0, st, weights, sorted,
+                       b, radiance, throughput);
     return hipGetLastError();
 }
 

@@ -263,6 +263,16 @@ void BuildTwoLevel(SceneStore* t, const hdcx_scene_view& v, hipStream_t st)
         }
     }
     if (drawn > 0) items.push_back(blas.back().box);
+    // World box of the scene: the instance boxes and the drawn triangles.
+    for (int k = 0; k < 3; ++k) {
+        float lo = items[0].lo[k], hi = items[0].hi[k];
+        for (const lbvh::Box4& b : items) {
+            lo = std::min(lo, b.lo[k]);
+            hi = std::max(hi, b.hi[k]);
+        }
+        t->sceneLo[k] = lo;
+        t->sceneInvExtent[k] = hi > lo ? 1.0F / (hi - lo) : 1.0F;
+    }
     const Bvh4 tlas = BuildBvh4OverBoxes(items);
     // Layout: instance tree, then each BLAS; BLAS references relocated.
     std::uint32_t nodeCount = static_cast<std::uint32_t>(tlas.nodes.size()), slotCount = 0, blasStack = 0, blasDepth = 0;
@@ -404,8 +414,29 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     // Dome importance tables re-laid out per light: marginal CDF column,
     // conditional CDF and pdf planes of the packed importance texture.
     const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
-    std::vector<ImportanceDesc> imp(lightRecords, ImportanceDesc{0, 0, 0, 0, 0});
+    std::vector<ImportanceDesc> imp(lightRecords, ImportanceDesc{});
     std::vector<float> impPool;
+    // Guide table of a CDF (2^shift + 1 uint32 entries appended to the pool),
+    // or 0 when the CDF is not monotone (then the full binary search runs).
+    auto appendGuide = [&impPool](const float* cdf, std::uint32_t n, std::uint32_t shift) -> bool {
+        for (std::uint32_t i = 1; i < n; ++i)
+            if (!(cdf[i] >= cdf[i - 1])) return false;
+        const std::uint32_t buckets = 1U << shift;
+        std::uint32_t i = 0;
+        for (std::uint32_t g = 0; g <= buckets; ++g) {
+            const float threshold = static_cast<float>(static_cast<double>(g) / buckets);  // exact
+            while (i + 1U < n && cdf[i] < threshold) ++i;
+            float word;
+            std::memcpy(&word, &i, 4);
+            impPool.push_back(word);
+        }
+        return true;
+    };
+    auto shiftFor = [](std::uint32_t n) {
+        std::uint32_t k = 0;
+        while ((1U << k) < n && k < 20U) ++k;
+        return k;
+    };
     for (std::uint32_t l = 0; l < v.light_count; ++l) {
         const std::uint32_t ti = ls[l].textureInfo.w;
         if (ls[l].textureInfo.z != 0U || ti == kMissingTexture) continue;
@@ -420,6 +451,27 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
         for (std::size_t p = 0; p < static_cast<std::size_t>(tv.width) * tv.height; ++p) impPool.push_back(px[p * 4 + 0]);
         d.pdf = impPool.size();
         for (std::size_t p = 0; p < static_cast<std::size_t>(tv.width) * tv.height; ++p) impPool.push_back(px[p * 4 + 2]);
+        d.rowShift = shiftFor(tv.height);
+        d.colShift = shiftFor(tv.width);
+        {
+            const std::vector<float> marginal(impPool.begin() + static_cast<std::ptrdiff_t>(d.marginal),
+                                              impPool.begin() + static_cast<std::ptrdiff_t>(d.marginal + tv.height));
+            const std::size_t at = impPool.size();
+            d.rowGuide = appendGuide(marginal.data(), tv.height, d.rowShift) ? at : 0;
+            if (!d.rowGuide) impPool.resize(at);
+        }
+        {
+            const std::size_t at = impPool.size();
+            bool ok = true;
+            std::vector<float> row(tv.width);
+            for (std::uint32_t y = 0; y < tv.height && ok; ++y) {
+                std::copy_n(impPool.begin() + static_cast<std::ptrdiff_t>(d.conditional + static_cast<std::size_t>(y) * tv.width),
+                            tv.width, row.begin());
+                ok = appendGuide(row.data(), tv.width, d.colShift);
+            }
+            d.colGuide = ok ? at : 0;
+            if (!ok) impPool.resize(at);
+        }
         imp[l] = d;
     }
     if (impPool.empty()) impPool.push_back(0.0F);
@@ -446,6 +498,10 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     s.lightCount = v.light_count;
     s.opaque = v.opaque ? 1U : 0U;
     s.hasEmissive = drawn > 0 && v.emissive[(static_cast<std::size_t>(drawn) - 1) * 4] > 0.0F ? 1U : 0U;
+    for (int k = 0; k < 3; ++k) {
+        s.boundsLo[k] = t->sceneLo[k];
+        s.boundsInvExtent[k] = t->sceneInvExtent[k];
+    }
     s.hasSubsurface = 0U;
     const auto* mats = static_cast<const GpuMaterialRecord*>(v.materials);
     for (std::uint32_t m = 0; m < v.material_count; ++m)

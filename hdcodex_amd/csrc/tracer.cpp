@@ -11,6 +11,7 @@
 #include "lbvh_host.hpp"
 #include "packer.hpp"
 #include "procedural.hpp"
+#include "rc_config.h"
 #include "rcache.h"
 
 #include <hip/hip_runtime.h>
@@ -160,18 +161,7 @@ struct hdcx_tracer : SceneStore, PathPool {
         rcWte.Allocate(4 * 4096 * sizeof(std::uint16_t));
         rcLoss.Allocate(2 * sizeof(float));
         std::vector<float> w(kRcParams);
-        std::uint64_t state = seed ^ 0x5deece66dULL;
-        auto next = [&]() {
-            std::uint64_t z = (state += 0x9e3779b97f4a7c15ULL);
-            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-            z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-            return static_cast<double>((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
-        };
-        for (int i = 0; i < kRcParams; ++i) {
-            const int fanIn = i < kRcW1 ? kRcIn : kRcHidden;
-            const double bound = std::sqrt(6.0 / fanIn) * (i >= kRcW5 ? 0.5 : 1.0);
-            w[i] = static_cast<float>((2.0 * next() - 1.0) * bound);
-        }
+        InitRcWeights(seed, w.data());
         Check(hipMemcpyAsync(rcMaster.as<void>(), w.data(), kRcParams * sizeof(float), hipMemcpyHostToDevice, stream), "rc init");
         Check(hipMemsetAsync(rcM.as<void>(), 0, rcM.bytes(), stream), "rc init");
         Check(hipMemsetAsync(rcV.as<void>(), 0, rcV.bytes(), stream), "rc init");
@@ -282,10 +272,6 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
         cp.queryCapacity = t->capacity;
         cp.recordSlots = pixelsPerChunk * samplesPerChunk / cp.trainStride + 2;
         cp.sampleCapacity = t->rcConfig.train_samples;
-        for (int k = 0; k < 3; ++k) {
-            cp.lo[k] = t->sceneLo[k];
-            cp.invExtent[k] = t->sceneInvExtent[k];
-        }
         t->rcChunkQueries.Allocate(std::max<std::size_t>(t->rcChunkQueries.bytes(), static_cast<std::size_t>(t->capacity) * sizeof(RcQuery)));
         t->rcCounters.Allocate(64);
         const std::size_t records = static_cast<std::size_t>(cp.recordSlots) * 3 * kRecordVertices;
@@ -692,15 +678,7 @@ int hdcx_set_radiance_cache(hdcx_tracer* tracer, const hdcx_radiance_cache_confi
             tracer->rcEnabled = false;
             return;
         }
-        hdcx_radiance_cache_config c = *config;
-        if (c.train_samples == 0) c.train_samples = 65536;
-        if (c.train_steps == 0) c.train_steps = 4;
-        if (!(c.learning_rate > 0.0F)) c.learning_rate = 1e-2F;
-        if (!(c.beta1 > 0.0F)) c.beta1 = 0.9F;
-        if (!(c.beta2 > 0.0F)) c.beta2 = 0.99F;
-        if (!(c.epsilon > 0.0F)) c.epsilon = 1e-8F;
-        if (c.terminate_bounce >= HDCX_MAX_BOUNCES) throw std::invalid_argument("terminate_bounce out of range");
-        if (!(c.ema_decay >= 0.0F && c.ema_decay < 1.0F)) throw std::invalid_argument("ema_decay outside [0, 1)");
+        const hdcx_radiance_cache_config c = NormalizedRcConfig(*config);
         tracer->rcConfig = c;
         tracer->EnsureRc(c.seed);
         tracer->rcEnabled = true;

@@ -387,6 +387,23 @@ int hdcx_rc_train(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t co
                   float* out_loss);
 int hdcx_rc_get_stats(hdcx_tracer* tracer, hdcx_rc_stats* out_stats);
 
+/* Radiance caches of a batch: one network per scene, as the tracer's (no
+ * counterpart in the reference). config->train_samples counts samples per
+ * scene and render; scene i's weights start from seed + i. The networks are
+ * (re)initialised when scenes were added or the seed changed; each
+ * hdcx_batch_render trains them, after its waves, on the samples it
+ * gathered. NULL disables. */
+int hdcx_batch_set_radiance_cache(hdcx_batch* batch, const hdcx_radiance_cache_config* config);
+typedef struct hdcx_batch_rc_stats {
+    uint64_t queries;        /* cache queries answered by the last render */
+    uint64_t train_samples;  /* samples trained on, all renders */
+    uint64_t adam_steps;     /* all renders, all scenes */
+    double train_ms;         /* wall time of training, all renders */
+    float mean_loss;         /* mean relative L2 loss of the last render's steps */
+    uint32_t scenes_trained; /* scenes with samples in the last render */
+} hdcx_batch_rc_stats;
+int hdcx_batch_get_rc_stats(hdcx_batch* batch, hdcx_batch_rc_stats* out_stats);
+
 /* ---- Measurement (no counterpart in the reference, which has no GPU
  * timing: docs/architecture.md:255-257). ---- */
 
