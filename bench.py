@@ -449,6 +449,8 @@ def main() -> int:
     ap.add_argument("--instanced-only", action="store_true", help="only the 10,000-instance workload")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch workload with plain launches instead of graph replay, no autotuning")
+    ap.add_argument("--no-batch-cache", action="store_true",
+                    help="batch workload without the per-scene radiance caches at equal time")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy kernel 4 times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -458,7 +460,7 @@ def main() -> int:
         import hdcodex_amd as hx
 
         print(json.dumps({"batch": batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph,
-                                                  cache=not args.no_graph)}),
+                                                  cache=not args.no_batch_cache)}),
               flush=True)
         return 0
     if args.instanced_only:
@@ -597,7 +599,7 @@ def main() -> int:
     if args.batch:
         log("batch workload")
         result["batch"] = batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph,
-                                         cache=not args.no_graph)
+                                         cache=not args.no_batch_cache)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         threads = min(os.cpu_count() or 1, 16)

@@ -51,7 +51,7 @@ struct RcState {
     __bf16* w;         // BF16 weights (training passes)
     __bf16* wt;        // BF16 transposes of W1..W4
     float* loss;       // [2]: summed relative L2 loss, sample count
-    float* ema;        // FP32 exponential moving average of the weights
+    float* ema;        // FP32 exponential moving average of the weights (from the published weights)
     __bf16* we;        // BF16 inference weights: the average (or the weights when emaDecay == 0)
     __bf16* wte;       // their transposes
     float emaDecay;    // per Adam step

@@ -349,8 +349,9 @@ typedef struct hdcx_radiance_cache_config {
     float beta2;               /* default 0.99 */
     float epsilon;             /* default 1e-8 */
     float ema_decay;           /* inference uses an exponential moving average of the
-                                * weights with this decay per Adam step, in [0, 1);
-                                * 0 = the current weights (default) */
+                                * weights with this decay per Adam step, in [0, 1),
+                                * started from the initial weights; 0 = the current
+                                * weights (default) */
     float spread_threshold;    /* a path stops into the cache at the first vertex from
                                 * terminate_bounce on where its spread exceeds this
                                 * times its primary footprint (Mueller et al. 2021);
