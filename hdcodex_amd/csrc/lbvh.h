@@ -42,7 +42,7 @@ public:
 
 private:
     DeviceBuffer morton_, mortonSorted_, ids_, idsSorted_, triBox_, child_, parent_, leafParent_, range_, arrivals_,
-        nodeBox_, source_, nodeStack_, counters_, frontier_, frontierBox_, temp_;
+        nodeBox_, nodeCost_, leafFlag_, source_, nodeStack_, counters_, frontier_, frontierBox_, temp_;
 };
 
 // Re-reads the slot vertices from `positions` (same triangles, moved

@@ -73,28 +73,31 @@ __global__ void radixTreeKernel(const std::uint32_t* __restrict__ keys, std::uin
 
 __global__ void bottomUpKernel(std::uint32_t n, const std::uint32_t* child2, const std::uint32_t* parent,
                                const std::uint32_t* leafParent, std::uint32_t* arrivals, const std::uint32_t* ids,
-                               const Box4* triBox, Box4* nodeBox)
+                               const Box4* triBox, const std::uint32_t* range2, Box4* nodeBox, float* nodeCost,
+                               std::uint32_t* leafFlag)
 {
     HDCX_GRID_STRIDE(k, 0U, n)
-    lbvh::BottomUpLeaf<DeviceAtomics>(k, child2, parent, leafParent, arrivals, ids, triBox, nodeBox);
+    lbvh::BottomUpLeaf<DeviceAtomics>(k, child2, parent, leafParent, arrivals, ids, triBox, range2, nodeBox, nodeCost,
+                                      leafFlag);
 }
 
-__global__ void frontierKernel(std::uint32_t n, std::uint32_t S, const std::uint32_t* child2,
+__global__ void frontierKernel(std::uint32_t n, std::uint32_t S, float maxArea, const std::uint32_t* child2,
                                const std::uint32_t* range2, const Box4* nodeBox, const Box4* triBox,
                                const std::uint32_t* ids, std::uint32_t* count, std::uint32_t capacity, FrontierItem* out)
 {
     HDCX_GRID_STRIDE(i, 0U, n - 1U)
-    lbvh::FrontierNode<DeviceAtomics>(i, S, child2, range2, nodeBox, triBox, ids, count, capacity, out);
+    lbvh::FrontierNode<DeviceAtomics>(i, S, maxArea, child2, range2, nodeBox, triBox, ids, count, capacity, out);
 }
 
 __global__ void collapseKernel(std::uint32_t begin, std::uint32_t end, std::uint32_t level, std::uint32_t* source,
                                const std::uint32_t* child2, const std::uint32_t* range2, const Box4* nodeBox,
-                               const Box4* triBox, const std::uint32_t* ids, std::uint32_t* nodeStack,
-                               std::uint32_t* counters, std::uint32_t capacity, Bvh4Node* nodes)
+                               const Box4* triBox, const std::uint32_t* ids, const std::uint32_t* leafFlag,
+                               std::uint32_t* nodeStack, std::uint32_t* counters, std::uint32_t capacity,
+                               Bvh4Node* nodes)
 {
     HDCX_GRID_STRIDE(id, begin, end)
-    lbvh::CollapseNode<DeviceAtomics>(id, level, source, child2, range2, nodeBox, triBox, ids, nodeStack, counters,
-                                      capacity, nodes);
+    lbvh::CollapseNode<DeviceAtomics>(id, level, source, child2, range2, nodeBox, triBox, ids, leafFlag, nodeStack,
+                                      counters, capacity, nodes);
 }
 
 __global__ void slotsKernel(std::uint32_t n, const std::uint32_t* __restrict__ ids, const float* __restrict__ pos,
@@ -150,6 +153,8 @@ void DeviceBvhBuilder::Build(const float4* positions, const std::uint32_t* indic
     range_.Reserve(sizeof(std::uint32_t) * 2 * inner);
     arrivals_.Reserve(sizeof(std::uint32_t) * n);
     nodeBox_.Reserve(sizeof(Box4) * inner);
+    nodeCost_.Reserve(sizeof(float) * n);
+    leafFlag_.Reserve(sizeof(std::uint32_t) * inner);
     source_.Reserve(sizeof(std::uint32_t) * nodeCap);
     nodeStack_.Reserve(sizeof(std::uint32_t) * nodeCap);
     counters_.Reserve(64);
@@ -169,7 +174,7 @@ void DeviceBvhBuilder::Build(const float4* positions, const std::uint32_t* indic
     auto* counters = counters_.as<std::uint32_t>();
 
     float scale[3];
-    for (int k = 0; k < 3; ++k) scale[k] = hi[k] > lo[k] ? 1024.0F / (hi[k] - lo[k]) : 0.0F;
+    lbvh::MortonScale(lo, hi, scale);
     const unsigned g = Grid(n);
     HDCX_LAUNCH(prepareKernel, g, pos, indices, n, lo[0], lo[1], lo[2], scale[0], scale[1], scale[2], keys, ids, triBox);
     std::size_t tempBytes = 0;
@@ -182,15 +187,18 @@ void DeviceBvhBuilder::Build(const float4* positions, const std::uint32_t* indic
     HDCX_LAUNCH(radixTreeKernel, g, sortedKeys, n, child2, parent_.as<std::uint32_t>(),
                 leafParent_.as<std::uint32_t>(), range2);
     Check(hipMemsetAsync(arrivals_.as<void>(), 0, sizeof(std::uint32_t) * n, st), "lbvh arrivals");
+    // Leaf flags of the SAH top nodes (ids >= n - 1) stay 0.
+    Check(hipMemsetAsync(leafFlag_.as<void>(), 0, sizeof(std::uint32_t) * inner, st), "lbvh leaf flags");
     HDCX_LAUNCH(bottomUpKernel, g, n, child2, parent_.as<std::uint32_t>(), leafParent_.as<std::uint32_t>(),
-                arrivals_.as<std::uint32_t>(), sortedIds, triBox, nodeBox);
+                arrivals_.as<std::uint32_t>(), sortedIds, triBox, range2, nodeBox, nodeCost_.as<float>(),
+                leafFlag_.as<std::uint32_t>());
 
     // SAH top levels over the frontier.
     std::uint32_t root = 0;
     if (S && n > S) {
         Check(hipMemsetAsync(counters, 0, 64, st), "lbvh counters");
-        HDCX_LAUNCH(frontierKernel, g, n, S, child2, range2, nodeBox, triBox, sortedIds, counters, frontierCap,
-                    frontier_.as<FrontierItem>());
+        HDCX_LAUNCH(frontierKernel, g, n, S, FrontierMaxArea(lo, hi, n, S), child2, range2, nodeBox, triBox,
+                    sortedIds, counters, frontierCap, frontier_.as<FrontierItem>());
         std::uint32_t count = 0;
         Check(hipMemcpyAsync(&count, counters, 4, hipMemcpyDeviceToHost, st), "lbvh frontier");
         Check(hipStreamSynchronize(st), "lbvh frontier");
@@ -231,8 +239,8 @@ void DeviceBvhBuilder::Build(const float4* positions, const std::uint32_t* indic
     std::uint32_t c[4] = {0, 0, 0, 0};
     while (begin < end) {
         HDCX_LAUNCH(collapseKernel, Grid(end - begin), begin, end, level, source_.as<std::uint32_t>(), child2, range2,
-                    nodeBox, triBox, sortedIds, nodeStack_.as<std::uint32_t>(), counters, nodeCap,
-                    nodes.as<Bvh4Node>());
+                    nodeBox, triBox, sortedIds, leafFlag_.as<std::uint32_t>(), nodeStack_.as<std::uint32_t>(),
+                    counters, nodeCap, nodes.as<Bvh4Node>());
         Check(hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, st), "lbvh level");
         Check(hipStreamSynchronize(st), "lbvh level");
         if (c[3] != 0U || c[0] > nodeCap) throw Failure(HDCX_ERR_INTERNAL, "device BVH collapse overflowed its node pool");

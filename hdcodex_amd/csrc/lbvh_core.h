@@ -30,7 +30,10 @@ namespace lbvh {
 // BVH2 child reference: an inner node id, or sorted position | kLeafRef.
 constexpr std::uint32_t kLeafRef = 0x80000000U;
 constexpr std::uint32_t kNone = 0xffffffffU;
-constexpr std::uint32_t kLeafTriangles = 4;       // subtrees this small become BVH4 leaves
+constexpr std::uint32_t kLeafTriangles = 4;       // largest subtree that may become a BVH4 leaf
+// Leaf decision (surface-area heuristic, areas relative to the parent's):
+// cost of a node visit and of one triangle test.
+constexpr float kCostNode = 1.0f, kCostTriangle = 1.0f;
 constexpr std::uint32_t kTopRangeHi = 0xfffffffeU;  // range.y of an SAH top node: never a leaf
 
 struct alignas(16) Box4 {
@@ -83,7 +86,20 @@ HDCX_HD inline float PadUp(float x) { return -PadDown(-x); }
 HDCX_HD inline float Min3(float a, float b, float c) { return fminf(fminf(a, b), c); }
 HDCX_HD inline float Max3(float a, float b, float c) { return fmaxf(fmaxf(a, b), c); }
 
-// Box and 30-bit Morton code of triangle i. scale = 1024 / extent per axis.
+// Quantisation of box centres for the Morton codes: one scale for all three
+// axes (1024 cells over the longest extent of the scene box), so that the
+// grid cells are cubes and a flat or elongated scene is split along its
+// long axes first. A scale per axis would give a thin axis as many code
+// bits as a long one: on a 1024^2 heightfield that raises the tree's SAH
+// cost by a fifth.
+inline void MortonScale(const float lo[3], const float hi[3], float scale[3])
+{
+    float extent = 0.0f;
+    for (int k = 0; k < 3; ++k) extent = fmaxf(extent, hi[k] - lo[k]);
+    for (int k = 0; k < 3; ++k) scale[k] = extent > 0.0f ? 1024.0f / extent : 0.0f;
+}
+
+// Box and 30-bit Morton code of triangle i (scale from MortonScale).
 HDCX_HD inline std::uint32_t PrepareTriangle(const float* pos, const std::uint32_t* idx, std::uint32_t i,
                                              const float lo[3], const float scale[3], Box4& box)
 {
@@ -171,13 +187,20 @@ HDCX_HD inline float HalfArea(const Box4& b)
 }
 
 // Walks up from leaf k; at each parent the first arrival stops and the
-// second writes the union of both children. Atomics::Arrive(counter)
-// returns the previous count with acquire/release ordering.
+// second writes the union of both children and the subtree's SAH cost:
+// a subtree of <= kLeafTriangles triangles is flagged as a leaf when
+// testing all of its triangles costs no more than one more node visit
+// plus its children's costs (the bottom-up leaf choice of Karras and
+// Aila, "Fast Parallel Construction of High-Quality Bounding Volume
+// Hierarchies", HPG 2013, section 4). Atomics::Arrive(counter) returns the
+// previous count with acquire/release ordering.
 template <class Atomics>
 HDCX_HD inline void BottomUpLeaf(std::uint32_t k, const std::uint32_t* child2, const std::uint32_t* parent,
                                  const std::uint32_t* leafParent, std::uint32_t* arrivals, const std::uint32_t* ids,
-                                 const Box4* triBox, Box4* nodeBox)
+                                 const Box4* triBox, const std::uint32_t* range2, Box4* nodeBox, float* nodeCost,
+                                 std::uint32_t* leafFlag)
 {
+    HDCX_NO_CONTRACT
     std::uint32_t p = leafParent[k];
     while (p != kNone) {
         if (Atomics::Arrive(&arrivals[p]) == 0U) return;
@@ -190,24 +213,45 @@ HDCX_HD inline void BottomUpLeaf(std::uint32_t k, const std::uint32_t* child2, c
             u.hi[x] = fmaxf(a.hi[x], b.hi[x]);
         }
         nodeBox[p] = u;
+        const std::uint32_t l = child2[2 * p], r = child2[2 * p + 1];
+        const float costL = (l & kLeafRef) ? kCostTriangle * HalfArea(a) : nodeCost[l];
+        const float costR = (r & kLeafRef) ? kCostTriangle * HalfArea(b) : nodeCost[r];
+        const float area = HalfArea(u);
+        const float split = kCostNode * area + costL + costR;
+        const std::uint32_t size = range2[2 * p + 1] - range2[2 * p] + 1U;
+        const float whole = kCostTriangle * area * static_cast<float>(size);
+        const bool leaf = size <= kLeafTriangles && whole <= split;
+        nodeCost[p] = leaf ? whole : split;
+        leafFlag[p] = leaf ? 1U : 0U;
         p = parent[p];
     }
 }
 
-// Frontier elements among the two children of inner node i: children of
-// <= S triangles under a parent of more. Atomics::Append(count) returns a
-// slot; slots >= capacity are dropped (the host sees the count).
+// A subtree the SAH top levels take as one item: a single triangle, or at
+// most S triangles in a box of half-area <= maxArea. The area bound keeps
+// a large triangle (a ground quad) out of a cluster of small ones, whose
+// nodes would all carry its box.
+HDCX_HD inline bool FrontierItemRef(std::uint32_t ref, std::uint32_t S, float maxArea, const std::uint32_t* range2,
+                                    const Box4* nodeBox)
+{
+    if (ref & kLeafRef) return true;
+    return RefSize(ref, range2) <= S && HalfArea(nodeBox[ref]) <= maxArea;
+}
+
+// Frontier elements among the two children of inner node i: items
+// (FrontierItemRef) under a parent that is not one. Atomics::Append(count)
+// returns a slot; slots >= capacity are dropped (the host sees the count).
 template <class Atomics>
-HDCX_HD inline void FrontierNode(std::uint32_t i, std::uint32_t S, const std::uint32_t* child2,
+HDCX_HD inline void FrontierNode(std::uint32_t i, std::uint32_t S, float maxArea, const std::uint32_t* child2,
                                  const std::uint32_t* range2, const Box4* nodeBox, const Box4* triBox,
                                  const std::uint32_t* ids, std::uint32_t* count, std::uint32_t capacity,
                                  FrontierItem* out)
 {
-    if (RefSize(i, range2) <= S) return;
+    if (FrontierItemRef(i, S, maxArea, range2, nodeBox)) return;
     for (int k = 0; k < 2; ++k) {
         const std::uint32_t ref = child2[2 * i + k];
         const std::uint32_t size = RefSize(ref, range2);
-        if (size > S) continue;
+        if (!FrontierItemRef(ref, S, maxArea, range2, nodeBox)) continue;
         const std::uint32_t slot = Atomics::Append(count);
         if (slot >= capacity) continue;
         FrontierItem f;
@@ -220,14 +264,16 @@ HDCX_HD inline void FrontierNode(std::uint32_t i, std::uint32_t S, const std::ui
 }
 
 // BVH4 node `id` from BVH2 node source[id]: opens the largest-area inner
-// child until four children; inner children of > kLeafTriangles triangles
-// get new node ids from Atomics::Append(&counters[0]) (counters[3] flags
-// an overflow of `capacity`). counters[1] / [2]: max stack, depth.
+// child that is not flagged as a leaf (leafFlag, null for none) until four
+// children; flagged children become leaves over their triangle range, the
+// other inner children get new node ids from Atomics::Append(&counters[0])
+// (counters[3] flags an overflow of `capacity`). counters[1] / [2]: max
+// stack, depth.
 template <class Atomics>
 HDCX_HD inline void CollapseNode(std::uint32_t id, std::uint32_t level, std::uint32_t* source, const std::uint32_t* child2,
                                  const std::uint32_t* range2, const Box4* nodeBox, const Box4* triBox,
-                                 const std::uint32_t* ids, std::uint32_t* nodeStack, std::uint32_t* counters,
-                                 std::uint32_t capacity, Bvh4Node* nodes)
+                                 const std::uint32_t* ids, const std::uint32_t* leafFlag, std::uint32_t* nodeStack,
+                                 std::uint32_t* counters, std::uint32_t capacity, Bvh4Node* nodes)
 {
     const std::uint32_t src = source[id];
     std::uint32_t kids[4];
@@ -238,7 +284,7 @@ HDCX_HD inline void CollapseNode(std::uint32_t id, std::uint32_t level, std::uin
         int best = -1;
         float bestArea = -1.0f;
         for (int i = 0; i < k; ++i) {
-            if (kids[i] & kLeafRef) continue;
+            if ((kids[i] & kLeafRef) || (leafFlag && leafFlag[kids[i]])) continue;
             const float a = HalfArea(nodeBox[kids[i]]);
             if (a > bestArea) {
                 bestArea = a;
@@ -270,7 +316,7 @@ HDCX_HD inline void CollapseNode(std::uint32_t id, std::uint32_t level, std::uin
         const std::uint32_t size = RefSize(kids[i], range2);
         if (kids[i] & kLeafRef) {
             node.child[i] = EncodeLeaf(kids[i] & ~kLeafRef, 1U);
-        } else if (size <= kLeafTriangles) {
+        } else if (leafFlag && leafFlag[kids[i]]) {
             node.child[i] = EncodeLeaf(range2[2 * kids[i]], size);
         } else {
             const std::uint32_t nid = Atomics::Append(&counters[0]);

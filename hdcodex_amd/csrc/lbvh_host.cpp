@@ -29,6 +29,8 @@ void Grow(float lo[3], float hi[3], const float blo[3], const float bhi[3])
 
 float Centre(const FrontierItem& f, int axis) { return 0.5F * (f.box.lo[axis] + f.box.hi[axis]); }
 
+constexpr std::uint32_t kMaxSahDepth = 32;  // SAH splits above this depth, median splits below
+
 struct SerialAtomics {
     static std::uint32_t Arrive(std::uint32_t* c) { return (*c)++; }
     static std::uint32_t Append(std::uint32_t* c) { return (*c)++; }
@@ -38,88 +40,103 @@ struct SerialAtomics {
 
 } // namespace
 
-std::uint32_t BuildSahTop(std::vector<FrontierItem>& items, std::uint32_t firstId, std::vector<std::uint32_t>& child2,
-                          std::vector<std::uint32_t>& range2, std::vector<Box4>& box)
+std::uint32_t BuildSahTop(const std::vector<FrontierItem>& items, std::uint32_t firstId,
+                          std::vector<std::uint32_t>& child2, std::vector<std::uint32_t>& range2,
+                          std::vector<Box4>& box)
 {
-    constexpr int kBins = 32;
-    if (items.size() < 2) throw std::invalid_argument("SAH top needs two frontier items");
+    const auto count = static_cast<std::uint32_t>(items.size());
+    if (count < 2) throw std::invalid_argument("SAH top needs two frontier items");
     child2.clear();
     range2.clear();
     box.clear();
-    // Work items: [b, e) of `items` and the (node, side) slot its reference
-    // goes into; node == kNone for the root.
-    struct Work { std::uint32_t b, e, node, side; };
-    std::vector<Work> work{{0, static_cast<std::uint32_t>(items.size()), lbvh::kNone, 0}};
+    // Item indices sorted by box centre along each axis (ties by index).
+    // Every node owns the same range [b, e) of all three lists; a split
+    // keeps each list sorted by partitioning it stably.
+    std::vector<std::uint32_t> order[3];
+    for (int a = 0; a < 3; ++a) {
+        std::vector<std::pair<float, std::uint32_t>> keys(count);
+        for (std::uint32_t i = 0; i < count; ++i) keys[i] = {Centre(items[i], a), i};
+        std::sort(keys.begin(), keys.end());
+        order[a].resize(count);
+        for (std::uint32_t i = 0; i < count; ++i) order[a][i] = keys[i].second;
+    }
+    std::vector<double> rightArea(count), rightCnt(count);
+    std::vector<std::uint8_t> left(count);
+    std::vector<std::uint32_t> scratch(count);
+    // Work items: [b, e) of the lists, the (node, side) slot its reference
+    // goes into (node == kNone for the root) and its depth. Left children
+    // are taken first, so node ids are in depth-first pre-order.
+    struct Work { std::uint32_t b, e, node, side, depth; };
+    std::vector<Work> work{{0, count, lbvh::kNone, 0, 0}};
     std::uint32_t root = lbvh::kNone;
     while (!work.empty()) {
         const Work w = work.back();
         work.pop_back();
         std::uint32_t ref;
         if (w.e - w.b == 1) {
-            ref = items[w.b].ref;
+            ref = items[order[0][w.b]].ref;
         } else {
             float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (std::uint32_t i = w.b; i < w.e; ++i) {
-                Grow(lo, hi, items[i].box.lo, items[i].box.hi);
+            for (std::uint32_t j = w.b; j < w.e; ++j) Grow(lo, hi, items[order[0][j]].box.lo, items[order[0][j]].box.hi);
+            // Full sweep: every split position along every axis; cost =
+            // area(left) * triangles(left) + area(right) * triangles(right);
+            // equal costs go to the more even split (coincident items).
+            // Below kMaxSahDepth the split is the median along the axis of
+            // largest centre extent, which bounds the depth (and the
+            // traversal stack) for chains of nested items.
+            double best = std::numeric_limits<double>::infinity();
+            int bestAxis = 0;
+            std::uint32_t mid = w.b + (w.e - w.b) / 2;
+            const std::uint32_t half = mid;
+            if (w.depth >= kMaxSahDepth) {
+                float extent = -1.0F;
                 for (int a = 0; a < 3; ++a) {
-                    clo[a] = std::min(clo[a], Centre(items[i], a));
-                    chi[a] = std::max(chi[a], Centre(items[i], a));
+                    const float ext = Centre(items[order[a][w.e - 1]], a) - Centre(items[order[a][w.b]], a);
+                    if (ext > extent) {
+                        extent = ext;
+                        bestAxis = a;
+                    }
                 }
             }
-            double best = std::numeric_limits<double>::infinity();
-            int bestAxis = -1, bestBin = 0;
-            for (int a = 0; a < 3; ++a) {
-                const float ext = chi[a] - clo[a];
-                if (!(ext > 0.0F)) continue;
-                float blo[kBins][3], bhi[kBins][3];
-                double cnt[kBins] = {};
-                for (int k = 0; k < kBins; ++k)
-                    for (int x = 0; x < 3; ++x) {
-                        blo[k][x] = INFINITY;
-                        bhi[k][x] = -INFINITY;
-                    }
-                const float scale = kBins / ext;
-                for (std::uint32_t i = w.b; i < w.e; ++i) {
-                    const int k = std::clamp(static_cast<int>((Centre(items[i], a) - clo[a]) * scale), 0, kBins - 1);
-                    Grow(blo[k], bhi[k], items[i].box.lo, items[i].box.hi);
-                    cnt[k] += items[i].size;
-                }
-                double rightArea[kBins], rightCnt[kBins];
+            for (int a = 0; a < 3 && w.depth < kMaxSahDepth; ++a) {
+                const std::uint32_t* o = order[a].data();
                 float alo[3] = {INFINITY, INFINITY, INFINITY}, ahi[3] = {-INFINITY, -INFINITY, -INFINITY};
                 double acc = 0.0;
-                for (int k = kBins - 1; k > 0; --k) {
-                    Grow(alo, ahi, blo[k], bhi[k]);
-                    acc += cnt[k];
-                    rightArea[k] = acc > 0.0 ? Area(alo, ahi) : 0.0;
-                    rightCnt[k] = acc;
+                for (std::uint32_t j = w.e; j-- > w.b + 1;) {
+                    const FrontierItem& f = items[o[j]];
+                    Grow(alo, ahi, f.box.lo, f.box.hi);
+                    acc += f.size;
+                    rightArea[j] = Area(alo, ahi);
+                    rightCnt[j] = acc;
                 }
                 for (int x = 0; x < 3; ++x) {
                     alo[x] = INFINITY;
                     ahi[x] = -INFINITY;
                 }
                 acc = 0.0;
-                for (int k = 0; k < kBins - 1; ++k) {
-                    Grow(alo, ahi, blo[k], bhi[k]);
-                    acc += cnt[k];
-                    if (acc == 0.0 || rightCnt[k + 1] == 0.0) continue;
-                    const double cost = Area(alo, ahi) * acc + rightArea[k + 1] * rightCnt[k + 1];
-                    if (cost < best) {
+                for (std::uint32_t j = w.b + 1; j < w.e; ++j) {
+                    const FrontierItem& f = items[o[j - 1]];
+                    Grow(alo, ahi, f.box.lo, f.box.hi);
+                    acc += f.size;
+                    const double cost = Area(alo, ahi) * acc + rightArea[j] * rightCnt[j];
+                    const auto skew = [half](std::uint32_t x) { return x > half ? x - half : half - x; };
+                    if (cost < best || (cost == best && skew(j) < skew(mid))) {
                         best = cost;
                         bestAxis = a;
-                        bestBin = k + 1;
+                        mid = j;
                     }
                 }
             }
-            std::uint32_t mid = w.b + (w.e - w.b) / 2;  // coincident centres: split by count
-            if (bestAxis >= 0) {
-                const int a = bestAxis;
-                const float scale = kBins / (chi[a] - clo[a]);
-                auto m = std::partition(items.begin() + w.b, items.begin() + w.e, [&](const FrontierItem& f) {
-                    return std::clamp(static_cast<int>((Centre(f, a) - clo[a]) * scale), 0, kBins - 1) < bestBin;
-                });
-                const auto split = static_cast<std::uint32_t>(m - items.begin());
-                if (split > w.b && split < w.e) mid = split;
+            for (std::uint32_t j = w.b; j < w.e; ++j) left[order[bestAxis][j]] = j < mid ? 1U : 0U;
+            for (int a = 0; a < 3; ++a) {
+                if (a == bestAxis) continue;
+                std::uint32_t* o = order[a].data();
+                std::uint32_t l = w.b, r = 0;
+                for (std::uint32_t j = w.b; j < w.e; ++j) {
+                    if (left[o[j]]) o[l++] = o[j];
+                    else scratch[r++] = o[j];
+                }
+                std::copy(scratch.begin(), scratch.begin() + r, o + l);
             }
             const auto self = static_cast<std::uint32_t>(box.size());
             child2.push_back(lbvh::kNone);
@@ -133,8 +150,8 @@ std::uint32_t BuildSahTop(std::vector<FrontierItem>& items, std::uint32_t firstI
             }
             box.push_back(b);
             ref = firstId + self;
-            work.push_back({mid, w.e, self, 1});
-            work.push_back({w.b, mid, self, 0});
+            work.push_back({mid, w.e, self, 1, w.depth + 1});
+            work.push_back({w.b, mid, self, 0, w.depth + 1});
         }
         if (w.node == lbvh::kNone) root = ref;
         else child2[2 * w.node + w.side] = ref;
@@ -181,7 +198,7 @@ Bvh4 BuildBvh4OverBoxes(const std::vector<Box4>& boxes)
     while (begin < end) {
         for (std::uint32_t id = begin; id < end; ++id)
             lbvh::CollapseNode<SerialAtomics>(id, level, source.data(), child2.data(), range2.data(), nodeBox.data(),
-                                              boxes.data(), ids.data(), nodeStack.data(), counters, inner,
+                                              boxes.data(), ids.data(), nullptr, nodeStack.data(), counters, inner,
                                               out.nodes.data());
         if (counters[3]) throw std::runtime_error("instance tree collapse overflowed its node pool");
         begin = end;
@@ -216,7 +233,7 @@ Bvh4 BuildBvh4LbvhSerial(const float* positions, const std::uint32_t* indices, s
     float lo[3], hi[3];
     VertexBounds(positions, maxIndex + 1, lo, hi);
     float scale[3];
-    for (int k = 0; k < 3; ++k) scale[k] = hi[k] > lo[k] ? 1024.0F / (hi[k] - lo[k]) : 0.0F;
+    lbvh::MortonScale(lo, hi, scale);
 
     std::vector<std::uint32_t> morton(n), ids(n);
     std::vector<Box4> triBox(n);
@@ -235,22 +252,26 @@ Bvh4 BuildBvh4LbvhSerial(const float* positions, const std::uint32_t* indices, s
     std::vector<std::uint32_t> child2(2 * inner, lbvh::kNone), range2(2 * inner, 0U), parent(n, lbvh::kNone),
         leafParent(n, lbvh::kNone), arrivals(n, 0U);
     std::vector<Box4> nodeBox(inner);
+    std::vector<float> nodeCost(n - 1, 0.0F);
+    std::vector<std::uint32_t> leafFlag(inner, 0U);  // SAH top nodes (ids >= n - 1) stay 0
     for (std::uint32_t i = 0; i + 1 < n; ++i)
         lbvh::RadixTreeNode(keys.data(), static_cast<int>(n), static_cast<int>(i), child2.data(), parent.data(),
                             leafParent.data(), range2.data());
     parent[0] = lbvh::kNone;
     for (std::uint32_t k = 0; k < n; ++k)
         lbvh::BottomUpLeaf<SerialAtomics>(k, child2.data(), parent.data(), leafParent.data(), arrivals.data(),
-                                          ids.data(), triBox.data(), nodeBox.data());
+                                          ids.data(), triBox.data(), range2.data(), nodeBox.data(), nodeCost.data(),
+                                          leafFlag.data());
 
     std::uint32_t root = 0;
     if (info) *info = LbvhBuildInfo{};
     if (S && n > S) {
         std::vector<FrontierItem> items(cap);
+        const float maxArea = FrontierMaxArea(lo, hi, n, S);
         std::uint32_t count = 0;
         for (std::uint32_t i = 0; i + 1 < n; ++i)
-            lbvh::FrontierNode<SerialAtomics>(i, S, child2.data(), range2.data(), nodeBox.data(), triBox.data(),
-                                              ids.data(), &count, cap, items.data());
+            lbvh::FrontierNode<SerialAtomics>(i, S, maxArea, child2.data(), range2.data(), nodeBox.data(),
+                                              triBox.data(), ids.data(), &count, cap, items.data());
         if (count >= 2 && count <= cap) {
             items.resize(count);
             std::sort(items.begin(), items.end(), [](const FrontierItem& a, const FrontierItem& b) { return a.ref < b.ref; });
@@ -275,8 +296,8 @@ Bvh4 BuildBvh4LbvhSerial(const float* positions, const std::uint32_t* indices, s
     while (begin < end) {
         for (std::uint32_t id = begin; id < end; ++id)
             lbvh::CollapseNode<SerialAtomics>(id, level, source.data(), child2.data(), range2.data(), nodeBox.data(),
-                                              triBox.data(), ids.data(), nodeStack.data(), counters, nodeCap,
-                                              out.nodes.data());
+                                              triBox.data(), ids.data(), leafFlag.data(), nodeStack.data(), counters,
+                                              nodeCap, out.nodes.data());
         if (counters[3]) throw std::runtime_error("LBVH collapse overflowed its node pool");
         levels.push_back(end);
         begin = end;

@@ -237,3 +237,64 @@ def test_lbvh_sah_top_never_costs_more_nodes_visited_on_the_heightfield():
             emulate_closest(nodes, slots, r[0:3], r[4:7], F(r[3]), F(r[7]), counter=count)
         visits[clusters] = count[0]
     assert visits[2048] <= visits[0] * 1.05, visits
+
+
+def _sah_cost(nodes):
+    """Surface-area-heuristic cost of a BVH4: expected node fetches plus
+    triangle tests of a ray that crosses the root box (both costs 1)."""
+    lo = np.stack([nodes["lox"], nodes["loy"], nodes["loz"]], -1).astype(np.float64)
+    hi = np.stack([nodes["hix"], nodes["hiy"], nodes["hiz"]], -1).astype(np.float64)
+    child = nodes["child"].astype(np.uint64)
+    used = child != EMPTY
+    ext = np.clip(hi - lo, 0, None)
+    area = ext[..., 0] * ext[..., 1] + ext[..., 1] * ext[..., 2] + ext[..., 2] * ext[..., 0]
+    root = np.clip(hi[0][used[0]].max(0) - lo[0][used[0]].min(0), 0, None)
+    root_area = root[0] * root[1] + root[1] * root[2] + root[2] * root[0]
+    leaf = (child & 0x80000000) != 0
+    count = (child & 7) + 1
+    inner = np.sum(np.where(used & ~leaf, area, 0.0))
+    tests = np.sum(np.where(used & leaf, area * count, 0.0))
+    return 1.0 + (inner + tests) / root_area
+
+
+@pytest.mark.parametrize("name,args,bound", [("heightfield", (4, 96, 16), 1.08), ("mixed", (3, 20000, 16), 1.03),
+                                             ("materials", (0,), 1.2)])
+def test_lbvh_sah_cost_close_to_the_host_sah_build(name, args, bound):
+    """Cubic Morton cells and SAH-chosen leaves keep the device builder's
+    tree within a few percent of the full binned-SAH build (a scale per
+    axis and leaves of every subtree <= 4 triangles cost 26% more on the
+    heightfield)."""
+    scene = hx.ProceduralScene(name, *args)
+    _, sah_nodes, _ = hx.build_bvh(scene, hx.BVH_HOST_SAH, 0)
+    _, lbvh_nodes, _ = hx.build_bvh(scene, hx.BVH_SERIAL_LBVH, 2048)
+    ratio = _sah_cost(lbvh_nodes) / _sah_cost(sah_nodes)
+    assert ratio <= bound, ratio
+
+
+@pytest.mark.parametrize("name,args", [("heightfield", (4, 64, 16)), ("mixed", (3, 20000, 16))])
+def test_lbvh_leaves_follow_the_sah_choice(name, args):
+    """A two-triangle subtree becomes one leaf only when testing both
+    triangles costs no more than a node visit plus a single-triangle leaf
+    for each: 2 * area <= area + area(t0) + area(t1) (both costs 1). Larger
+    leaves hold at most four triangles."""
+    scene = hx.ProceduralScene(name, *args)
+    _, nodes, slots = hx.build_bvh(scene, hx.BVH_SERIAL_LBVH, 2048)
+    tri = slots[:, :3, :3].astype(np.float64)
+    tlo, thi = tri.min(axis=1), tri.max(axis=1)
+    text = thi - tlo
+    tarea = text[:, 0] * text[:, 1] + text[:, 1] * text[:, 2] + text[:, 2] * text[:, 0]
+    checked = 0
+    for n in nodes:
+        for i in range(4):
+            c = int(n["child"][i])
+            if c == EMPTY or not (c & 0x80000000) or (c & 7) == 0:
+                continue
+            first, count = (c >> 3) & 0x0FFFFFFF, (c & 7) + 1
+            assert count <= 4
+            if count != 2:
+                continue
+            e = thi[first:first + 2].max(axis=0) - tlo[first:first + 2].min(axis=0)
+            area = e[0] * e[1] + e[1] * e[2] + e[2] * e[0]
+            assert area <= (tarea[first] + tarea[first + 1]) * (1 + 1e-5)
+            checked += 1
+    assert checked > 1000, checked
