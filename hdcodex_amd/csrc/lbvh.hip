@@ -196,12 +196,17 @@ void DeviceBvhBuilder::Build(const float4* positions, const std::uint32_t* indic
     // SAH top levels over the frontier.
     std::uint32_t root = 0;
     if (S && n > S) {
-        Check(hipMemsetAsync(counters, 0, 64, st), "lbvh counters");
-        HDCX_LAUNCH(frontierKernel, g, n, S, FrontierMaxArea(lo, hi, n, S), child2, range2, nodeBox, triBox,
-                    sortedIds, counters, frontierCap, frontier_.as<FrontierItem>());
+        // A frontier that outgrows its capacity under the area bound (many
+        // large triangles) is taken again without it.
         std::uint32_t count = 0;
-        Check(hipMemcpyAsync(&count, counters, 4, hipMemcpyDeviceToHost, st), "lbvh frontier");
-        Check(hipStreamSynchronize(st), "lbvh frontier");
+        for (const float maxArea : {FrontierMaxArea(lo, hi, n, S), INFINITY}) {
+            Check(hipMemsetAsync(counters, 0, 64, st), "lbvh counters");
+            HDCX_LAUNCH(frontierKernel, g, n, S, maxArea, child2, range2, nodeBox, triBox, sortedIds, counters,
+                        frontierCap, frontier_.as<FrontierItem>());
+            Check(hipMemcpyAsync(&count, counters, 4, hipMemcpyDeviceToHost, st), "lbvh frontier");
+            Check(hipStreamSynchronize(st), "lbvh frontier");
+            if (count <= frontierCap) break;
+        }
         if (count >= 2 && count <= frontierCap) {
             std::vector<FrontierItem> items(count);
             Check(hipMemcpy(items.data(), frontier_.as<void>(), sizeof(FrontierItem) * count, hipMemcpyDeviceToHost),

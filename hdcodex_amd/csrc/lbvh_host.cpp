@@ -267,11 +267,14 @@ Bvh4 BuildBvh4LbvhSerial(const float* positions, const std::uint32_t* indices, s
     if (info) *info = LbvhBuildInfo{};
     if (S && n > S) {
         std::vector<FrontierItem> items(cap);
-        const float maxArea = FrontierMaxArea(lo, hi, n, S);
         std::uint32_t count = 0;
-        for (std::uint32_t i = 0; i + 1 < n; ++i)
-            lbvh::FrontierNode<SerialAtomics>(i, S, maxArea, child2.data(), range2.data(), nodeBox.data(),
-                                              triBox.data(), ids.data(), &count, cap, items.data());
+        for (const float maxArea : {FrontierMaxArea(lo, hi, n, S), INFINITY}) {
+            count = 0;
+            for (std::uint32_t i = 0; i + 1 < n; ++i)
+                lbvh::FrontierNode<SerialAtomics>(i, S, maxArea, child2.data(), range2.data(), nodeBox.data(),
+                                                  triBox.data(), ids.data(), &count, cap, items.data());
+            if (count <= cap) break;
+        }
         if (count >= 2 && count <= cap) {
             items.resize(count);
             std::sort(items.begin(), items.end(), [](const FrontierItem& a, const FrontierItem& b) { return a.ref < b.ref; });

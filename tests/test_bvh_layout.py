@@ -189,6 +189,8 @@ def _soup(kind, n, seed=0):
     elif kind == "random":
         c = rng.uniform(-10, 10, (n, 3))
         pos = np.repeat(c, 3, axis=0) + rng.normal(0, 0.3, (3 * n, 3))
+    elif kind == "spanning":  # every triangle spans the scene: no cluster meets the frontier's area bound
+        pos = rng.uniform(-10, 10, (3 * n, 3))
     else:  # "planar": coincident plane, zero extent along y
         c = rng.uniform(-5, 5, (n, 3))
         c[:, 1] = 0.0
@@ -198,13 +200,17 @@ def _soup(kind, n, seed=0):
 
 @pytest.mark.parametrize("clusters", [0, 8, 2048])
 @pytest.mark.parametrize("kind,n", [("identical", 2), ("identical", 5), ("identical", 300), ("geometric", 60),
-                                    ("random", 3), ("random", 4000), ("planar", 500)])
+                                    ("random", 3), ("random", 4000), ("planar", 500), ("spanning", 3000)])
 def test_lbvh_degenerate_inputs(kind, n, clusters):
     """Validated by the library (every triangle once, leaves in range, empty
-    slots inverted); traversal of the result equals brute force."""
+    slots inverted); traversal of the result equals brute force. A frontier
+    that outgrows its capacity under the area bound is taken without it, so
+    the SAH top levels are still built."""
     g = _soup(kind, n)
     info, nodes, slots = hx.build_bvh(g, hx.BVH_SERIAL_LBVH, clusters)
     assert info["slot_count"] == n and info["max_stack"] <= MAX_STACK
+    if clusters and n > -(-n // clusters) and kind in ("random", "spanning"):
+        assert info["top_clusters"] >= 2
     lo, hi = g.positions[:, :3].min(axis=0), g.positions[:, :3].max(axis=0)
     rng = np.random.default_rng(n)
     o = rng.uniform(lo - 1, hi + 1, (40, 3)).astype(F)
