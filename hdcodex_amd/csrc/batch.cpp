@@ -403,7 +403,7 @@ struct hdcx_batch : PathPool {
             Check(LaunchRaygenBatch(pb, bw, cp, maxPaths, maxBlocks, st), "batch raygen");
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
-                Check(LaunchExtendBatch(table, twoLevel, pb, in, maxPaths, maxBlocks, nullptr, st), "batch extend");
+                Check(LaunchExtendBatch(table, twoLevel, pb, in, b == 0, maxPaths, maxBlocks, nullptr, st), "batch extend");
                 const int last = b + 1 == bounces ? 1 : 0;
                 Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, walk, maxPaths, maxBlocks, st),
                       "batch shade");

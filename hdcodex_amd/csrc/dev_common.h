@@ -98,7 +98,8 @@ struct PathBuffers {
     std::uint32_t* queue[2];
     std::uint32_t* shadowQueue;
     std::uint32_t* counters; // [0],[1] extend queue sizes; [2] shadow queue size; [3] walk queue size
-    unsigned long long* totals;  // [0] extend rays, [1] shadow-queued paths, [2] shadow rays traced
+    unsigned long long* totals;  // [0] paths extended, [1] shadow-queued paths, [2] shadow rays traced,
+                                 // [3] extend rays traced (a pixel sample's lanes share the camera ray)
     float* lambda0;          // wavelength of lane 0 of the path's hero triple
     float2* spread;          // radiance cache: primary footprint a0, accumulated sqrt spread
     std::uint32_t* pathScene;  // batch renderer: scene of each path

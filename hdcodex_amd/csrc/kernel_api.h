@@ -9,7 +9,9 @@ hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t
                         unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, std::uint32_t records, unsigned maxBlocks,
                                 hipStream_t st);
-hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+// primary: the launch right after raygen (bounce 0); the three lanes of a
+// pixel sample share one traced camera ray.
+hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, bool primary, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
 // walk: kShadeWalkInline runs subsurface walks inside shade;
 // kShadeWalkDefer queues them (then LaunchWalk, then shade again with
@@ -27,7 +29,8 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32
 // cursor, and the cursor step. maxWork sizes the grid (as a wave's paths or
 // pixels can only be known on the device).
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
-                             std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
+                             bool primary, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
+                             hipStream_t st);
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
                             int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,
                             hipStream_t st);

@@ -210,9 +210,17 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 
 // ---------------------------------------------------------------------------
 // extend: closest hit for every queued path (:1228-1243).
+//
+// primary: the queue is the one raygen wrote, every lane path of the
+// launch in order, three per pixel sample. The three lanes of a sample
+// start with the same camera ray and the same random state (the seed and
+// the jitter do not depend on the lane; only the wavelength does), so the
+// ray is traced once, for lane 0, and its hit (and, in a scene with
+// cut-outs, the random state after the opacity draws) is written to all
+// three: the same values each lane would compute for itself.
 // ---------------------------------------------------------------------------
 template <bool kCount, class Scenes, bool k2L>
-__global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in,
+__global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, bool primary,
                                                       TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -222,26 +230,35 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
         pb.counters[3] = 0U;
     }
     const std::uint32_t count = pb.counters[in];
-    if (blockIdx.x == 0 && threadIdx.x == 0) pb.totals[0] += count;
+    const std::uint32_t lanes = primary && count % 3U == 0U ? 3U : 1U;
+    const std::uint32_t rays = count / lanes;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.totals[0] += count;
+        pb.totals[3] += rays;
+    }
     const std::uint32_t* queue = pb.queue[in];
     TraversalStack st;
     st.lds = stackMem + threadIdx.x;
     st.stride = kBlock;
-    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const std::uint32_t pid = queue[i];
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rays; i += gridDim.x * blockDim.x) {
+        const std::uint32_t pid = lanes == 3U ? 3U * i : queue[i];
         const DeviceScene& s = scenes.Get(pid);
         const float4 o = pb.rayO[pid], d = pb.rayD[pid];
         const RayPre r = prepareRay(o.x, o.y, o.z, d.x, d.y, d.z);
         HitRec h;
         bool found;
+        std::uint32_t rng = 0U;
         if (s.opaque) {
             found = traceClosestS<kCount, k2L>(s, r, 0.001f, 10000.0f, 0.0f, 0xffffffffU, h, st, ctr);
         } else {
-            std::uint32_t rng = pb.rng[pid];
+            rng = pb.rng[pid];
             found = traceStochastic<kCount, k2L>(s, r, rng, h, st, ctr);
-            pb.rng[pid] = rng;
         }
-        pb.hit[pid] = make_float4(h.t, __uint_as_float(found ? h.prim : 0xffffffffU), h.u, h.v);
+        const float4 hit = make_float4(h.t, __uint_as_float(found ? h.prim : 0xffffffffU), h.u, h.v);
+        for (std::uint32_t k = 0; k < lanes; ++k) {
+            pb.hit[pid + k] = hit;
+            if (!s.opaque) pb.rng[pid + k] = rng;
+        }
     }
 }
 
@@ -1252,15 +1269,15 @@ hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, st
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
-hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, bool primary, std::uint32_t maxWork,
                          unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock),
-                           0, st, s, pb, in, ctr);
+                           0, st, s, pb, in, primary, ctr);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)),
-                           dim3(kBlock), 0, st, s, pb, in, ctr);
+                           dim3(kBlock), 0, st, s, pb, in, primary, ctr);
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
@@ -1298,11 +1315,11 @@ hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, std::uint32_t m
                            dim3(kBlock), 0, st, s, pb, ctr);
     return hipGetLastError();
 }
-hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, bool primary, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
-    return s.instanceCount ? LaunchExtendT<SingleScene, true>(SingleScene{s}, pb, in, maxWork, maxBlocks, ctr, st)
-                           : LaunchExtendT<SingleScene, false>(SingleScene{s}, pb, in, maxWork, maxBlocks, ctr, st);
+    return s.instanceCount ? LaunchExtendT<SingleScene, true>(SingleScene{s}, pb, in, primary, maxWork, maxBlocks, ctr, st)
+                           : LaunchExtendT<SingleScene, false>(SingleScene{s}, pb, in, primary, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
                        const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
@@ -1324,11 +1341,12 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32
                            : LaunchShadowT<SingleScene, false>(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
-                             std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+                             bool primary, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
+                             hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchExtendT<SceneTable, true>(t, pb, in, maxWork, maxBlocks, ctr, st)
-                    : LaunchExtendT<SceneTable, false>(t, pb, in, maxWork, maxBlocks, ctr, st);
+    return twoLevel ? LaunchExtendT<SceneTable, true>(t, pb, in, primary, maxWork, maxBlocks, ctr, st)
+                    : LaunchExtendT<SceneTable, false>(t, pb, in, primary, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
                             int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,

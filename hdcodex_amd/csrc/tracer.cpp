@@ -316,7 +316,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
             t->stats.rays[HDCX_KERNEL_RAYGEN] += paths;
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
-                t->Timed(HDCX_KERNEL_EXTEND, [&] { return LaunchExtend(t->scene, pb, in, paths, t->maxBlocks, ctr, st); });
+                const bool primary = b == 0 && !(t->instrumentation & 8U);
+                t->Timed(HDCX_KERNEL_EXTEND,
+                         [&] { return LaunchExtend(t->scene, pb, in, primary, paths, t->maxBlocks, ctr, st); });
                 const int last = b + 1 == bounces ? 1 : 0;
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
                     return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, walk, paths, t->maxBlocks, st);
@@ -794,18 +796,18 @@ int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats)
         tracer->Activate();
         hdcx_kernel_stats s = tracer->stats;
         if (tracer->totals.bytes()) {
-            unsigned long long totals[3] = {0, 0, 0};
+            unsigned long long totals[4] = {0, 0, 0, 0};
             TraversalCounters ctr[2] = {};
             Check(hipMemcpy(totals, tracer->totals.as<void>(), sizeof(totals), hipMemcpyDeviceToHost), "stats");
             Check(hipMemcpy(ctr, tracer->traversalCounters.as<void>(), sizeof(ctr), hipMemcpyDeviceToHost), "stats");
-            s.rays[HDCX_KERNEL_EXTEND] = totals[0];
+            s.rays[HDCX_KERNEL_EXTEND] = totals[3];
             s.rays[HDCX_KERNEL_SHADE] = totals[0];
             s.rays[HDCX_KERNEL_SHADOW] = totals[2];
             s.nodes_visited = ctr[0].nodes;
             s.triangles_tested = ctr[0].triangles;
             s.shadow_nodes_visited = ctr[1].nodes;
             s.shadow_triangles_tested = ctr[1].triangles;
-            s.extend_rays_counted = (tracer->instrumentation & 2U) ? totals[0] : 0;
+            s.extend_rays_counted = (tracer->instrumentation & 2U) ? totals[3] : 0;
             s.shadow_rays_counted = (tracer->instrumentation & 2U) ? totals[2] : 0;
         }
         *out_stats = s;

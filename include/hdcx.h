@@ -420,7 +420,10 @@ int hdcx_batch_get_rc_stats(hdcx_batch* batch, hdcx_batch_rc_stats* out_stats);
 typedef struct hdcx_kernel_stats {
     uint64_t launches[HDCX_KERNEL_COUNT];
     double milliseconds[HDCX_KERNEL_COUNT]; /* HIP-event time summed over launches */
-    uint64_t rays[HDCX_KERNEL_COUNT];       /* work items processed */
+    uint64_t rays[HDCX_KERNEL_COUNT];       /* work items processed: extend counts the
+                                               closest-hit rays traced (the three lanes of
+                                               a pixel sample share one camera ray), shade
+                                               the path vertices */
     uint64_t nodes_visited;                 /* counted only when counting is on */
     uint64_t triangles_tested;
     uint64_t extend_rays_counted;
@@ -438,7 +441,9 @@ typedef struct hdcx_kernel_stats {
  * counters (a separate instantiation; the counting build is slower and is
  * used only to measure algorithmic bytes); bit 2: run subsurface walks
  * inside the shade kernel instead of deferring them to the walk kernel
- * (the same results; kept to check the split against). */
+ * (the same results; kept to check the split against); bit 3: trace the
+ * camera ray of every lane instead of once per pixel sample (the same
+ * results; kept to check the sharing against). */
 int hdcx_set_instrumentation(hdcx_tracer* tracer, uint32_t flags);
 int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats);
 int hdcx_reset_kernel_stats(hdcx_tracer* tracer);

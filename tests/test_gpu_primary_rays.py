@@ -1,0 +1,71 @@
+"""GPU: the three lanes of a pixel sample share one traced camera ray.
+
+The lanes of a sample start from the same seed, jitter and camera ray;
+only their wavelengths differ. The extend launch after raygen traces the
+ray once and writes its hit (and, in a scene with cut-outs, the random
+state after the opacity draws) to all three lanes. Images must equal, bit
+for bit, those rendered with every lane tracing its own camera ray
+(instrumentation bit 3), and the extend kernel must trace two rays fewer
+per pixel sample.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import hdcodex_amd as hx
+
+pytestmark = pytest.mark.gpu
+
+TIMED, PER_LANE_CAMERA_RAYS = 1, 8
+OPACITY_OFFSET = 36  # transmissionOpacityIor.y in the 368-byte material record
+
+
+def _cutout(scene):
+    """The scene view with every other material at opacity 0.5 (not opaque)."""
+    v = hx.SceneView.from_buffer_copy(scene.view)
+    mats = np.array(scene.arrays()["materials"], dtype=np.uint8, copy=True)
+    op = mats[:, OPACITY_OFFSET:OPACITY_OFFSET + 4].copy().view(np.float32).reshape(-1)
+    op[::2] = 0.5
+    mats[:, OPACITY_OFFSET:OPACITY_OFFSET + 4] = op.view(np.uint8).reshape(-1, 4)
+    mats = np.ascontiguousarray(mats)
+    v.materials = mats.ctypes.data
+    v.opaque = 0
+    return v, mats
+
+
+def _render(view, cam, w, h, spp, flags):
+    t = hx.PathTracer(0)
+    try:
+        t.set_scene(view)
+        t.set_instrumentation(TIMED | flags)
+        img = t.render(cam, w, h, 0, 8, spp)
+        return img, t.kernel_stats()
+    finally:
+        t.close()
+
+
+SCENES = [("heightfield", (2, 48, 16)), ("materials", (0,)), ("mixed", (3, 20000, 64)),
+          ("instanced_small", (1, 30, 64))]
+
+
+@pytest.mark.parametrize("cutout", [False, True], ids=["opaque", "cutout"])
+@pytest.mark.parametrize("kind,args", SCENES, ids=[s[0] for s in SCENES])
+def test_shared_camera_rays_equal_per_lane_rays(kind, args, cutout):
+    scene = hx.ProceduralScene(kind, *args)
+    keep = None
+    view = scene
+    if cutout:
+        view, keep = _cutout(scene)
+    w, h, spp = 80, 48, 3
+    cam = scene.camera(w, h)
+    shared, ss = _render(view, cam, w, h, spp, 0)
+    lanes, ls = _render(view, cam, w, h, spp, PER_LANE_CAMERA_RAYS)
+    assert np.all(np.isfinite(shared))
+    assert np.array_equal(shared.view(np.uint32), lanes.view(np.uint32))
+    samples = w * h * spp
+    print(kind, "cutout" if cutout else "opaque", "extend rays", ss["extend"]["items"], ls["extend"]["items"])
+    assert ls["extend"]["items"] - ss["extend"]["items"] == 2 * samples
+    assert ss["shade"]["items"] == ls["shade"]["items"]
+    del keep
