@@ -117,3 +117,14 @@ def test_autotune_picks_and_caches_a_tuning(scenes, tmp_path):
         assert b.stats()["pixel_samples"] == sum(w * h * spp for _, w, h, spp in scenes)
     finally:
         b.close()
+
+
+def test_batch_with_small_waves_is_reproducible(scenes):
+    """Small waves replayed from one graph, whose queues are filled wave by
+    wave in atomic order (a different order on every run): every
+    repetition equals the single-scene renders bit for bit."""
+    refs = [_single(scene, w, h, spp) for scene, w, h, spp in scenes]
+    for rep in range(8):
+        films, _ = _batch(scenes, (3 * 8 * 300, 97, 64, True))
+        for film, ref in zip(films, refs):
+            assert np.array_equal(film.view(np.uint32), ref.view(np.uint32)), rep
