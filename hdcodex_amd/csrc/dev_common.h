@@ -81,6 +81,8 @@ enum PathFlags : std::uint32_t {
 constexpr int kShadowSlots = 3;
 constexpr std::uint32_t kShadowMeshLight = 0xfffffffeU;  // boolean visibility, no shadow controls
 
+constexpr std::uint32_t kCounterStride = 32;  // 32 x 4 bytes: one queue size per 128-byte line
+
 struct PathBuffers {
     float4* rayO;            // origin xyz
     float4* rayD;            // direction xyz
@@ -97,7 +99,10 @@ struct PathBuffers {
     std::uint32_t* shadowLight;  // light index, kShadowMeshLight, or ~0 = empty
     std::uint32_t* queue[2];
     std::uint32_t* shadowQueue;
-    std::uint32_t* counters; // [0],[1] extend queue sizes; [2] shadow queue size; [3] walk queue size
+    // Queue sizes, one per 128-byte line (index k at counters[k * kCounterStride],
+    // so that the per-wave atomics on different queues do not share a line):
+    // [0],[1] extend queue sizes; [2] shadow queue size; [3] walk queue size.
+    std::uint32_t* counters;
     unsigned long long* totals;  // [0] paths extended, [1] shadow-queued paths, [2] shadow rays traced,
                                  // [3] extend rays traced (a pixel sample's lanes share the camera ray)
     float* lambda0;          // wavelength of lane 0 of the path's hero triple

@@ -225,11 +225,11 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.counters[in ^ 1U] = 0U;
-        pb.counters[2] = 0U;
-        pb.counters[3] = 0U;
+        pb.counters[(in ^ 1U) * kCounterStride] = 0U;
+        pb.counters[2 * kCounterStride] = 0U;
+        pb.counters[3 * kCounterStride] = 0U;
     }
-    const std::uint32_t count = pb.counters[in];
+    const std::uint32_t count = pb.counters[in * kCounterStride];
     const std::uint32_t lanes = primary && count % 3U == 0U ? 3U : 1U;
     const std::uint32_t rays = count / lanes;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -341,10 +341,11 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
     TraversalStack st;
     st.lds = stackMem + threadIdx.x;
     st.stride = kBlock;
-    const std::uint32_t count = kWalk == kShadeWalkResume ? pb.counters[3] : pb.counters[in];
+    const std::uint32_t count =
+        kWalk == kShadeWalkResume ? pb.counters[3 * kCounterStride] : pb.counters[in * kCounterStride];
     const std::uint32_t* queue = kWalk == kShadeWalkResume ? pb.walkQueue : pb.queue[in];
     std::uint32_t* outQueue = pb.queue[in ^ 1U];
-    std::uint32_t* outCounter = pb.counters + (in ^ 1U);
+    std::uint32_t* outCounter = pb.counters + (in ^ 1U) * kCounterStride;
     // Every lane of a wave must reach the ballots: iterate over the padded range.
     const std::uint32_t stride = gridDim.x * blockDim.x;
     const std::uint32_t padded = (count + stride - 1U) / stride * stride;
@@ -937,14 +938,14 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
         waveAppend(keep, pid, outQueue, outCounter);
         if constexpr (kWalk != kShadeWalkResume) {
             // A resumed path queued its shadow rays when it was deferred.
-            waveAppend(slot > 0, pid, pb.shadowQueue, pb.counters + 2);
+            waveAppend(slot > 0, pid, pb.shadowQueue, pb.counters + 2 * kCounterStride);
             if (cp.enabled) {
                 const std::uint32_t qslot = waveSlot(wantQuery, cp.queryCount);
                 if (wantQuery && qslot < cp.queryCapacity) cp.queries[qslot] = query;
             }
         }
         if constexpr (kWalk == kShadeWalkDefer) {
-            const std::uint32_t wslot = waveSlot(deferred, pb.counters + 3);
+            const std::uint32_t wslot = waveSlot(deferred, pb.counters + 3 * kCounterStride);
             if (deferred) {
                 pb.walk[3U * wslot] = walkIn0;
                 pb.walk[3U * wslot + 1U] = walkIn1;
@@ -965,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void walkKernel(Scenes scenes, PathBuffers 
     TraversalStack st;
     st.lds = stackMem + threadIdx.x;
     st.stride = kBlock;
-    const std::uint32_t count = pb.counters[3];
+    const std::uint32_t count = pb.counters[3 * kCounterStride];
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const DeviceScene& s = scenes.Get(pb.walkQueue[i]);
         const float4 w0 = pb.walk[3U * i], w1 = pb.walk[3U * i + 1U], w2 = pb.walk[3U * i + 2U];
@@ -995,7 +996,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
     if (threadIdx.x == 0) tracedInBlock = 0U;
     __syncthreads();
     unsigned int traced = 0U;
-    const std::uint32_t count = pb.counters[2];
+    const std::uint32_t count = pb.counters[2 * kCounterStride];
     if (blockIdx.x == 0 && threadIdx.x == 0) pb.totals[1] += count;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const std::uint32_t pid = pb.shadowQueue[i];

@@ -514,7 +514,7 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
 void PathPool::EnsurePaths(std::uint32_t paths)
 {
     if (!counters.bytes()) {
-        counters.Allocate(64);
+        counters.Allocate(4 * kCounterStride * sizeof(std::uint32_t));
         totals.Allocate(64);
         Check(hipMemset(totals.as<void>(), 0, 64), "memset totals");
     }
