@@ -442,8 +442,8 @@ def main() -> int:
     ap.add_argument("--cache-warmup", type=int, default=50)
     ap.add_argument("--bvh", choices=("lbvh", "sah"), default="sah",
                     help="BVH builder: device LBVH with SAH top levels, or host binned SAH")
-    ap.add_argument("--batch", action="store_true",
-                    help="also render the 64-scene batch workload (adds about a minute)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-scene batch workload (about a minute and a half)")
     ap.add_argument("--batch-only", action="store_true", help="only the 64-scene batch workload")
     ap.add_argument("--no-instanced", action="store_true", help="skip the 10,000-instance two-level BVH workload")
     ap.add_argument("--instanced-only", action="store_true", help="only the 10,000-instance workload")
@@ -596,7 +596,7 @@ def main() -> int:
     if not args.no_instanced:
         log("instanced workload")
         result["instanced"] = instanced_workload(hx)
-    if args.batch:
+    if not args.no_batch:
         log("batch workload")
         result["batch"] = batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph,
                                          cache=not args.no_batch_cache)
