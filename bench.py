@@ -143,10 +143,18 @@ def equal_time(hx, tracer, cam, plain_frame: np.ndarray, plain_ms: float, warmup
         tracer.render(cam, WIDTH, HEIGHT, 1000 + 2 * i, BOUNCES, 2, readback=False)
     warm_s = time.perf_counter() - t0
     stats0 = tracer.rc_stats()
-    t0 = time.perf_counter()
-    tracer.render(cam, WIDTH, HEIGHT, 0, BOUNCES, 4, readback=False)
-    per_spp = (time.perf_counter() - t0) / 4
-    spp = max(1, int(plain_ms * 1e-3 / per_spp))
+    # Frame time = fixed cost (training, launches) + per-spp cost: fitted on
+    # 4- and 12-spp frames; the cache gets the largest spp that fits in 98%
+    # of the plain frame's time (the margin keeps timing noise from giving
+    # the cache more time than the plain frame).
+    times = {}
+    for n in (4, 12):
+        t0 = time.perf_counter()
+        tracer.render(cam, WIDTH, HEIGHT, 0, BOUNCES, n, readback=False)
+        times[n] = time.perf_counter() - t0
+    per_spp = max((times[12] - times[4]) / 8, 1e-6)
+    fixed = max(times[4] - 4 * per_spp, 0.0)
+    spp = max(1, int((0.98 * plain_ms * 1e-3 - fixed) / per_spp))
     tracer.reset_kernel_stats()
     tracer.set_instrumentation(1)
     before = tracer.rc_stats()

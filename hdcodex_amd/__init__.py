@@ -31,7 +31,7 @@ SHADING_FUSED = 0
 SHADING_MODULAR = 1
 SHADING_RASTER_PREVIEW = 2
 
-KERNELS = ("raygen", "extend", "shade", "shadow", "film", "cache", "walk")
+KERNELS = ("raygen", "extend", "shade", "shadow", "film", "cache", "walk", "pair")
 
 # Every symbol include/hdcx.h declares.
 EXPORTED_SYMBOLS = (
@@ -80,7 +80,7 @@ class SceneView(ctypes.Structure):
 
 class KernelStats(ctypes.Structure):
     _fields_ = [
-        ("launches", c_uint64 * 7), ("milliseconds", c_double * 7), ("rays", c_uint64 * 7),
+        ("launches", c_uint64 * 8), ("milliseconds", c_double * 8), ("rays", c_uint64 * 8),
         ("nodes_visited", c_uint64), ("triangles_tested", c_uint64), ("extend_rays_counted", c_uint64),
         ("shadow_rays_counted", c_uint64), ("shadow_nodes_visited", c_uint64), ("shadow_triangles_tested", c_uint64),
         ("node_bytes", c_uint64), ("triangle_bytes", c_uint64), ("bvh_nodes", c_uint64), ("bvh_triangles", c_uint64),

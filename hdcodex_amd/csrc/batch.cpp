@@ -403,7 +403,9 @@ struct hdcx_batch : PathPool {
             Check(LaunchRaygenBatch(pb, bw, cp, maxPaths, maxBlocks, st), "batch raygen");
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
-                Check(LaunchExtendBatch(table, twoLevel, pb, in, b == 0, maxPaths, maxBlocks, nullptr, st), "batch extend");
+                Check(LaunchExtendBatch(table, twoLevel, pb, in, b == 0 ? kShareCamera : kShareLanes, maxPaths, maxBlocks,
+                                        nullptr, st),
+                      "batch extend");
                 const int last = b + 1 == bounces ? 1 : 0;
                 Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, walk, maxPaths, maxBlocks, st),
                       "batch shade");
@@ -413,6 +415,7 @@ struct hdcx_batch : PathPool {
                                            maxBlocks, st),
                           "batch shade (resume)");
                 }
+                if (!last) Check(LaunchPairBatch(table, pb, in ^ 1U, maxPaths, maxBlocks, st), "batch pair");
                 if (cache && (b == cp.terminateBounce || (b > cp.terminateBounce && cp.spreadThreshold > 0.0F))) {
                     // Queries of this bounce, answered by their scenes' networks.
                     Check(LaunchRcInferScenes(rcWe.as<__bf16>(), cp.queries, cp.queryCount, maxPaths, pb.pathScene, buckets,
@@ -542,7 +545,7 @@ struct hdcx_batch : PathPool {
                 const double ms = measure(t);
                 if (ms < bestMs) { bestMs = ms; best = t; }
             }
-            for (std::uint32_t wp : {3U << 19, 3U << 21}) {
+            for (std::uint32_t wp : {3U << 19, 3U << 21, 3U << 23}) {
                 hdcx_batch_tuning t = best;
                 t.wave_paths = wp;
                 const double ms = measure(t);

@@ -99,12 +99,15 @@ struct PathBuffers {
     std::uint32_t* shadowLight;  // light index, kShadowMeshLight, or ~0 = empty
     std::uint32_t* queue[2];
     std::uint32_t* shadowQueue;
+    std::uint32_t* extendQueue[2];  // lane sharing: the surviving paths that trace their ray
+    std::uint32_t* follow;   // lane sharing: bit j = path pid + 1 + j takes this path's hit
     // Queue sizes, one per 128-byte line (index k at counters[k * kCounterStride],
     // so that the per-wave atomics on different queues do not share a line):
-    // [0],[1] extend queue sizes; [2] shadow queue size; [3] walk queue size.
+    // [0],[1] path queue sizes; [2] shadow queue size; [3] walk queue size;
+    // [4],[5] extendQueue sizes.
     std::uint32_t* counters;
     unsigned long long* totals;  // [0] paths extended, [1] shadow-queued paths, [2] shadow rays traced,
-                                 // [3] extend rays traced (a pixel sample's lanes share the camera ray)
+                                 // [3] extend rays traced (lanes of a pixel sample trace an equal ray once)
     float* lambda0;          // wavelength of lane 0 of the path's hero triple
     float2* spread;          // radiance cache: primary footprint a0, accumulated sqrt spread
     std::uint32_t* pathScene;  // batch renderer: scene of each path

@@ -9,10 +9,18 @@ hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t
                         unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, std::uint32_t records, unsigned maxBlocks,
                                 hipStream_t st);
-// primary: the launch right after raygen (bounce 0); the three lanes of a
-// pixel sample share one traced camera ray.
-hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, bool primary, std::uint32_t maxWork,
+// Extend sharing: kShareNone traces every queued path; kShareCamera (the
+// launch after raygen) traces one camera ray per pixel sample for its
+// three lanes; kShareLanes traces the paths LaunchPair queued as tracing
+// and writes their hits to the later lanes of their sample with the same
+// ray.
+constexpr int kShareNone = 0, kShareCamera = 1, kShareLanes = 2;
+hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
+// Lane sharing for the next extend (run with kShareLanes), over the paths
+// that survived this bounce: queue[out], after shade and its walk resume.
+hipError_t LaunchPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
+                      unsigned maxBlocks, hipStream_t st);
 // walk: kShadeWalkInline runs subsurface walks inside shade;
 // kShadeWalkDefer queues them (then LaunchWalk, then shade again with
 // kShadeWalkResume over the walk queue: same results, the walks' traversal
@@ -29,8 +37,10 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32
 // cursor, and the cursor step. maxWork sizes the grid (as a wave's paths or
 // pixels can only be known on the device).
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
-                             bool primary, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
+                             int sharing, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
                              hipStream_t st);
+hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
+                           unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
                             int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,
                             hipStream_t st);

@@ -211,16 +211,20 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // ---------------------------------------------------------------------------
 // extend: closest hit for every queued path (:1228-1243).
 //
-// primary: the queue is the one raygen wrote, every lane path of the
-// launch in order, three per pixel sample. The three lanes of a sample
-// start with the same camera ray and the same random state (the seed and
-// the jitter do not depend on the lane; only the wavelength does), so the
-// ray is traced once, for lane 0, and its hit (and, in a scene with
-// cut-outs, the random state after the opacity draws) is written to all
-// three: the same values each lane would compute for itself.
+// sharing: kShareNone traces every queued path. kShareCamera (the launch
+// after raygen): the queue is every lane path of the launch in order,
+// three per pixel sample, and the three lanes of a sample start with the
+// same camera ray and the same random state (the seed and the jitter do
+// not depend on the lane; only the wavelength does), so the ray is traced
+// once, for lane 0. kShareLanes: pairKernel queued the paths that trace
+// (pb.extendQueue) with a mask of the later lanes of their sample whose
+// next ray it found equal (pb.follow). The hit (and, in a scene with
+// cut-outs, the random state after the opacity draws) is written to the
+// path and to the lanes that share its ray: the same values each lane
+// would compute for itself.
 // ---------------------------------------------------------------------------
 template <bool kCount, class Scenes, bool k2L>
-__global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, bool primary,
+__global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
                                                       TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -228,20 +232,23 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
         pb.counters[(in ^ 1U) * kCounterStride] = 0U;
         pb.counters[2 * kCounterStride] = 0U;
         pb.counters[3 * kCounterStride] = 0U;
+        pb.counters[(4U + (in ^ 1U)) * kCounterStride] = 0U;
     }
     const std::uint32_t count = pb.counters[in * kCounterStride];
-    const std::uint32_t lanes = primary && count % 3U == 0U ? 3U : 1U;
-    const std::uint32_t rays = count / lanes;
+    const bool camera = sharing == kShareCamera && count % 3U == 0U;
+    const bool leaders = sharing == kShareLanes;
+    const std::uint32_t rays = camera ? count / 3U : leaders ? pb.counters[(4U + in) * kCounterStride] : count;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         pb.totals[0] += count;
         pb.totals[3] += rays;
     }
-    const std::uint32_t* queue = pb.queue[in];
+    const std::uint32_t* queue = leaders ? pb.extendQueue[in] : pb.queue[in];
     TraversalStack st;
     st.lds = stackMem + threadIdx.x;
     st.stride = kBlock;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rays; i += gridDim.x * blockDim.x) {
-        const std::uint32_t pid = lanes == 3U ? 3U * i : queue[i];
+        const std::uint32_t pid = camera ? 3U * i : queue[i];
+        const std::uint32_t followers = camera ? 3U : leaders ? pb.follow[pid] : 0U;
         const DeviceScene& s = scenes.Get(pid);
         const float4 o = pb.rayO[pid], d = pb.rayD[pid];
         const RayPre r = prepareRay(o.x, o.y, o.z, d.x, d.y, d.z);
@@ -255,10 +262,104 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
             found = traceStochastic<kCount, k2L>(s, r, rng, h, st, ctr);
         }
         const float4 hit = make_float4(h.t, __uint_as_float(found ? h.prim : 0xffffffffU), h.u, h.v);
-        for (std::uint32_t k = 0; k < lanes; ++k) {
-            pb.hit[pid + k] = hit;
-            if (!s.opaque) pb.rng[pid + k] = rng;
+        pb.hit[pid] = hit;
+        if (!s.opaque) pb.rng[pid] = rng;
+        for (std::uint32_t k = 0; k < 2U; ++k) {
+            if (!(followers & (1U << k))) continue;
+            pb.hit[pid + 1U + k] = hit;
+            if (!s.opaque) pb.rng[pid + 1U + k] = rng;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pair: lane sharing for the next extend (kShareLanes), over the paths that
+// survived this bounce (queue[out]) in queue order. With the same random
+// numbers, the lanes of a pixel sample keep the same rays until a lobe or
+// light choice differs. A path whose next ray (and, in a scene with
+// cut-outs, random state) equals bit for bit that of an earlier lane of its
+// sample, held by one of the two preceding threads of the wave, is not
+// traced: that lane's extend writes the hit to both. Lanes of a sample have
+// consecutive path ids (lane = pid % 3) and keep their order in the queues,
+// but another path can sit between two of them (the queues are filled wave
+// by wave in atomic order), so the earliest equal lane may be out of a
+// path's sight while the lane it sees follows that earlier one: the
+// earliest visible equal lane is taken, and kept only if it traces itself,
+// so every path that does not trace has a leader that does. The tracing
+// paths go to extendQueue[out], one atomic per block of 16 waves, with the
+// mask of their followers in follow[].
+// ---------------------------------------------------------------------------
+constexpr unsigned kPairBlock = 1024;
+
+template <class Scenes>
+__global__ __launch_bounds__(kPairBlock) void pairKernel(Scenes scenes, PathBuffers pb, std::uint32_t out)
+{
+    constexpr unsigned kWaves = kPairBlock / 64U;
+    __shared__ std::uint32_t waveCount[kWaves], waveBase[kWaves];
+    const std::uint32_t count = pb.counters[out * kCounterStride];
+    const std::uint32_t* queue = pb.queue[out];
+    std::uint32_t* counter = pb.counters + (4U + out) * kCounterStride;
+    const std::uint32_t stride = gridDim.x * blockDim.x;
+    const std::uint32_t padded = (count + stride - 1U) / stride * stride;  // every thread reaches the barriers
+    const std::uint32_t lane = laneId(), wave = threadIdx.x >> 6;
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < padded; i += stride) {
+        const bool live = i < count;
+        std::uint32_t pid = 0U, rng = 0U;
+        float4 o{}, d{};
+        bool opaque = true;
+        if (live) {
+            pid = queue[i];
+            o = pb.rayO[pid];
+            d = pb.rayD[pid];
+            opaque = scenes.Get(pid).opaque != 0U;
+            if (!opaque) rng = pb.rng[pid];
+        }
+        const std::uint32_t sampleLane = pid % 3U;
+        std::uint32_t back = 0U;  // 0: traced here; 1 or 2: path pid - back traces it
+        std::uint32_t dist = 0U;  // threads back to that path
+        for (std::uint32_t k = 1U; k <= 2U; ++k) {
+            const std::uint32_t qp = __shfl_up(pid, k);
+            const int ql = __shfl_up(static_cast<int>(live), k);
+            const float qox = __shfl_up(o.x, k), qoy = __shfl_up(o.y, k), qoz = __shfl_up(o.z, k);
+            const float qdx = __shfl_up(d.x, k), qdy = __shfl_up(d.y, k), qdz = __shfl_up(d.z, k);
+            const std::uint32_t qr = __shfl_up(rng, k);
+            const std::uint32_t b = pid - qp;
+            const bool same = live && lane >= k && ql != 0 && qp < pid && b <= sampleLane &&
+                              __float_as_uint(qox) == __float_as_uint(o.x) && __float_as_uint(qoy) == __float_as_uint(o.y) &&
+                              __float_as_uint(qoz) == __float_as_uint(o.z) && __float_as_uint(qdx) == __float_as_uint(d.x) &&
+                              __float_as_uint(qdy) == __float_as_uint(d.y) && __float_as_uint(qdz) == __float_as_uint(d.z) &&
+                              (opaque || qr == rng);
+            if (same && b > back) {
+                back = b;
+                dist = k;
+            }
+        }
+        const std::uint32_t leaderBack1 = __shfl_up(back, 1U), leaderBack2 = __shfl_up(back, 2U);
+        if (back != 0U && (dist == 1U ? leaderBack1 : leaderBack2) != 0U) back = 0U;
+        std::uint32_t followers = 0U;  // bit j: path pid + 1 + j takes this path's hit
+        for (std::uint32_t k = 1U; k <= 2U; ++k) {
+            const std::uint32_t qp = __shfl_down(pid, k);
+            const std::uint32_t qb = __shfl_down(back, k);
+            if (lane + k < 64U && qb != 0U && qp - qb == pid) followers |= 1U << (qp - pid - 1U);
+        }
+        const bool traces = live && back == 0U;
+        if (traces) pb.follow[pid] = followers;
+        const unsigned long long mask = __ballot(traces);
+        if (lane == 0U) waveCount[wave] = static_cast<std::uint32_t>(__popcll(mask));
+        __syncthreads();
+        if (threadIdx.x == 0U) {
+            std::uint32_t total = 0U;
+            for (unsigned k = 0; k < kWaves; ++k) total += waveCount[k];
+            std::uint32_t at = total ? atomicAdd(counter, total) : 0U;
+            for (unsigned k = 0; k < kWaves; ++k) {
+                waveBase[k] = at;
+                at += waveCount[k];
+            }
+        }
+        __syncthreads();
+        if (traces)
+            pb.extendQueue[out][waveBase[wave] + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)))] = pid;
+        __syncthreads();
     }
 }
 
@@ -1270,15 +1371,23 @@ hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, st
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
-hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, bool primary, std::uint32_t maxWork,
+hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
                          unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock),
-                           0, st, s, pb, in, primary, ctr);
+                           0, st, s, pb, in, sharing, ctr);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)),
-                           dim3(kBlock), 0, st, s, pb, in, primary, ctr);
+                           dim3(kBlock), 0, st, s, pb, in, sharing, ctr);
+    return hipGetLastError();
+}
+template <class Scenes>
+hipError_t LaunchPairT(const Scenes& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
+                       unsigned maxBlocks, hipStream_t st)
+{
+    const unsigned blocks = std::max(1U, std::min((maxWork + kPairBlock - 1U) / kPairBlock, maxBlocks));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(pairKernel<Scenes>), dim3(blocks), dim3(kPairBlock), 0, st, s, pb, out);
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
@@ -1316,11 +1425,16 @@ hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, std::uint32_t m
                            dim3(kBlock), 0, st, s, pb, ctr);
     return hipGetLastError();
 }
-hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, bool primary, std::uint32_t maxWork,
+hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
-    return s.instanceCount ? LaunchExtendT<SingleScene, true>(SingleScene{s}, pb, in, primary, maxWork, maxBlocks, ctr, st)
-                           : LaunchExtendT<SingleScene, false>(SingleScene{s}, pb, in, primary, maxWork, maxBlocks, ctr, st);
+    return s.instanceCount ? LaunchExtendT<SingleScene, true>(SingleScene{s}, pb, in, sharing, maxWork, maxBlocks, ctr, st)
+                           : LaunchExtendT<SingleScene, false>(SingleScene{s}, pb, in, sharing, maxWork, maxBlocks, ctr, st);
+}
+hipError_t LaunchPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
+                      unsigned maxBlocks, hipStream_t st)
+{
+    return LaunchPairT<SingleScene>(SingleScene{s}, pb, out, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
                        const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
@@ -1342,12 +1456,17 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32
                            : LaunchShadowT<SingleScene, false>(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
-                             bool primary, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
+                             int sharing, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
                              hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchExtendT<SceneTable, true>(t, pb, in, primary, maxWork, maxBlocks, ctr, st)
-                    : LaunchExtendT<SceneTable, false>(t, pb, in, primary, maxWork, maxBlocks, ctr, st);
+    return twoLevel ? LaunchExtendT<SceneTable, true>(t, pb, in, sharing, maxWork, maxBlocks, ctr, st)
+                    : LaunchExtendT<SceneTable, false>(t, pb, in, sharing, maxWork, maxBlocks, ctr, st);
+}
+hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
+                           unsigned maxBlocks, hipStream_t st)
+{
+    return LaunchPairT<SceneTable>(SceneTable{table, pb.pathScene}, pb, out, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
                             int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,

@@ -514,7 +514,7 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
 void PathPool::EnsurePaths(std::uint32_t paths)
 {
     if (!counters.bytes()) {
-        counters.Allocate(4 * kCounterStride * sizeof(std::uint32_t));
+        counters.Allocate(6 * kCounterStride * sizeof(std::uint32_t));
         totals.Allocate(64);
         Check(hipMemset(totals.as<void>(), 0, 64), "memset totals");
     }
@@ -541,6 +541,9 @@ void PathPool::EnsurePaths(std::uint32_t paths)
     pathScene.Allocate(n * 4);
     walk.Allocate(n * 48);
     walkQueue.Allocate(n * 4);
+    extendQueue0.Allocate(n * 4);
+    extendQueue1.Allocate(n * 4);
+    follow.Allocate(n * 4);
     capacity = paths;
 }
 
@@ -570,6 +573,9 @@ PathBuffers PathPool::Paths()
     pb.pathScene = pathScene.as<std::uint32_t>();
     pb.walk = walk.as<float4>();
     pb.walkQueue = walkQueue.as<std::uint32_t>();
+    pb.extendQueue[0] = extendQueue0.as<std::uint32_t>();
+    pb.extendQueue[1] = extendQueue1.as<std::uint32_t>();
+    pb.follow = follow.as<std::uint32_t>();
     pb.capacity = capacity;
     return pb;
 }

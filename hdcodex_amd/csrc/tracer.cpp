@@ -29,7 +29,7 @@
 namespace hdcx {
 namespace {
 
-constexpr std::uint32_t kChunkPixelSamples = 1U << 21;  // 6.3M lane paths per chunk
+constexpr std::uint32_t kChunkPixelSamples = 1U << 25;  // 100.7M lane paths (~29 GB of path state) per chunk
 
 } // namespace
 } // namespace hdcx
@@ -252,6 +252,12 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     std::uint32_t pixelsPerChunk = std::min(pixels, kChunkPixelSamples);
     std::uint32_t samplesPerChunk = pixels <= kChunkPixelSamples ? std::max(1U, kChunkPixelSamples / pixels) : 1U;
     samplesPerChunk = std::min(samplesPerChunk, samples);
+    {
+        // The same number of samples in every chunk (to within one): a short
+        // last chunk would run its launches on a fraction of the GPU.
+        const std::uint32_t chunks = (samples + samplesPerChunk - 1U) / samplesPerChunk;
+        samplesPerChunk = (samples + chunks - 1U) / chunks;
+    }
     t->EnsurePaths(pixelsPerChunk * samplesPerChunk * 3U);
     PathBuffers pb = t->Paths();
     // Radiance cache: training pixel samples are spread evenly over the
@@ -292,6 +298,7 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     }
     TraversalCounters* ctr = (t->instrumentation & 2U) ? t->traversalCounters.as<TraversalCounters>() : nullptr;
     const int walk = !t->scene.hasSubsurface || (t->instrumentation & 4U) ? kShadeWalkInline : kShadeWalkDefer;
+    const bool share = !(t->instrumentation & 8U);  // lanes of a pixel sample trace an equal ray once
     TraversalCounters* shadowCtr = ctr ? ctr + 1 : nullptr;
     ChunkDesc c{};
     c.width = w;
@@ -316,9 +323,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
             t->stats.rays[HDCX_KERNEL_RAYGEN] += paths;
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
-                const bool primary = b == 0 && !(t->instrumentation & 8U);
+                const int sharing = !share ? kShareNone : b == 0 ? kShareCamera : kShareLanes;
                 t->Timed(HDCX_KERNEL_EXTEND,
-                         [&] { return LaunchExtend(t->scene, pb, in, primary, paths, t->maxBlocks, ctr, st); });
+                         [&] { return LaunchExtend(t->scene, pb, in, sharing, paths, t->maxBlocks, ctr, st); });
                 const int last = b + 1 == bounces ? 1 : 0;
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
                     return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, walk, paths, t->maxBlocks, st);
@@ -329,6 +336,8 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                         return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, kShadeWalkResume, paths, t->maxBlocks, st);
                     });
                 }
+                if (share && !last)
+                    t->Timed(HDCX_KERNEL_PAIR, [&] { return LaunchPair(t->scene, pb, in ^ 1U, paths, t->maxBlocks, st); });
                 // Queries come from the terminate vertex, or from any later
                 // one when the spread rule is on.
                 if (cache && (b == cp.terminateBounce || (b > cp.terminateBounce && cp.spreadThreshold > 0.0F))) {

@@ -415,15 +415,16 @@ int hdcx_batch_get_rc_stats(hdcx_batch* batch, hdcx_batch_rc_stats* out_stats);
 #define HDCX_KERNEL_FILM 4
 #define HDCX_KERNEL_CACHE 5 /* radiance-cache inference */
 #define HDCX_KERNEL_WALK 6  /* subsurface random walks deferred by shade */
-#define HDCX_KERNEL_COUNT 7
+#define HDCX_KERNEL_PAIR 7  /* finds the lanes of a pixel sample with equal next rays */
+#define HDCX_KERNEL_COUNT 8
 
 typedef struct hdcx_kernel_stats {
     uint64_t launches[HDCX_KERNEL_COUNT];
     double milliseconds[HDCX_KERNEL_COUNT]; /* HIP-event time summed over launches */
     uint64_t rays[HDCX_KERNEL_COUNT];       /* work items processed: extend counts the
-                                               closest-hit rays traced (the three lanes of
-                                               a pixel sample share one camera ray), shade
-                                               the path vertices */
+                                               closest-hit rays traced (lanes of a pixel
+                                               sample with equal rays trace them once),
+                                               shade the path vertices */
     uint64_t nodes_visited;                 /* counted only when counting is on */
     uint64_t triangles_tested;
     uint64_t extend_rays_counted;
@@ -441,9 +442,9 @@ typedef struct hdcx_kernel_stats {
  * counters (a separate instantiation; the counting build is slower and is
  * used only to measure algorithmic bytes); bit 2: run subsurface walks
  * inside the shade kernel instead of deferring them to the walk kernel
- * (the same results; kept to check the split against); bit 3: trace the
- * camera ray of every lane instead of once per pixel sample (the same
- * results; kept to check the sharing against). */
+ * (the same results; kept to check the split against); bit 3: every lane
+ * traces its own rays (by default the lanes of a pixel sample trace an
+ * equal ray once: the same results; kept to check the sharing against). */
 int hdcx_set_instrumentation(hdcx_tracer* tracer, uint32_t flags);
 int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats);
 int hdcx_reset_kernel_stats(hdcx_tracer* tracer);
