@@ -423,6 +423,7 @@ struct hdcx_batch : PathPool {
                           "batch rc infer");
                     Check(LaunchConsumeQueries(cp, st), "batch rc queries");
                 }
+                Check(LaunchShadowPairBatch(table, pb, maxPaths, maxBlocks, st), "batch shadow pair");
                 Check(LaunchShadowBatch(table, twoLevel, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
             }
             if (cache)

@@ -101,10 +101,12 @@ struct PathBuffers {
     std::uint32_t* shadowQueue;
     std::uint32_t* extendQueue[2];  // lane sharing: the surviving paths that trace their ray
     std::uint32_t* follow;   // lane sharing: bit j = path pid + 1 + j takes this path's hit
+    std::uint32_t* shadowLeaders;  // shadow sharing: the queued paths that trace their shadow rays
+    std::uint32_t* shadowFollow;   // shadow sharing: bit j = path pid + 1 + j shares them
     // Queue sizes, one per 128-byte line (index k at counters[k * kCounterStride],
     // so that the per-wave atomics on different queues do not share a line):
     // [0],[1] path queue sizes; [2] shadow queue size; [3] walk queue size;
-    // [4],[5] extendQueue sizes.
+    // [4],[5] extendQueue sizes; [6] shadowLeaders size.
     std::uint32_t* counters;
     unsigned long long* totals;  // [0] paths extended, [1] shadow-queued paths, [2] shadow rays traced,
                                  // [3] extend rays traced (lanes of a pixel sample trace an equal ray once)

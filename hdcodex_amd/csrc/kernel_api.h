@@ -30,7 +30,11 @@ hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_
                        const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                       hipStream_t st);
-hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+// Groups the lanes of a pixel sample with equal shadow rays (shadowQueue);
+// LaunchShadow with shared = true then traces each group once.
+hipError_t LaunchShadowPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                            hipStream_t st);
+hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st);
 // Batch renderer: the same stages over a table of scenes (scene of each
 // path in pb.pathScene; twoLevel when any scene has instances), the waves' raygen / film reading the device wave
@@ -46,6 +50,9 @@ hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathB
                             hipStream_t st);
 hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchShadowPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork,
+                                 unsigned maxBlocks, hipStream_t st);
+// The batch always shares shadow rays (LaunchShadowPairBatch first).
 hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                              unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, const CacheParams& cp, std::uint32_t maxWork,

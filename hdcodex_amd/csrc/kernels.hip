@@ -233,6 +233,7 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
         pb.counters[2 * kCounterStride] = 0U;
         pb.counters[3 * kCounterStride] = 0U;
         pb.counters[(4U + (in ^ 1U)) * kCounterStride] = 0U;
+        pb.counters[6 * kCounterStride] = 0U;
     }
     const std::uint32_t count = pb.counters[in * kCounterStride];
     const bool camera = sharing == kShareCamera && count % 3U == 0U;
@@ -291,75 +292,131 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
 // ---------------------------------------------------------------------------
 constexpr unsigned kPairBlock = 1024;
 
+// Leader / follower decision of one path among the lanes of its pixel
+// sample, from kWords words that must be equal bit for bit (see pairKernel).
+// Returns the follower mask of a path that traces (bit j: pid + 1 + j) in
+// `followers`; true when the path traces.
+template <int kWords>
+__device__ __forceinline__ bool pairLanes(bool live, std::uint32_t pid, const std::uint32_t (&w)[kWords],
+                                          std::uint32_t& followers)
+{
+    const std::uint32_t lane = laneId(), sampleLane = pid % 3U;
+    std::uint32_t back = 0U;  // 0: traced here; 1 or 2: path pid - back traces it
+    std::uint32_t dist = 0U;  // threads back to that path
+    for (std::uint32_t k = 1U; k <= 2U; ++k) {
+        const std::uint32_t qp = __shfl_up(pid, k);
+        const int ql = __shfl_up(static_cast<int>(live), k);
+        bool same = live && lane >= k && ql != 0 && qp < pid && pid - qp <= sampleLane;
+        for (int j = 0; j < kWords; ++j) same = (__shfl_up(w[j], k) == w[j]) && same;
+        if (same && pid - qp > back) {
+            back = pid - qp;
+            dist = k;
+        }
+    }
+    const std::uint32_t leaderBack1 = __shfl_up(back, 1U), leaderBack2 = __shfl_up(back, 2U);
+    if (back != 0U && (dist == 1U ? leaderBack1 : leaderBack2) != 0U) back = 0U;
+    followers = 0U;
+    for (std::uint32_t k = 1U; k <= 2U; ++k) {
+        const std::uint32_t qp = __shfl_down(pid, k);
+        const std::uint32_t qb = __shfl_down(back, k);
+        if (lane + k < 64U && qb != 0U && qp - qb == pid) followers |= 1U << (qp - pid - 1U);
+    }
+    return live && back == 0U;
+}
+
+// Appends pid for threads with `want` to `queue`: one atomic per block of
+// kPairBlock threads. Every thread of the block must call it.
+__device__ __forceinline__ void pairAppend(bool want, std::uint32_t pid, std::uint32_t* queue, std::uint32_t* counter,
+                                           std::uint32_t* waveCount, std::uint32_t* waveBase)
+{
+    constexpr unsigned kWaves = kPairBlock / 64U;
+    const std::uint32_t lane = laneId(), wave = threadIdx.x >> 6;
+    const unsigned long long mask = __ballot(want);
+    if (lane == 0U) waveCount[wave] = static_cast<std::uint32_t>(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0U) {
+        std::uint32_t total = 0U;
+        for (unsigned k = 0; k < kWaves; ++k) total += waveCount[k];
+        std::uint32_t at = total ? atomicAdd(counter, total) : 0U;
+        for (unsigned k = 0; k < kWaves; ++k) {
+            waveBase[k] = at;
+            at += waveCount[k];
+        }
+    }
+    __syncthreads();
+    if (want) queue[waveBase[wave] + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)))] = pid;
+    __syncthreads();
+}
+
 template <class Scenes>
 __global__ __launch_bounds__(kPairBlock) void pairKernel(Scenes scenes, PathBuffers pb, std::uint32_t out)
 {
-    constexpr unsigned kWaves = kPairBlock / 64U;
-    __shared__ std::uint32_t waveCount[kWaves], waveBase[kWaves];
+    __shared__ std::uint32_t waveCount[kPairBlock / 64U], waveBase[kPairBlock / 64U];
     const std::uint32_t count = pb.counters[out * kCounterStride];
     const std::uint32_t* queue = pb.queue[out];
-    std::uint32_t* counter = pb.counters + (4U + out) * kCounterStride;
     const std::uint32_t stride = gridDim.x * blockDim.x;
     const std::uint32_t padded = (count + stride - 1U) / stride * stride;  // every thread reaches the barriers
-    const std::uint32_t lane = laneId(), wave = threadIdx.x >> 6;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < padded; i += stride) {
         const bool live = i < count;
-        std::uint32_t pid = 0U, rng = 0U;
-        float4 o{}, d{};
-        bool opaque = true;
+        std::uint32_t pid = 0U;
+        std::uint32_t w[7] = {0U, 0U, 0U, 0U, 0U, 0U, 0U};  // ray origin, direction, random state (cut-outs only)
         if (live) {
             pid = queue[i];
-            o = pb.rayO[pid];
-            d = pb.rayD[pid];
-            opaque = scenes.Get(pid).opaque != 0U;
-            if (!opaque) rng = pb.rng[pid];
+            const float4 o = pb.rayO[pid], d = pb.rayD[pid];
+            w[0] = __float_as_uint(o.x);
+            w[1] = __float_as_uint(o.y);
+            w[2] = __float_as_uint(o.z);
+            w[3] = __float_as_uint(d.x);
+            w[4] = __float_as_uint(d.y);
+            w[5] = __float_as_uint(d.z);
+            if (!scenes.Get(pid).opaque) w[6] = pb.rng[pid];
         }
-        const std::uint32_t sampleLane = pid % 3U;
-        std::uint32_t back = 0U;  // 0: traced here; 1 or 2: path pid - back traces it
-        std::uint32_t dist = 0U;  // threads back to that path
-        for (std::uint32_t k = 1U; k <= 2U; ++k) {
-            const std::uint32_t qp = __shfl_up(pid, k);
-            const int ql = __shfl_up(static_cast<int>(live), k);
-            const float qox = __shfl_up(o.x, k), qoy = __shfl_up(o.y, k), qoz = __shfl_up(o.z, k);
-            const float qdx = __shfl_up(d.x, k), qdy = __shfl_up(d.y, k), qdz = __shfl_up(d.z, k);
-            const std::uint32_t qr = __shfl_up(rng, k);
-            const std::uint32_t b = pid - qp;
-            const bool same = live && lane >= k && ql != 0 && qp < pid && b <= sampleLane &&
-                              __float_as_uint(qox) == __float_as_uint(o.x) && __float_as_uint(qoy) == __float_as_uint(o.y) &&
-                              __float_as_uint(qoz) == __float_as_uint(o.z) && __float_as_uint(qdx) == __float_as_uint(d.x) &&
-                              __float_as_uint(qdy) == __float_as_uint(d.y) && __float_as_uint(qdz) == __float_as_uint(d.z) &&
-                              (opaque || qr == rng);
-            if (same && b > back) {
-                back = b;
-                dist = k;
-            }
-        }
-        const std::uint32_t leaderBack1 = __shfl_up(back, 1U), leaderBack2 = __shfl_up(back, 2U);
-        if (back != 0U && (dist == 1U ? leaderBack1 : leaderBack2) != 0U) back = 0U;
-        std::uint32_t followers = 0U;  // bit j: path pid + 1 + j takes this path's hit
-        for (std::uint32_t k = 1U; k <= 2U; ++k) {
-            const std::uint32_t qp = __shfl_down(pid, k);
-            const std::uint32_t qb = __shfl_down(back, k);
-            if (lane + k < 64U && qb != 0U && qp - qb == pid) followers |= 1U << (qp - pid - 1U);
-        }
-        const bool traces = live && back == 0U;
+        std::uint32_t followers;
+        const bool traces = pairLanes<7>(live, pid, w, followers);
         if (traces) pb.follow[pid] = followers;
-        const unsigned long long mask = __ballot(traces);
-        if (lane == 0U) waveCount[wave] = static_cast<std::uint32_t>(__popcll(mask));
-        __syncthreads();
-        if (threadIdx.x == 0U) {
-            std::uint32_t total = 0U;
-            for (unsigned k = 0; k < kWaves; ++k) total += waveCount[k];
-            std::uint32_t at = total ? atomicAdd(counter, total) : 0U;
-            for (unsigned k = 0; k < kWaves; ++k) {
-                waveBase[k] = at;
-                at += waveCount[k];
+        pairAppend(traces, pid, pb.extendQueue[out], pb.counters + (4U + out) * kCounterStride, waveCount, waveBase);
+    }
+}
+
+// shadowPair: the same grouping over the paths with shadow rays
+// (shadowQueue), comparing every slot's light, origin, extent and
+// direction. The shadow kernel then traces each group's rays once and
+// applies the visibility to every lane of the group with the lane's own
+// wavelength and contribution (the visibility test itself draws no random
+// numbers). Leaders go to shadowLeaders with their masks in shadowFollow.
+template <class Scenes>
+__global__ __launch_bounds__(kPairBlock) void shadowPairKernel(Scenes scenes, PathBuffers pb)
+{
+    __shared__ std::uint32_t waveCount[kPairBlock / 64U], waveBase[kPairBlock / 64U];
+    const std::uint32_t count = pb.counters[2 * kCounterStride];
+    const std::uint32_t stride = gridDim.x * blockDim.x;
+    const std::uint32_t padded = (count + stride - 1U) / stride * stride;  // every thread reaches the barriers
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < padded; i += stride) {
+        const bool live = i < count;
+        std::uint32_t pid = 0U;
+        std::uint32_t w[8 * kShadowSlots];
+        for (int j = 0; j < 8 * kShadowSlots; ++j) w[j] = 0U;
+        if (live) {
+            pid = pb.shadowQueue[i];
+            for (int k = 0; k < kShadowSlots; ++k) {
+                const std::size_t idx = static_cast<std::size_t>(k) * pb.capacity + pid;
+                const std::uint32_t light = pb.shadowLight[idx];
+                w[8 * k] = light;
+                if (light == 0xffffffffU) break;
+                const float4 o = pb.shadowO[idx], d = pb.shadowD[idx];
+                w[8 * k + 1] = __float_as_uint(o.x);
+                w[8 * k + 2] = __float_as_uint(o.y);
+                w[8 * k + 3] = __float_as_uint(o.z);
+                w[8 * k + 4] = __float_as_uint(o.w);
+                w[8 * k + 5] = __float_as_uint(d.x);
+                w[8 * k + 6] = __float_as_uint(d.y);
+                w[8 * k + 7] = __float_as_uint(d.z);
             }
         }
-        __syncthreads();
-        if (traces)
-            pb.extendQueue[out][waveBase[wave] + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)))] = pid;
-        __syncthreads();
+        std::uint32_t followers;
+        const bool traces = pairLanes<8 * kShadowSlots>(live, pid, w, followers);
+        if (traces) pb.shadowFollow[pid] = followers;
+        pairAppend(traces, pid, pb.shadowLeaders, pb.counters + 6U * kCounterStride, waveCount, waveBase);
     }
 }
 
@@ -1087,7 +1144,7 @@ __global__ __launch_bounds__(kBlock) void walkKernel(Scenes scenes, PathBuffers 
 // :891-912).
 // ---------------------------------------------------------------------------
 template <bool kCount, class Scenes, bool k2L>
-__global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffers pb, TraversalCounters* ctr)
+__global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffers pb, bool shared, TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
     TraversalStack st;
@@ -1097,13 +1154,26 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
     if (threadIdx.x == 0) tracedInBlock = 0U;
     __syncthreads();
     unsigned int traced = 0U;
-    const std::uint32_t count = pb.counters[2 * kCounterStride];
-    if (blockIdx.x == 0 && threadIdx.x == 0) pb.totals[1] += count;
+    const std::uint32_t queued = pb.counters[2 * kCounterStride];
+    if (blockIdx.x == 0 && threadIdx.x == 0) pb.totals[1] += queued;
+    // shared: shadowPairKernel grouped the lanes of a pixel sample with
+    // equal shadow rays; each group's rays are traced once and the
+    // visibility applied to every lane of the group.
+    const std::uint32_t count = shared ? pb.counters[6 * kCounterStride] : queued;
+    const std::uint32_t* queue = shared ? pb.shadowLeaders : pb.shadowQueue;
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        const std::uint32_t pid = pb.shadowQueue[i];
+        const std::uint32_t pid = queue[i];
         const DeviceScene& s = scenes.Get(pid);
-        const float wl = pb.wavelength[pid];
-        float radiance = pb.radiance[pid];
+        const std::uint32_t followers = shared ? pb.shadowFollow[pid] : 0U;
+        std::uint32_t member[3] = {pid, pid, pid};
+        int members = 1;
+        for (std::uint32_t k = 0; k < 2U; ++k)
+            if (followers & (1U << k)) member[members++] = pid + 1U + k;
+        float wls[3], rad[3];
+        for (int m = 0; m < members; ++m) {
+            wls[m] = pb.wavelength[member[m]];
+            rad[m] = pb.radiance[member[m]];
+        }
         for (int k = 0; k < kShadowSlots; ++k) {
             const std::size_t idx = static_cast<std::size_t>(k) * pb.capacity + pid;
             const std::uint32_t light = pb.shadowLight[idx];
@@ -1111,8 +1181,10 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
             const float4 o = pb.shadowO[idx], d = pb.shadowD[idx];
             float tmax = o.w;
             const bool mesh = light == kShadowMeshLight;
-            float visibility = 1.0f;
             bool trace = true;
+            int outcome = 0;  // 1: occluded
+            float occluderDistance = 0.0f;
+            bool falloffSet = false;
             float4 shadow = make_float4(1.0f, -1.0f, -1.0f, 1.0f);
             float4 shadowColor = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (!mesh) {
@@ -1127,6 +1199,7 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
                 ++traced;
                 const RayPre r = prepareRay(o.x, o.y, o.z, d.x, d.y, d.z);
                 const bool falloff = !mesh && shadow.y >= 0.0f && shadow.z > 0.0f;
+                falloffSet = falloff;
                 bool occluded;
                 float hitDistance = 0.0f;
                 if (s.opaque && !falloff) {
@@ -1149,14 +1222,23 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
                         loPrim = h.prim;
                     }
                 }
-                if (occluded) {
+                outcome = occluded ? 1 : 0;
+                occluderDistance = hitDistance;
+            }
+            // Visibility of each lane of the group with its own wavelength, and
+            // its own contribution, in slot order.
+            for (int m = 0; m < members; ++m) {
+                const float wl = wls[m];
+                const float contribution = m == 0 ? d.w : pb.shadowD[static_cast<std::size_t>(k) * pb.capacity + member[m]].w;
+                float visibility = 1.0f;
+                if (trace && outcome == 1) {
                     if (mesh) {
                         visibility = 0.0f;
                     } else {
                         float strength = 1.0f;
-                        if (falloff) {
+                        if (falloffSet) {
                             const float f = fminf(shadow.z, fmaxf(shadow.y, 0.0f));
-                            strength = clampf((shadow.y - hitDistance) / fmaxf(f, 1e-6f), 0.0f, 1.0f);
+                            strength = clampf((shadow.y - occluderDistance) / fmaxf(f, 1e-6f), 0.0f, 1.0f);
                             strength = powf(strength, fmaxf(shadow.w, 0.0f));
                         }
                         const V3 vis = v3(mixf(1.0f, shadowColor.x, strength), mixf(1.0f, shadowColor.y, strength),
@@ -1166,16 +1248,14 @@ __global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffer
                 } else if (!mesh) {
                     visibility = reflectanceSpectrum(v3(1.0f), wl);
                 }
-            } else if (!mesh) {
-                visibility = reflectanceSpectrum(v3(1.0f), wl);
-            }
-            if (mesh) {
-                if (visibility > 0.0f) radiance += d.w;
-            } else {
-                radiance += d.w * visibility;
+                if (mesh) {
+                    if (visibility > 0.0f) rad[m] += contribution;
+                } else {
+                    rad[m] += contribution * visibility;
+                }
             }
         }
-        pb.radiance[pid] = radiance;
+        for (int m = 0; m < members; ++m) pb.radiance[member[m]] = rad[m];
     }
     atomicAdd(&tracedInBlock, traced);
     __syncthreads();
@@ -1414,15 +1494,15 @@ hipError_t LaunchWalkT(const Scenes& s, const PathBuffers& pb, std::uint32_t max
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
-hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                          TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock),
-                           0, st, s, pb, ctr);
+                           0, st, s, pb, shared, ctr);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)),
-                           dim3(kBlock), 0, st, s, pb, ctr);
+                           dim3(kBlock), 0, st, s, pb, shared, ctr);
     return hipGetLastError();
 }
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
@@ -1449,11 +1529,19 @@ hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t
     return s.instanceCount ? LaunchWalkT<SingleScene, true>(SingleScene{s}, pb, maxWork, maxBlocks, st)
                            : LaunchWalkT<SingleScene, false>(SingleScene{s}, pb, maxWork, maxBlocks, st);
 }
-hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+hipError_t LaunchShadowPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                            hipStream_t st)
+{
+    const unsigned blocks = std::max(1U, std::min((maxWork + kPairBlock - 1U) / kPairBlock, maxBlocks));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowPairKernel<SingleScene>), dim3(blocks), dim3(kPairBlock), 0, st,
+                       SingleScene{s}, pb);
+    return hipGetLastError();
+}
+hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st)
 {
-    return s.instanceCount ? LaunchShadowT<SingleScene, true>(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st)
-                           : LaunchShadowT<SingleScene, false>(SingleScene{s}, pb, maxWork, maxBlocks, ctr, st);
+    return s.instanceCount ? LaunchShadowT<SingleScene, true>(SingleScene{s}, pb, shared, maxWork, maxBlocks, ctr, st)
+                           : LaunchShadowT<SingleScene, false>(SingleScene{s}, pb, shared, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
                              int sharing, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
@@ -1483,12 +1571,20 @@ hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBu
     return twoLevel ? LaunchWalkT<SceneTable, true>(t, pb, maxWork, maxBlocks, st)
                     : LaunchWalkT<SceneTable, false>(t, pb, maxWork, maxBlocks, st);
 }
+hipError_t LaunchShadowPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork,
+                                 unsigned maxBlocks, hipStream_t st)
+{
+    const unsigned blocks = std::max(1U, std::min((maxWork + kPairBlock - 1U) / kPairBlock, maxBlocks));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowPairKernel<SceneTable>), dim3(blocks), dim3(kPairBlock), 0, st,
+                       SceneTable{table, pb.pathScene}, pb);
+    return hipGetLastError();
+}
 hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                              unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchShadowT<SceneTable, true>(t, pb, maxWork, maxBlocks, ctr, st)
-                    : LaunchShadowT<SceneTable, false>(t, pb, maxWork, maxBlocks, ctr, st);
+    return twoLevel ? LaunchShadowT<SceneTable, true>(t, pb, true, maxWork, maxBlocks, ctr, st)
+                    : LaunchShadowT<SceneTable, false>(t, pb, true, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, const CacheParams& cp, std::uint32_t maxWork,
                              unsigned maxBlocks, hipStream_t st)

@@ -514,7 +514,7 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
 void PathPool::EnsurePaths(std::uint32_t paths)
 {
     if (!counters.bytes()) {
-        counters.Allocate(6 * kCounterStride * sizeof(std::uint32_t));
+        counters.Allocate(7 * kCounterStride * sizeof(std::uint32_t));
         totals.Allocate(64);
         Check(hipMemset(totals.as<void>(), 0, 64), "memset totals");
     }
@@ -544,6 +544,8 @@ void PathPool::EnsurePaths(std::uint32_t paths)
     extendQueue0.Allocate(n * 4);
     extendQueue1.Allocate(n * 4);
     follow.Allocate(n * 4);
+    shadowLeaders.Allocate(n * 4);
+    shadowFollow.Allocate(n * 4);
     capacity = paths;
 }
 
@@ -576,6 +578,8 @@ PathBuffers PathPool::Paths()
     pb.extendQueue[0] = extendQueue0.as<std::uint32_t>();
     pb.extendQueue[1] = extendQueue1.as<std::uint32_t>();
     pb.follow = follow.as<std::uint32_t>();
+    pb.shadowLeaders = shadowLeaders.as<std::uint32_t>();
+    pb.shadowFollow = shadowFollow.as<std::uint32_t>();
     pb.capacity = capacity;
     return pb;
 }

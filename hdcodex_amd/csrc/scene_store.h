@@ -48,7 +48,7 @@ void UploadScene(SceneStore* s, const hdcx_scene_view* view, hipStream_t st);
 struct PathPool {
     DeviceBuffer rayO, rayD, hit, rng, wavelength, throughput, radiance, flags, mis, medium, shadowO, shadowD,
         shadowLight, queue0, queue1, shadowQueue, lambda0, spread, pathScene, walk, walkQueue, extendQueue0, extendQueue1,
-        follow, counters, totals;
+        follow, shadowLeaders, shadowFollow, counters, totals;
     std::uint32_t capacity = 0;
 
     // Room for `paths` lane paths (grows only).

@@ -348,7 +348,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                     });
                     Check(LaunchConsumeQueries(cp, st), "rc queries");
                 }
-                t->Timed(HDCX_KERNEL_SHADOW, [&] { return LaunchShadow(t->scene, pb, paths, t->maxBlocks, shadowCtr, st); });
+                if (share) t->Timed(HDCX_KERNEL_PAIR, [&] { return LaunchShadowPair(t->scene, pb, paths, t->maxBlocks, st); });
+                t->Timed(HDCX_KERNEL_SHADOW,
+                         [&] { return LaunchShadow(t->scene, pb, share, paths, t->maxBlocks, shadowCtr, st); });
             }
             if (cache) {
                 const std::uint32_t records = cp.recordSlots * 3U * kRecordVertices;
