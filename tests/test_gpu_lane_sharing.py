@@ -6,10 +6,12 @@ numbers, so their later rays often stay equal too. The extend launch after
 raygen traces the camera ray once for the three lanes; at later bounces
 the pair kernel queues only the lanes whose next ray differs (bit for bit,
 with the random state in a scene with cut-outs) from an earlier lane of
-the sample, and extend writes each traced hit to the lanes that share it.
-Images must equal, bit for bit, those rendered with every lane tracing its
-own rays (instrumentation bit 3); the extend kernel must trace at least
-two rays fewer per pixel sample.
+the sample, and extend writes each traced hit to the lanes that share it;
+the shadow rays are grouped the same way and each group's visibility is
+applied to every lane with its own wavelength and contribution. Images
+must equal, bit for bit, those rendered with every lane tracing its own
+rays (instrumentation bit 3); the extend kernel must trace at least two
+rays fewer per pixel sample.
 """
 
 import ctypes
