@@ -78,19 +78,90 @@ def algorithmic_bytes(stats: dict) -> dict:
     return out
 
 
-def cpu_baseline(scene, cam, threads: int) -> dict:
-    """FP32 CPU restatement timed on a bounded sample of the frame."""
+# Bytes the shade kernel moves per path vertex (single-level scene, no
+# radiance cache), from its loads and stores:
+SHADE_VERTEX_READ = 96    # queue entry 4, hit 16, ray origin 16 + direction 16, rng 4, wavelength 4,
+                          # throughput 4, radiance 4, flags 4, MIS pdfs 8, medium 16
+SHADE_VERTEX_WRITE = 12   # the three shadow-slot light ids cleared
+SHADE_MISS_WRITE = 4      # radiance (environment on a miss)
+SHADE_HIT_READ = 136      # three vertex indices 12, positions 3 x 16, corner normals 3 x 16, UVs 3 x 8,
+                          # triangle material id 4
+SHADE_HIT_WRITE = 12      # rng, throughput, radiance
+SHADE_KEPT_WRITE = 64     # next ray origin + direction 32, flags 4, MIS pdfs 8, medium 16, queue entry 4
+SHADE_SHADOW_WRITE = 40   # shadow slot: origin + tmax 16, direction + contribution 16, light 4; queue entry 4
+
+
+def shade_bytes(timed: dict, counted: dict) -> dict:
+    """Algorithmic bytes of the shade launches of the timed steps: the bytes
+    unique to each path vertex (its path state, hit record, triangle
+    attributes and the state, queue and shadow-ray records it writes).
+    Scene-wide tables that every vertex reads (material records, light
+    records, dome texels and importance tables, textures: a few MB in
+    L2/MALL) are not counted per vertex. The hit fraction comes from the
+    counting pass over the same frame."""
+    items = timed["shade"]["items"]
+    hit_frac = counted["extend_hits_counted"] / max(counted["shade"]["items"], 1)
+    hits = items * hit_frac
+    kept = max(items - timed["raygen"]["items"], 0)  # every later vertex was kept by the one before
+    queued = timed["shadow_queued_paths"]
+    total = (items * (SHADE_VERTEX_READ + SHADE_VERTEX_WRITE) + (items - hits) * SHADE_MISS_WRITE
+             + hits * (SHADE_HIT_READ + SHADE_HIT_WRITE) + kept * SHADE_KEPT_WRITE + queued * SHADE_SHADOW_WRITE)
+    return {"bytes": total, "bytes_per_vertex": total / max(items, 1), "vertices": items, "hit_fraction": hit_frac,
+            "shadow_queued_paths": queued}
+
+
+def cpu_share() -> dict:
+    """Host CPUs this process may use: the scheduler affinity, capped by a
+    cgroup CPU quota and by OMP_NUM_THREADS when either is set (the share of
+    the host a job is granted), with the host's model and logical CPU count."""
     import oracle
 
+    info = oracle.host_cpu()
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    share = affinity
+    quota = None
+    try:
+        q, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+            share = min(share, max(1, int(quota)))
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        share = min(share, int(omp))
+    return {"threads": share, "model": info["model"], "logical_cpus": info["logical_cpus"],
+            "affinity": affinity, "cgroup_quota": quota, "omp_num_threads": omp}
+
+
+def cpu_baseline(scene, cam, share: dict, target_s: float = 15.0) -> dict:
+    """FP32 CPU restatement (the -O3 build for this host's x86-64 level, SAH
+    BVH) timed on the host's CPU share over the heightfield frame: a pilot on
+    every 60th row picks the row stride so that the timed sample takes about
+    target_s seconds (every row when that fits), all columns, all 16 spp."""
+    import oracle
+
+    threads = share["threads"]
     o = oracle.OracleScene(scene.view)
-    rows = np.arange(0, HEIGHT, 8, dtype=np.uint32)
+    pilot = np.arange(0, HEIGHT, 60, dtype=np.uint32)
+    t0 = time.perf_counter()
+    o.render(cam, WIDTH, HEIGHT, 0, BOUNCES, SPP, precision=32, rows=pilot, col_step=1, threads=threads)
+    rate = len(pilot) * WIDTH * SPP / (time.perf_counter() - t0)
+    step = 1
+    while step < 60 and (HEIGHT // step) * WIDTH * SPP / rate > target_s:
+        step += 1
+    rows = np.arange(0, HEIGHT, step, dtype=np.uint32)
     t0 = time.perf_counter()
     o.render(cam, WIDTH, HEIGHT, 0, BOUNCES, SPP, precision=32, rows=rows, col_step=1, threads=threads)
     dt = time.perf_counter() - t0
     paths = len(rows) * WIDTH * SPP
     return {"value": paths / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
-            "sample": f"FP32 restatement (oracle/oracle.cpp), rows 0,8,..,{HEIGHT - 8} x {WIDTH} columns x {SPP} spp "
-                      f"= {paths} paths in {dt:.2f} s"}
+            "sample": f"FP32 restatement (oracle/oracle.cpp, {oracle.BUILD_USED}, binned-SAH BVH), rows 0,{step},.."
+                      f" x {WIDTH} columns x {SPP} spp = {paths} paths in {dt:.2f} s on {threads} threads",
+            "host": {k: share[k] for k in ("model", "logical_cpus", "affinity", "cgroup_quota", "omp_num_threads")}}
 
 
 def parity(scene, cam, frame: np.ndarray) -> dict:
@@ -100,7 +171,7 @@ def parity(scene, cam, frame: np.ndarray) -> dict:
 
     rows = np.arange(0, HEIGHT, 16, dtype=np.uint32)
     o = oracle.OracleScene(scene.view)
-    threads = min(os.cpu_count() or 8, 16)
+    threads = cpu_share()["threads"]
     gpu = frame[rows.astype(np.int64)][:, ::4, :3].astype(np.float64)
     out = {"pixels": int(gpu.shape[0] * gpu.shape[1]), "sample": "rows 0,16,.. x columns 0,4,.. at the bench's seeds"}
     for precision, key in ((64, "rel_mse_vs_fp64_oracle"), (32, "rel_mse_vs_fp32_restatement")):
@@ -466,8 +537,10 @@ def main() -> int:
         return copy_calibration()
     if args.batch_only:
         import hdcodex_amd as hx
+        import oracle
 
-        print(json.dumps({"batch": batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph,
+        oracle.use_fastest_build()
+        print(json.dumps({"batch": batch_workload(hx, cpu_share()["threads"], graph=not args.no_graph,
                                                   cache=not args.no_batch_cache)}),
               flush=True)
         return 0
@@ -482,7 +555,9 @@ def main() -> int:
         return 2
 
     import hdcodex_amd as hx
+    import oracle
 
+    oracle.use_fastest_build()  # the -O3 CPU build for this host (parity and baseline legs)
     log("building scene")
     scene = hx.ProceduralScene("heightfield", 0, GRID, DOME)
     cam = scene.camera(WIDTH, HEIGHT)
@@ -537,7 +612,8 @@ def main() -> int:
 
     kernels = {}
     for k in hx.KERNELS:
-        kernels[k] = {"ms_per_step": timed[k]["ms"] / args.steps, "launches_per_step": timed[k]["launches"] / args.steps}
+        kernels[k] = {"ms_per_step": timed[k]["ms"] / args.steps, "launches_per_step": timed[k]["launches"] / args.steps,
+                      "items_per_step": timed[k]["items"] / args.steps}
         traffic = measured_traffic(k)
         if traffic and timed[k]["launches"]:
             # Measured HBM bytes per launch (rocprofv3 counters) over this run's average launch time.
@@ -545,22 +621,33 @@ def main() -> int:
             kernels[k]["hbm_GBs"] = traffic / avg_s / 1e9
             kernels[k]["hbm_frac"] = kernels[k]["hbm_GBs"] / HBM_PEAK_GBS
     dominant = max(("extend", "shade", "shadow"), key=lambda k: timed[k]["ms"])
+    # Rooflines of the three path kernels: algorithmic bytes of the timed
+    # launches over their HIP-event time.
     roof = {}
     for k in ("extend", "shadow"):
         if k in alg and timed[k]["ms"] > 0:
             rays = extend_rays if k == "extend" else shadow_rays
-            achieved = rays * alg[k]["bytes_per_ray"] / (timed[k]["ms"] * 1e-3) / 1e9
-            roof[k] = {"achieved_GBs": achieved, "frac": achieved / HBM_PEAK_GBS, **alg[k],
-                       "avg_launch_ms": timed[k]["ms"] / max(timed[k]["launches"], 1),
-                       "rays_per_launch": rays / max(timed[k]["launches"], 1)}
-    rk = "extend" if dominant not in roof else dominant
+            roof[k] = {**alg[k], "alg_bytes": rays * alg[k]["bytes_per_ray"], "rays_per_launch":
+                       rays / max(timed[k]["launches"], 1)}
+    if timed["shade"]["ms"] > 0:
+        roof["shade"] = shade_bytes(timed, counted)
+        roof["shade"]["alg_bytes"] = roof["shade"].pop("bytes")
+    for k, r in roof.items():
+        launches = max(timed[k]["launches"], 1)
+        r["achieved_GBs"] = r["alg_bytes"] / (timed[k]["ms"] * 1e-3) / 1e9
+        r["frac"] = r["achieved_GBs"] / HBM_PEAK_GBS
+        r["avg_launch_ms"] = timed[k]["ms"] / launches
+        r["alg_bytes_per_launch"] = r["alg_bytes"] / launches
     roofline = None
-    if rk in roof:
-        r = roof[rk]
-        roofline = {"kernel": rk, "bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": r["frac"], "traffic": measured_traffic(rk),
-                    "alg_bytes_per_launch": r["bytes_per_ray"] * r["rays_per_launch"],
-                    "avg_launch_ms": r["avg_launch_ms"]}
+    if dominant in roof:
+        r = roof[dominant]
+        roofline = {"kernel": dominant, "bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": r["frac"], "traffic": measured_traffic(dominant),
+                    "alg_bytes_per_launch": r["alg_bytes_per_launch"], "avg_launch_ms": r["avg_launch_ms"],
+                    "others": {k: {"frac": v["frac"], "achieved": v["achieved_GBs"],
+                                   "alg_bytes_per_launch": v["alg_bytes_per_launch"],
+                                   "avg_launch_ms": v["avg_launch_ms"], "traffic": measured_traffic(k)}
+                               for k, v in roof.items() if k != dominant}}
 
     result = {
         "metric": METRIC,
@@ -589,7 +676,7 @@ def main() -> int:
                 "other_nodes": other_bvh["node_count"]},
         "dominant_kernel": dominant,
         "kernels": kernels,
-        "traversal": roof,
+        "kernel_rooflines": roof,
         "roofline": roofline,
     }
     if not args.no_cache:
@@ -600,18 +687,17 @@ def main() -> int:
         result["parity"] = parity(scene, cam, frame)
         result["parity_rel_mse_vs_fp64_oracle"] = result["parity"]["rel_mse_vs_fp64_oracle"]
     log("cornell workload")
-    result["cornell"] = cornell_workload(hx, min(os.cpu_count() or 1, 16))
+    result["cornell"] = cornell_workload(hx, cpu_share()["threads"])
     if not args.no_instanced:
         log("instanced workload")
         result["instanced"] = instanced_workload(hx)
     if not args.no_batch:
         log("batch workload")
-        result["batch"] = batch_workload(hx, min(os.cpu_count() or 1, 16), graph=not args.no_graph,
+        result["batch"] = batch_workload(hx, cpu_share()["threads"], graph=not args.no_graph,
                                          cache=not args.no_batch_cache)
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        threads = min(os.cpu_count() or 1, 16)
-        result["cpu_baseline"] = cpu_baseline(scene, cam, threads)
+        result["cpu_baseline"] = cpu_baseline(scene, cam, cpu_share())
     tracer.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -84,6 +84,7 @@ class KernelStats(ctypes.Structure):
         ("nodes_visited", c_uint64), ("triangles_tested", c_uint64), ("extend_rays_counted", c_uint64),
         ("shadow_rays_counted", c_uint64), ("shadow_nodes_visited", c_uint64), ("shadow_triangles_tested", c_uint64),
         ("node_bytes", c_uint64), ("triangle_bytes", c_uint64), ("bvh_nodes", c_uint64), ("bvh_triangles", c_uint64),
+        ("extend_hits_counted", c_uint64), ("shadow_queued_paths", c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -5,6 +5,12 @@
 * ``oracle/liboracle.so`` — the CPU oracle (test infrastructure only), built
   with ``-ffp-contract=off`` so that its FP32 arithmetic is the IEEE sequence
   the restatement spells out.
+* ``oracle/liboracle_x86-64-v3.so`` and ``oracle/liboracle_x86-64-v4.so`` —
+  the same source at ``-O3`` for the AVX2 and AVX-512 levels of x86-64: the
+  CPU baseline loads the highest level the host supports (the library is
+  built here, before the run, so ``-march=native`` would describe this
+  machine rather than the host that runs the baseline). Contraction stays
+  off, so results equal the checker's.
 
 Usage: ``python -m hdcodex_amd.build [--lib] [--oracle]`` (default: both).
 """
@@ -72,13 +78,26 @@ def build_library() -> Path:
     return LIB
 
 
+ORACLE_LEVELS = ("x86-64-v3", "x86-64-v4")
+
+
+def oracle_variant(level: str) -> Path:
+    return ORACLE_DIR / f"liboracle_{level}.so"
+
+
 def build_oracle() -> Path:
-    tmp = ORACLE_LIB.with_suffix(".so.tmp")
-    _run([
-        "g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
-        "-I", str(ROOT / "include"), str(ORACLE_DIR / "oracle.cpp"), "-o", str(tmp),
-    ])
-    os.replace(tmp, ORACLE_LIB)
+    jobs = [(ORACLE_LIB, ["-O2"])] + [(oracle_variant(level), ["-O3", f"-march={level}", "-mtune=generic"])
+                                        for level in ORACLE_LEVELS]
+
+    def one(job):
+        out, flags = job
+        tmp = out.with_suffix(".so.tmp")
+        _run(["g++", *flags, "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
+              "-I", str(ROOT / "include"), str(ORACLE_DIR / "oracle.cpp"), "-o", str(tmp)])
+        os.replace(tmp, out)
+
+    with futures.ThreadPoolExecutor(max_workers=len(jobs)) as pool:
+        list(pool.map(one, jobs))
     return ORACLE_LIB
 
 

@@ -213,6 +213,7 @@ struct TraversalCounters {
     unsigned long long nodes;
     unsigned long long triangles;
     unsigned long long rays;
+    unsigned long long hits;  // extend: paths given a surface hit (followers included)
 };
 
 // ---------------------------------------------------------------------------

@@ -263,6 +263,9 @@ __global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffer
             found = traceStochastic<kCount, k2L>(s, r, rng, h, st, ctr);
         }
         const float4 hit = make_float4(h.t, __uint_as_float(found ? h.prim : 0xffffffffU), h.u, h.v);
+        if constexpr (kCount) {
+            if (ctr && found) atomicAdd(&ctr->hits, 1ULL + static_cast<unsigned long long>(__popc(followers & 3U)));
+        }
         pb.hit[pid] = hit;
         if (!s.opaque) pb.rng[pid] = rng;
         for (std::uint32_t k = 0; k < 2U; ++k) {

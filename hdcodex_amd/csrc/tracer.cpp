@@ -819,6 +819,8 @@ int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats)
             s.shadow_nodes_visited = ctr[1].nodes;
             s.shadow_triangles_tested = ctr[1].triangles;
             s.extend_rays_counted = (tracer->instrumentation & 2U) ? totals[3] : 0;
+            s.extend_hits_counted = (tracer->instrumentation & 2U) ? ctr[0].hits : 0;
+            s.shadow_queued_paths = totals[1];
             s.shadow_rays_counted = (tracer->instrumentation & 2U) ? totals[2] : 0;
         }
         *out_stats = s;

@@ -435,6 +435,10 @@ typedef struct hdcx_kernel_stats {
     uint64_t triangle_bytes;                /* bytes per leaf triangle record */
     uint64_t bvh_nodes;
     uint64_t bvh_triangles;
+    uint64_t extend_hits_counted;           /* paths whose closest-hit query found a surface
+                                               (shade vertices that are not misses); counted
+                                               only when counting is on */
+    uint64_t shadow_queued_paths;           /* paths that shade gave at least one shadow ray */
 } hdcx_kernel_stats;
 
 /* flags bit 0: time every kernel launch with HIP events on the launch

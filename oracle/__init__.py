@@ -17,16 +17,57 @@ from pathlib import Path
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+# -O3 builds of the same source per x86-64 level, highest first, with the
+# CPU flags each level needs (the CPU baseline loads the best one the host
+# runs; results equal the default build's: contraction is off in all).
+LEVELS = (("x86-64-v4", ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")),
+          ("x86-64-v3", ("avx2", "fma", "bmi2", "movbe")))
 _lib = None
+_lib_path = LIB_PATH
+BUILD_USED = "g++ -O2 -ffp-contract=off"
+
+
+def host_cpu() -> dict:
+    """Model name, logical CPUs and flags of this host (from /proc/cpuinfo)."""
+    model, flags, count = "unknown", set(), 0
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            key, _, value = line.partition(":")
+            key = key.strip()
+            if key == "processor":
+                count += 1
+            elif key == "model name" and model == "unknown":
+                model = value.strip()
+            elif key == "flags" and not flags:
+                flags = set(value.split())
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": count, "flags": flags}
+
+
+def use_fastest_build() -> str:
+    """Switches this process to the -O3 build of the highest x86-64 level the
+    host supports (call before the first OracleScene); returns its name."""
+    global _lib, _lib_path, BUILD_USED
+    flags = host_cpu()["flags"]
+    for level, needed in LEVELS:
+        path = LIB_PATH.parent / f"liboracle_{level}.so"
+        if path.exists() and all(f in flags for f in needed):
+            if _lib is not None and _lib_path != path:
+                raise RuntimeError("the oracle library is already loaded")
+            _lib_path = path
+            BUILD_USED = f"g++ -O3 -march={level} -ffp-contract=off"
+            break
+    return BUILD_USED
 
 
 def library() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
-    if not LIB_PATH.exists():
-        raise RuntimeError(f"{LIB_PATH} missing: run `python -m hdcodex_amd.build --oracle`")
-    lib = ctypes.CDLL(str(LIB_PATH))
+    if not _lib_path.exists():
+        raise RuntimeError(f"{_lib_path} missing: run `python -m hdcodex_amd.build --oracle`")
+    lib = ctypes.CDLL(str(_lib_path))
     lib.oracle_last_error.restype = ctypes.c_char_p
     lib.oracle_scene_create.argtypes = [c_void_p, POINTER(c_void_p)]
     lib.oracle_scene_destroy.argtypes = [c_void_p]

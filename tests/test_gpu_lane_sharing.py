@@ -72,6 +72,12 @@ def test_shared_rays_equal_per_lane_rays(kind, args, cutout):
     samples = w * h * spp
     print(kind, "cutout" if cutout else "opaque", "extend rays", ss["extend"]["items"], ls["extend"]["items"])
     assert ls["extend"]["items"] - ss["extend"]["items"] >= 2 * samples
+    # Shadow rays: each group of lanes with equal shadow rays traces them once
+    # (rays[SHADOW] counts traced shadow rays), and every lane still gets its
+    # visibility (the images above are equal).
+    print(kind, "shadow rays", ss["shadow"]["items"], ls["shadow"]["items"])
+    assert ss["shadow_queued_paths"] == ls["shadow_queued_paths"] > 0
+    assert ss["shadow"]["items"] < ls["shadow"]["items"]
     assert ls["pair"]["launches"] == 0 and ss["pair"]["launches"] > 0
     assert ss["shade"]["items"] == ls["shade"]["items"]
     del keep
