@@ -444,6 +444,154 @@ ProceduralScene MakeFurnace()
     return out;
 }
 
+// ---- Irradiance cases: one white Lambert receiver (a 40 x 40 quad at
+// y = 0, flat albedo 1, no specular weight) and one light whose irradiance
+// at the quad's centre has a closed form, seen by a narrow camera 10 units
+// above looking straight down (a 1-pixel image covers 2 mm of the quad). Cases (param):
+//   0 untextured white dome (intensity 1)          1 rect 1 x 1 at height 2, facing down
+//   2 disk of radius 1 at height 3, facing down    3 sphere of radius 0.5 centred at height 3
+//   4 distant, delta (angle 0), 20 deg off the normal
+//   5 distant, 30 deg cone along the normal        6 case 1 with a 40 deg shaping cone
+//   7 rect at height 2 moved 2 units sideways (the centre is 45 deg off its
+//     axis) with a 30 deg shaping cone             8 no lights: fallback sun and sky
+//   9 the same with a Z-up camera (receiver in the z = 0 plane)
+//  10 glass slab 0.5 thick (IOR 1.5, transmission colour 0.6 at depth 1)
+//     between y = 0 and y = -0.5 under the white dome
+//  11 the slab with a colourless medium (transmission colour 1)
+//  12 a closed icosphere of radius 1 at the origin (level 5), white dome ----
+ProceduralScene MakeIrradiance(std::uint32_t which)
+{
+    ProceduralScene out;
+    SceneSnapshot& s = out.snapshot;
+    s.revision = 1;
+    const bool zUp = which == 9;
+    SceneMaterial white;
+    white.id = "/irradiance/white";
+    white.baseColor = {1.0F, 1.0F, 1.0F};
+    white.specularWeight = 0.0F;
+    if (which >= 10 && which <= 11) {
+        SceneMesh slab;
+        slab.id = "/irradiance/slab";
+        const double top[4][3] = {{-20, 0, 20}, {20, 0, 20}, {20, 0, -20}, {-20, 0, -20}};
+        const double bottom[4][3] = {{-20, -0.5, -20}, {20, -0.5, -20}, {20, -0.5, 20}, {-20, -0.5, 20}};
+        AddQuad(slab, top);
+        AddQuad(slab, bottom);
+        slab.materialId = "/irradiance/glass";
+        s.meshes.push_back(slab);
+        SceneMaterial glass;
+        glass.id = "/irradiance/glass";
+        glass.baseColor = {1.0F, 1.0F, 1.0F};
+        glass.transmission = 1.0F;
+        glass.transmissionColor = which == 10 ? std::array<float, 3>{0.6F, 0.6F, 0.6F}
+                                              : std::array<float, 3>{1.0F, 1.0F, 1.0F};
+        glass.transmissionDepth = 1.0F;
+        glass.indexOfRefraction = 1.5F;
+        glass.specularWeight = 0.0F;
+        glass.roughness = 0.02F;
+        glass.roughnessV = 0.02F;
+        s.materials.push_back(glass);
+    } else if (which == 12) {
+        SceneMesh ball;
+        ball.id = "/irradiance/ball";
+        const double c[3] = {0.0, 0.0, 0.0};
+        AppendIcosphere(ball, 5, c, 1.0);
+        ball.materialId = "/irradiance/white";
+        s.meshes.push_back(ball);
+        s.materials.push_back(white);
+    } else {
+        SceneMesh quad;
+        quad.id = "/irradiance/receiver";
+        if (zUp) {
+            const double q[4][3] = {{-20, -20, 0}, {20, -20, 0}, {20, 20, 0}, {-20, 20, 0}};
+            AddQuad(quad, q);
+        } else {
+            const double q[4][3] = {{-20, 0, 20}, {20, 0, 20}, {20, 0, -20}, {-20, 0, -20}};
+            AddQuad(quad, q);
+        }
+        quad.materialId = "/irradiance/white";
+        s.meshes.push_back(quad);
+        s.materials.push_back(white);
+    }
+    SceneLight l;
+    l.id = "/irradiance/light";
+    l.basisX = {1.0F, 0.0F, 0.0F};
+    l.basisY = {0.0F, 0.0F, 1.0F};
+    l.basisZ = {0.0F, 1.0F, 0.0F};  // emits along -basisZ: straight down
+    switch (which) {
+    case 0: case 10: case 11: case 12:
+        l.type = SceneLightType::Dome;
+        l.basisY = {0.0F, 1.0F, 0.0F};
+        l.basisZ = {0.0F, 0.0F, 1.0F};
+        break;
+    case 1: case 6: case 7:
+        l.type = SceneLightType::Rect;
+        l.intensity = 5.0F;
+        l.position = {which == 7 ? 2.0F : 0.0F, 2.0F, 0.0F};
+        l.axisU = {1.0F, 0.0F, 0.0F};
+        l.axisV = {0.0F, 0.0F, 1.0F};
+        l.width = l.height = 1.0F;
+        l.area = 1.0F;
+        if (which == 6) l.shapingConeAngle = 40.0F;
+        if (which == 7) l.shapingConeAngle = 30.0F;
+        break;
+    case 2:
+        l.type = SceneLightType::Disk;
+        l.intensity = 4.0F;
+        l.position = {0.0F, 3.0F, 0.0F};
+        l.axisU = {1.0F, 0.0F, 0.0F};
+        l.axisV = {0.0F, 0.0F, 1.0F};
+        l.radius = 1.0F;
+        l.area = static_cast<float>(kPi);
+        break;
+    case 3:
+        l.type = SceneLightType::Sphere;
+        l.intensity = 6.0F;
+        l.position = {0.0F, 3.0F, 0.0F};
+        l.radius = 0.5F;
+        l.area = static_cast<float>(4.0 * kPi * 0.25);
+        break;
+    case 4: {
+        l.type = SceneLightType::Distant;
+        l.intensity = 2.0F;
+        l.angle = 0.0F;
+        const double a = 20.0 * kPi / 180.0;  // direction toward the light: 20 deg from +y in the xy plane
+        l.basisZ = {static_cast<float>(std::sin(a)), static_cast<float>(std::cos(a)), 0.0F};
+        l.basisX = {static_cast<float>(std::cos(a)), static_cast<float>(-std::sin(a)), 0.0F};
+        break;
+    }
+    case 5:
+        l.type = SceneLightType::Distant;
+        l.intensity = 2.0F;
+        l.angle = 60.0F;  // full cone angle: 30 deg half angle
+        l.basisX = {1.0F, 0.0F, 0.0F};
+        l.basisY = {0.0F, 0.0F, -1.0F};
+        l.basisZ = {0.0F, 1.0F, 0.0F};
+        break;
+    default:
+        break;
+    }
+    if (which != 8 && which != 9) s.lights.push_back(l);
+    SortSnapshot(s);
+    out.explicitCamera = true;
+    if (zUp) {
+        // Looking straight down -z; the vertical vector has the larger z
+        // component, which makes the camera Z-up for the fallback sky
+        // (vulkan_path_tracer.cpp:3237-3241).
+        out.camera.origin = {0.0F, 0.0F, 10.0F};
+        out.camera.lowerLeft = {-0.0001F, -0.0001F, 9.0F - 0.000105F};
+        out.camera.horizontal = {0.0002F, 0.0F, 0.0F};
+        out.camera.vertical = {0.0F, 0.0002F, 0.00021F};
+    } else {
+        // Looking straight down -y; the vertical vector has the larger y
+        // component, so the camera is Y-up for the fallback sky.
+        out.camera.origin = {0.0F, 10.0F, 0.0F};
+        out.camera.lowerLeft = {-0.0001F, 9.0F - 0.000105F, 0.0001F};
+        out.camera.horizontal = {0.0002F, 0.0F, 0.0F};
+        out.camera.vertical = {0.0F, 0.00021F, -0.0002F};
+    }
+    return out;
+}
+
 } // namespace
 
 PathTracerCamera ProceduralScene::Camera(std::uint32_t width, std::uint32_t height) const
@@ -478,6 +626,7 @@ ProceduralScene MakeProceduralScene(const std::string& name, std::uint64_t seed,
     if (name == "triangle") return MakeTriangle();
     if (name == "materials") return MakeMaterialShowcase(seed);
     if (name == "furnace") return MakeFurnace();
+    if (name == "irradiance") return MakeIrradiance(param);
     if (name == "mixed") return MakeMixedScene(seed, param == 0 ? 20000 : param, domeWidth == 0 ? 256 : domeWidth, 8);
     if (name == "instanced") return MakeInstancedScene(seed, param == 0 ? 10000 : param, 5, domeWidth == 0 ? 256 : domeWidth, 64);
     if (name == "instanced_small") return MakeInstancedScene(seed, param == 0 ? 50 : param, 3, domeWidth == 0 ? 64 : domeWidth, 8);

@@ -203,7 +203,11 @@ int hdcx_trace_occluded(hdcx_tracer* tracer, const hdcx_ray* rays,
  * "heightfield" (param = grid vertices per side, e.g. 1024 -> 2,093,058
  * triangles; GGX dielectric over Burley; HDR lat-long dome of
  * param_dome_width x param_dome_width/2 texels), "triangle" (the
- * reference's one-triangle test scene). */
+ * reference's one-triangle test scene), "materials" (every closure lobe,
+ * light type and texture path), "furnace", "mixed", "mixed_lights",
+ * "instanced", "instanced_small", and "irradiance" (param = case: a white
+ * Lambert receiver under one light whose irradiance has a closed form, a
+ * glass slab, or a closed sphere; listed in procedural.cpp). */
 int hdcx_scene_create_procedural(const char* name, uint64_t seed,
                                  uint32_t param, uint32_t param_dome_width,
                                  hdcx_scene** out_scene);

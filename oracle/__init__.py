@@ -107,6 +107,18 @@ def library() -> ctypes.CDLL:
     lib.oracle_lat_long_direction.argtypes = [c_double, c_double, c_void_p]
     lib.oracle_sample_dome.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), c_void_p, c_void_p]
     lib.oracle_sample_dome.restype = c_double
+    lib.oracle_transmission.argtypes = [c_void_p, c_void_p, c_int, c_double, c_int, POINTER(c_uint32), c_void_p]
+    lib.oracle_transmission.restype = c_int
+    lib.oracle_dome_pdf.argtypes = [c_void_p, c_uint32, c_void_p]
+    lib.oracle_dome_pdf.restype = c_double
+    lib.oracle_dome_radiance.argtypes = [c_void_p, c_uint32, c_void_p, c_void_p]
+    lib.oracle_texture.argtypes = [c_void_p, c_uint32, c_double, c_double, c_void_p]
+    lib.oracle_material_texture.argtypes = [c_void_p, c_uint32, c_double, c_double, c_void_p, c_void_p]
+    lib.oracle_normal_map.argtypes = [c_void_p, c_uint32, c_int, c_double, c_double, c_double, c_void_p, c_void_p,
+                                      c_void_p, c_void_p]
+    lib.oracle_random_walk.argtypes = [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, POINTER(c_uint32),
+                                       c_void_p, c_void_p, c_void_p]
+    lib.oracle_random_walk.restype = c_int
     _lib = lib
     return lib
 
@@ -164,6 +176,43 @@ class OracleScene:
         pdf_eval = np.zeros(1)
         pdf = library().oracle_sample_dome(self._h, light, ctypes.byref(st), d.ctypes.data, pdf_eval.ctypes.data)
         return d, pdf, float(pdf_eval[0]), st.value
+
+    def dome_pdf(self, light: int, d) -> float:
+        a = _vec(d)
+        return float(library().oracle_dome_pdf(self._h, light, a.ctypes.data))
+
+    def dome_radiance(self, light: int, d) -> np.ndarray:
+        a = _vec(d)
+        out = np.zeros(3)
+        library().oracle_dome_radiance(self._h, light, a.ctypes.data, out.ctypes.data)
+        return out
+
+    def texture(self, index: int, u: float, v: float) -> np.ndarray:
+        out = np.zeros(4)
+        library().oracle_texture(self._h, index, u, v, out.ctypes.data)
+        return out
+
+    def material_texture(self, slot: int, u: float, v: float, fallback=(0.0, 0.0, 0.0, 0.0)) -> np.ndarray:
+        fb = _vec(fallback)
+        out = np.zeros(4)
+        library().oracle_material_texture(self._h, slot, u, v, fb.ctypes.data, out.ctypes.data)
+        return out
+
+    def normal_map(self, texture: int, flip_y: bool, scale: float, u: float, v: float, gn, positions, uvs) -> np.ndarray:
+        g, p, t = _vec(gn), _vec(positions), _vec(uvs)
+        out = np.zeros(3)
+        library().oracle_normal_map(self._h, texture, int(flip_y), scale, u, v, g.ctypes.data, p.ctypes.data,
+                                    t.ctypes.data, out.ctypes.data)
+        return out
+
+    def random_walk(self, entry, out_n, mfp: float, albedo: float, aniso: float, state: int):
+        """(exited, exit origin, exit direction, attenuation, new state)."""
+        a, b = _vec(entry), _vec(out_n)
+        st = c_uint32(state)
+        eo, ed, att = np.zeros(3), np.zeros(3), np.zeros(1)
+        ok = library().oracle_random_walk(self._h, a.ctypes.data, b.ctypes.data, mfp, albedo, aniso, ctypes.byref(st),
+                                          eo.ctypes.data, ed.ctypes.data, att.ctypes.data)
+        return bool(ok), eo, ed, float(att[0]), st.value
 
     def close(self):
         if self._h:
@@ -271,6 +320,16 @@ def dome_uv(d, fmt: int):
     uv = np.zeros(2)
     library().oracle_dome_uv(a.ctypes.data, fmt, uv.ctypes.data)
     return uv
+
+
+def transmission(rd, n, front: bool, ior: float, thin: bool, state: int):
+    """(crossed, new direction, new state) of a dielectric boundary event."""
+    a, b = _vec(rd), _vec(n)
+    st = c_uint32(state)
+    out = np.zeros(3)
+    crossed = library().oracle_transmission(a.ctypes.data, b.ctypes.data, int(front), ior, int(thin), ctypes.byref(st),
+                                            out.ctypes.data)
+    return bool(crossed), out, st.value
 
 
 def lat_long_direction(u: float, v: float) -> np.ndarray:

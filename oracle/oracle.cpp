@@ -1202,6 +1202,27 @@ struct Integrator {
         const V3<R> b = cross(w, t);
         return normalize(t * (std::cos(phi) * sine) + b * (std::sin(phi) * sine) + w * cosine);
     }
+    // Reflection or refraction at a dielectric boundary (:1737-1750): Schlick
+    // Fresnel from the spectral IOR picks the event with one random number
+    // (after TIR, which always reflects); thin walls pass straight through.
+    // rd becomes the new direction; returns true when the path crosses into
+    // or out of the medium.
+    static bool Transmit(V3<R>& rd, const V3<R>& normal, bool front, R ior, bool thin, std::uint32_t& rng)
+    {
+        const R eta = front ? R(1) / ior : ior;
+        const R cosT = std::min(dot(-rd, normal), R(1));
+        R r0 = (R(1) - ior) / (R(1) + ior);
+        r0 *= r0;
+        const R fres = r0 + (R(1) - r0) * std::pow(R(1) - cosT, R(5));
+        const V3<R> refr = refract(rd, normal, eta);
+        const bool tir = dot(refr, refr) < R(1e-8);
+        if (tir || Rand(rng) < fres) {
+            rd = reflect(rd, normal);
+            return false;
+        }
+        rd = thin ? rd : normalize(refr);
+        return !thin;
+    }
     // traceOpaqueBoundary (:1130-1151)
     bool OpaqueBoundary(const V3<R>& o, const V3<R>& d, V3<R>& boundary, V3<R>& bn, R& dist) const
     {
@@ -1586,21 +1607,7 @@ struct Integrator {
             }
             // Transmission (:1735-1777).
             if (Rand(rng) < transmission) {
-                const R eta = front ? R(1) / ior : ior;
-                const R cosT = std::min(dot(-rd, normal), R(1));
-                R r0 = (R(1) - ior) / (R(1) + ior);
-                r0 *= r0;
-                const R fres = r0 + (R(1) - r0) * std::pow(R(1) - cosT, R(5));
-                const V3<R> refr = refract(rd, normal, eta);
-                const bool tir = dot(refr, refr) < R(1e-8);
-                const bool thin = m.transmissionColorThinWalled.w > 0.5F;
-                bool through = false;
-                if (tir || Rand(rng) < fres) {
-                    rd = reflect(rd, normal);
-                } else {
-                    rd = thin ? rd : normalize(refr);
-                    through = !thin;
-                }
+                const bool through = Transmit(rd, normal, front, ior, m.transmissionColorThinWalled.w > 0.5F, rng);
                 const R depth = std::max(R(m.transmissionMedium.x), R(0));
                 throughput *= depth > R(0) ? R(1) : transmissionColor;
                 if (through) {
@@ -1959,6 +1966,82 @@ double oracle_sample_dome(void* sp, std::uint32_t light, std::uint32_t* state, d
     dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
     if (pdf_eval) *pdf_eval = integ.DomePdf(os->scene.lights[light], d);
     return pdf;
+}
+
+// Transmission event at a boundary (Integrator::Transmit): rd is the
+// incident direction, n the shading normal facing it, front whether the
+// path enters the object; writes the new direction, returns 1 when the path
+// crosses the boundary.
+int oracle_transmission(const double* rd, const double* n, int front, double ior, int thin, std::uint32_t* state,
+                        double* out)
+{
+    using V = oracle::V3<double>;
+    V d(rd[0], rd[1], rd[2]);
+    const bool through = oracle::Integrator<double>::Transmit(d, V(n[0], n[1], n[2]), front != 0, ior, thin != 0, *state);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+    return through ? 1 : 0;
+}
+// domeLightPdf (:583-598) of light `light` for world direction dir.
+double oracle_dome_pdf(void* sp, std::uint32_t light, const double* dir)
+{
+    auto* os = static_cast<OracleScene*>(sp);
+    const oracle::Integrator<double> integ{os->scene, os->scene.lightCount, false, os->scene.opaque};
+    return integ.DomePdf(os->scene.lights[light], oracle::V3<double>(dir[0], dir[1], dir[2]));
+}
+// sampleDome (:527-537): RGB radiance of dome light `light` toward dir.
+void oracle_dome_radiance(void* sp, std::uint32_t light, const double* dir, double* out)
+{
+    auto* os = static_cast<OracleScene*>(sp);
+    const oracle::Integrator<double> integ{os->scene, os->scene.lightCount, false, os->scene.opaque};
+    const auto c = integ.SampleDome(os->scene.lights[light], oracle::V3<double>(dir[0], dir[1], dir[2]));
+    out[0] = c.x; out[1] = c.y; out[2] = c.z;
+}
+// Bilinear REPEAT lookup of texture `texture` (decoded RGBA) at (u, v).
+void oracle_texture(void* sp, std::uint32_t texture, double u, double v, double* out)
+{
+    auto* os = static_cast<OracleScene*>(sp);
+    const oracle::Integrator<double> integ{os->scene, os->scene.lightCount, false, os->scene.opaque};
+    double a = 0.0;
+    const auto c = integ.TexRgb(texture, u, v, &a);
+    out[0] = c.x; out[1] = c.y; out[2] = c.z; out[3] = a;
+}
+// sampleMaterialTexture (:760-775) for an encoded texture slot (plain index,
+// UDIM set 0x80000000 | mask << 8 | first, or 0xffffffff = missing).
+void oracle_material_texture(void* sp, std::uint32_t slot, double u, double v, const double* fallback, double* out)
+{
+    auto* os = static_cast<OracleScene*>(sp);
+    const oracle::Integrator<double> integ{os->scene, os->scene.lightCount, false, os->scene.opaque};
+    integ.SampleMat(slot, u, v, fallback, out);
+}
+// applyNormalMap (:807-829): p holds the triangle's three positions, uv its
+// three corner UVs; gn is the (shading) normal the map perturbs.
+void oracle_normal_map(void* sp, std::uint32_t texture, int flip_y, double scale, double u, double v, const double* gn,
+                       const double* p, const double* uv, double* out)
+{
+    using V = oracle::V3<double>;
+    auto* os = static_cast<OracleScene*>(sp);
+    const oracle::Integrator<double> integ{os->scene, os->scene.lightCount, false, os->scene.opaque};
+    const V r = integ.NormalMap(texture, flip_y != 0, scale, u, v, V(gn[0], gn[1], gn[2]), V(p[0], p[1], p[2]),
+                                V(p[3], p[4], p[5]), V(p[6], p[7], p[8]), uv[0], uv[1], uv[2], uv[3], uv[4], uv[5]);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+// randomWalkSubsurface (:1153-1186) from entry point `entry` with outward
+// normal out_n; returns 1 when the walk exits (exit origin, direction and
+// attenuation written).
+int oracle_random_walk(void* sp, const double* entry, const double* out_n, double mfp, double albedo, double aniso,
+                       std::uint32_t* state, double* exit_o, double* exit_d, double* att)
+{
+    using V = oracle::V3<double>;
+    auto* os = static_cast<OracleScene*>(sp);
+    const oracle::Integrator<double> integ{os->scene, os->scene.lightCount, false, os->scene.opaque};
+    V eo, ed;
+    double a = 0.0;
+    const bool exited = integ.RandomWalk(V(entry[0], entry[1], entry[2]), V(out_n[0], out_n[1], out_n[2]), mfp, albedo,
+                                         aniso, *state, eo, ed, a);
+    exit_o[0] = eo.x; exit_o[1] = eo.y; exit_o[2] = eo.z;
+    exit_d[0] = ed.x; exit_d[1] = ed.y; exit_d[2] = ed.z;
+    *att = a;
+    return exited ? 1 : 0;
 }
 
 } // extern "C"

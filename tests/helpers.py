@@ -47,3 +47,38 @@ def scene_rays(camera, n: int, seed: int, spread: float = 1.0, inside: np.ndarra
     r[:, 4:7] = d
     r[:, 7] = 1e4
     return r
+
+
+OPACITY_OFFSET = 36  # transmissionOpacityIor.y in the 368-byte material record
+
+
+def cutout_view(scene):
+    """The scene's view with every other material at opacity 0.5 and the
+    opaque flag cleared (a scene with cut-outs). Returns (view, buffer); keep
+    the buffer alive while the view is used."""
+    import hdcodex_amd as hx
+
+    v = hx.SceneView.from_buffer_copy(scene.view)
+    mats = np.array(scene.arrays()["materials"], dtype=np.uint8, copy=True)
+    op = mats[:, OPACITY_OFFSET:OPACITY_OFFSET + 4].copy().view(np.float32).reshape(-1)
+    op[::2] = 0.5
+    mats[:, OPACITY_OFFSET:OPACITY_OFFSET + 4] = op.view(np.uint8).reshape(-1, 4)
+    mats = np.ascontiguousarray(mats)
+    v.materials = mats.ctypes.data
+    v.opaque = 0
+    return v, mats
+
+
+def cpu_threads() -> int:
+    """Worker threads for the CPU oracle: the scheduler affinity, capped by
+    OMP_NUM_THREADS when set (the host share a job is granted)."""
+    import os
+
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)

@@ -14,30 +14,15 @@ rays (instrumentation bit 3); the extend kernel must trace at least two
 rays fewer per pixel sample.
 """
 
-import ctypes
-
 import numpy as np
 import pytest
 
 import hdcodex_amd as hx
+from helpers import cutout_view
 
 pytestmark = pytest.mark.gpu
 
 TIMED, PER_LANE_RAYS = 1, 8
-OPACITY_OFFSET = 36  # transmissionOpacityIor.y in the 368-byte material record
-
-
-def _cutout(scene):
-    """The scene view with every other material at opacity 0.5 (not opaque)."""
-    v = hx.SceneView.from_buffer_copy(scene.view)
-    mats = np.array(scene.arrays()["materials"], dtype=np.uint8, copy=True)
-    op = mats[:, OPACITY_OFFSET:OPACITY_OFFSET + 4].copy().view(np.float32).reshape(-1)
-    op[::2] = 0.5
-    mats[:, OPACITY_OFFSET:OPACITY_OFFSET + 4] = op.view(np.uint8).reshape(-1, 4)
-    mats = np.ascontiguousarray(mats)
-    v.materials = mats.ctypes.data
-    v.opaque = 0
-    return v, mats
 
 
 def _render(view, cam, w, h, spp, flags):
@@ -62,7 +47,7 @@ def test_shared_rays_equal_per_lane_rays(kind, args, cutout):
     keep = None
     view = scene
     if cutout:
-        view, keep = _cutout(scene)
+        view, keep = cutout_view(scene)
     w, h, spp = 80, 48, 3
     cam = scene.camera(w, h)
     shared, ss = _render(view, cam, w, h, spp, 0)

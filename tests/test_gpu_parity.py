@@ -14,7 +14,7 @@ import pytest
 
 import hdcodex_amd as hx
 import oracle
-from helpers import rel_mse, scene_rays, ulp_distance
+from helpers import cutout_view, rel_mse, scene_rays, ulp_distance
 
 pytestmark = pytest.mark.gpu
 
@@ -115,6 +115,26 @@ def test_render_heightfield_matches_fp64_oracle(tracer):
     err = rel_mse(gpu, ref)
     print("heightfield rel-MSE vs FP64 oracle", err)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("kind,args,size", [("materials", (0,), (96, 48, 8)), ("mixed", (3, 20000, 64), (80, 48, 4))],
+                         ids=["materials", "mixed"])
+def test_render_cutout_matches_fp64_oracle(tracer, kind, args, size):
+    """Scenes with cut-outs: every other material at opacity 0.5, so closest
+    hits go through the any-hit loop (one random number per candidate in
+    (t, primitive) order, :1228-1243) and shadow rays confirm occluders of
+    opacity >= 0.5 (:855-889)."""
+    scene = hx.ProceduralScene(kind, *args)
+    view, keep = cutout_view(scene)
+    w, h, spp = size
+    tracer.set_scene(view)
+    cam = scene.camera(w, h)
+    gpu = tracer.render(cam, w, h, 0, 8, spp)
+    ref = oracle.OracleScene(view).render(cam, w, h, 0, 8, spp, precision=64) / spp
+    err = rel_mse(gpu, ref)
+    print(kind, "cut-out rel-MSE vs FP64 oracle", err)
+    assert err < 1e-4
+    del keep
 
 
 def test_progressive_batching_matches_sequential(tracer):
