@@ -198,33 +198,119 @@ def score(frame: np.ndarray, ref) -> dict:
     return {"rel_mse": e, "debiased": e - floor, "noise_floor": floor, "mean_ratio": float(x.mean() / y.mean())}
 
 
-def equal_time(hx, tracer, cam, plain_frame: np.ndarray, plain_ms: float, warmup_frames: int) -> dict:
-    """Radiance cache vs plain at equal wall time on the heightfield
-    workload, both scored against FP64 oracle references of the same pixel
-    subset (4096 and 32768 spp). The cache is warm: it has trained online on
-    warmup_frames frames of 2 spp (not timed), as in an interactive session;
-    the measured frame trains as well (its training time is inside the
-    timed region)."""
-    refs = {k: np.load(p) for k, p in REFERENCES.items() if p.exists()}
-    out = {"plain": {"spp": SPP, "ms": plain_ms, **{k: score(plain_frame, r) for k, r in refs.items()}}}
-    config = dict(train_samples=65536, terminate_bounce=1, train_steps=4, learning_rate=1e-2, seed=1, ema_decay=0.99)
-    tracer.set_radiance_cache(True, **config)
+CACHE_CONFIG = dict(train_samples=65536, terminate_bounce=1, train_steps=4, learning_rate=1e-2, ema_decay=0.99)
+WARM_W, WARM_H, WARM_SPP = 240, 135, 2  # training-only frames: the same view at 1/64 of the pixels
+
+
+def _timed_render(tracer, cam, w, h, index, spp) -> float:
     t0 = time.perf_counter()
-    for i in range(warmup_frames):
-        tracer.render(cam, WIDTH, HEIGHT, 1000 + 2 * i, BOUNCES, 2, readback=False)
-    warm_s = time.perf_counter() - t0
-    stats0 = tracer.rc_stats()
-    # Frame time = fixed cost (training, launches) + per-spp cost: fitted on
-    # 4- and 12-spp frames; the cache gets the largest spp that fits in 98%
-    # of the plain frame's time (the margin keeps timing noise from giving
-    # the cache more time than the plain frame).
-    times = {}
-    for n in (4, 12):
-        t0 = time.perf_counter()
-        tracer.render(cam, WIDTH, HEIGHT, 0, BOUNCES, n, readback=False)
-        times[n] = time.perf_counter() - t0
+    tracer.render(cam, w, h, index, BOUNCES, spp, readback=False)  # synchronous
+    return time.perf_counter() - t0
+
+
+def cache_frame_model(hx, scene, cam, seed: int) -> tuple:
+    """Time of one cached 1080p frame as fixed + per-spp cost (its training
+    included), fitted on a separate tracer and network from 4- and 12-spp
+    frames: only used to plan how many samples fit a budget."""
+    t = hx.PathTracer(0)
+    try:
+        t.set_bvh_options(hx.BVH_HOST_SAH, 2048, True)
+        t.set_scene(scene)
+        t.set_radiance_cache(True, seed=seed, **CACHE_CONFIG)
+        for i in range(3):
+            _timed_render(t, cam, WIDTH, HEIGHT, 500 + i, 4)
+        times = {n: min(_timed_render(t, cam, WIDTH, HEIGHT, 0, n) for _ in range(2)) for n in (4, 12)}
+    finally:
+        t.close()
     per_spp = max((times[12] - times[4]) / 8, 1e-6)
-    fixed = max(times[4] - 4 * per_spp, 0.0)
+    return max(times[4] - 4 * per_spp, 0.0), per_spp
+
+
+def cold_start_session(tracer, cam, budget_s: float, model: tuple, seed: int, warm_fraction: float,
+                       max_warm_frames: int = 200) -> dict:
+    """The radiance-cache method from a fresh network within a wall-clock
+    budget: training-only frames (the same view at 240 x 135, 2 spp) for
+    `warm_fraction` of the budget, then 1080p frames accumulated
+    progressively with the cache (each trains the network too), the last one
+    given the samples that still fit. Everything after the network's
+    initialisation is inside the budget."""
+    fixed, per_spp = model
+    tracer.set_radiance_cache(True, seed=seed, **CACHE_CONFIG)  # a new seed re-initialises the network
+    t0 = time.perf_counter()
+    warm = 0
+    while warm < max_warm_frames and time.perf_counter() - t0 < warm_fraction * budget_s:
+        tracer.render(cam, WARM_W, WARM_H, 2 * warm, BOUNCES, WARM_SPP, readback=False)
+        warm += 1
+    warm_s = time.perf_counter() - t0
+    spp_done, frames = 0, 0
+    while True:
+        left = budget_s - (time.perf_counter() - t0)
+        spp = min(16, int((0.97 * left - fixed) / per_spp))
+        if spp < 1:
+            break
+        tracer.render(cam, WIDTH, HEIGHT, spp_done, BOUNCES, spp, readback=False)
+        spp_done += spp
+        frames += 1
+    total_s = time.perf_counter() - t0
+    return {"warm_frames": warm, "warm_s": warm_s, "frames": frames, "spp": spp_done, "seconds": total_s}
+
+
+def equal_time(hx, scene, tracer, cam, plain_frame: np.ndarray, plain_ms: float, sessions: int = 8) -> dict:
+    """Radiance cache vs plain at equal wall time on the heightfield
+    workload, from a cold start, scored against FP64 oracle references
+    (4096 spp on every 8th row and column: 32,400 pixels; 32,768 spp on
+    every 32nd). For budgets of K plain frames (16 spp each, accumulated
+    progressively), the cache method starts from an untrained network and
+    spends the same wall time on training-only frames and cached frames.
+    rel-MSE is heavy-tailed (a rare bright path shared by both methods can
+    dominate an image), so each budget is run as `sessions` independent
+    sessions (disjoint sample streams, a fresh network each) and the ratio
+    of the mean errors is reported. Also: a warm network (trained on 50
+    earlier frames, outside the budget) rendering one frame in one plain
+    frame's time."""
+    refs = {k: np.load(p) for k, p in REFERENCES.items() if p.exists()}
+    model = cache_frame_model(hx, scene, cam, 77)
+    out = {"config": {**CACHE_CONFIG, "terminate_vertex": "second (index 1)"},
+           "frame_model": {"fixed_ms": model[0] * 1e3, "per_spp_ms": model[1] * 1e3},
+           "warm_frame": f"{WARM_W}x{WARM_H}, {WARM_SPP} spp, 30% of the budget", "sessions": sessions, "budgets": []}
+    for k in (1, 4):
+        per = {key: {"plain": [], "cache": []} for key in refs}
+        runs = []
+        for j in range(sessions):
+            tracer.set_sample_offset(1_000_000 + 10_000 * j)
+            tracer.set_radiance_cache(False)
+            t0 = time.perf_counter()
+            for f in range(k):
+                tracer.render(cam, WIDTH, HEIGHT, 16 * f, BOUNCES, 16, readback=False)
+            plain_s = time.perf_counter() - t0
+            plain = tracer.read_film(WIDTH, HEIGHT)
+            session = cold_start_session(tracer, cam, plain_s, model, 1000 + 100 * k + j, 0.3)
+            if session["spp"] == 0:
+                raise RuntimeError("the cold-start session rendered no cached frame within its budget")
+            cached = tracer.read_film(WIDTH, HEIGHT)
+            session["plain_s"] = plain_s
+            runs.append(session)
+            for key, ref in refs.items():
+                per[key]["plain"].append(score(plain, ref)["rel_mse"])
+                per[key]["cache"].append(score(cached, ref)["rel_mse"])
+        tracer.set_sample_offset(0)
+        entry = {"plain_frames": k, "plain_spp": 16 * k, "budget_ms": 1e3 * float(np.mean([r["plain_s"] for r in runs])),
+                 "cache_spp": [r["spp"] for r in runs], "cache_warm_frames": [r["warm_frames"] for r in runs],
+                 "cache_ms": [1e3 * r["seconds"] for r in runs]}
+        for key in refs:
+            p, c = np.array(per[key]["plain"]), np.array(per[key]["cache"])
+            entry[key] = {"plain_rel_mse": p.tolist(), "cache_rel_mse": c.tolist(),
+                          "mean_plain": float(p.mean()), "mean_cache": float(c.mean()),
+                          "ratio_of_means": float(p.mean() / c.mean()),
+                          "median_ratio": float(np.median(p / c)), "sessions_cache_better": int((c < p).sum())}
+        out["budgets"].append(entry)
+        log(f"equal time, {k} plain frame(s) ~ {entry['budget_ms']:.0f} ms, {sessions} cold sessions: "
+            + ", ".join(f"{key} ratio of means {entry[key]['ratio_of_means']:.2f}" for key in refs))
+    # Warm network, one plain frame's time (the interactive case).
+    tracer.set_radiance_cache(True, seed=1, **CACHE_CONFIG)
+    for i in range(50):
+        tracer.render(cam, WARM_W, WARM_H, 1000 + 2 * i, BOUNCES, WARM_SPP, readback=False)
+    fixed, per_spp = model
     spp = max(1, int((0.98 * plain_ms * 1e-3 - fixed) / per_spp))
     tracer.reset_kernel_stats()
     tracer.set_instrumentation(1)
@@ -238,22 +324,21 @@ def equal_time(hx, tracer, cam, plain_frame: np.ndarray, plain_ms: float, warmup
     stats = tracer.rc_stats()
     queries = stats["queries"] - before["queries"]
     tracer.set_radiance_cache(False)
-    cache = {"spp": spp, "ms": cache_ms, **{k2: score(frame, r) for k2, r in refs.items()},
-             "config": config, "warmup_frames": warmup_frames, "warmup_spp": 2, "warmup_s": warm_s,
-             "train_loss": stats["last_loss"], "queries": int(queries),
-             "train_ms": stats["train_ms"] - before["train_ms"],
-             "warmup_queries": int(before["queries"] - stats0["queries"])}
+    warm = {"spp": spp, "ms": cache_ms, "plain_ms": plain_ms, "train_loss": stats["last_loss"],
+            "queries": int(queries), "train_ms": stats["train_ms"] - before["train_ms"]}
+    for key, ref in refs.items():
+        p, c = score(plain_frame, ref), score(frame, ref)
+        warm[key] = {"plain": p, "cache": c, "ratio_plain_over_cache": p["rel_mse"] / max(c["rel_mse"], 1e-30)}
     infer_ms = k["cache"]["ms"]
-    cache["mlp_infer_ms"] = infer_ms
+    warm["mlp_infer_ms"] = infer_ms
     if infer_ms > 0 and queries > 0:
         tflops = queries * MLP_FLOP_PER_QUERY / (infer_ms * 1e-3) / 1e12
-        cache["mlp_mfma"] = {"bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                             "frac": tflops / BF16_PEAK_TFLOPS, "flop_per_query": MLP_FLOP_PER_QUERY}
-    out["cache"] = cache
-    for key in refs:
-        out[f"ratio_plain_over_cache_{key}"] = out["plain"][key]["rel_mse"] / max(cache[key]["rel_mse"], 1e-30)
-        out[f"ratio_plain_over_cache_{key}_debiased"] = (out["plain"][key]["debiased"] /
-                                                         max(cache[key]["debiased"], 1e-30))
+        warm["mlp_mfma"] = {"bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": tflops / BF16_PEAK_TFLOPS, "flop_per_query": MLP_FLOP_PER_QUERY}
+    out["warm_network_one_frame"] = warm
+    head = out["budgets"][0]
+    out["headline"] = {"budget": "one plain frame (16 spp), cold start, mean over sessions", "reference": "4096spp",
+                       "ratio_plain_over_cache": head["4096spp"]["ratio_of_means"] if "4096spp" in head else None}
     return out
 
 
@@ -518,7 +603,6 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-cache", action="store_true", help="skip the radiance-cache equal-time comparison")
-    ap.add_argument("--cache-warmup", type=int, default=50)
     ap.add_argument("--bvh", choices=("lbvh", "sah"), default="sah",
                     help="BVH builder: device LBVH with SAH top levels, or host binned SAH")
     ap.add_argument("--no-batch", action="store_true",
@@ -681,7 +765,7 @@ def main() -> int:
     }
     if not args.no_cache:
         log("radiance cache vs plain at equal time")
-        result["equal_time"] = equal_time(hx, tracer, cam, frame, ms_per_step, args.cache_warmup)
+        result["equal_time"] = equal_time(hx, scene, tracer, cam, frame, ms_per_step)
     if not args.no_parity:
         log("parity vs FP64 oracle (subset)")
         result["parity"] = parity(scene, cam, frame)

@@ -36,7 +36,7 @@ KERNELS = ("raygen", "extend", "shade", "shadow", "film", "cache", "walk", "pair
 # Every symbol include/hdcx.h declares.
 EXPORTED_SYMBOLS = (
     "hdcx_last_error", "hdcx_create", "hdcx_destroy", "hdcx_set_scene", "hdcx_validate_scene", "hdcx_set_shading_mode",
-    "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_has_geometry", "hdcx_trace_closest",
+    "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_set_sample_offset", "hdcx_has_geometry", "hdcx_trace_closest",
     "hdcx_trace_occluded", "hdcx_scene_create_procedural", "hdcx_scene_view_get", "hdcx_scene_camera_get",
     "hdcx_scene_destroy", "hdcx_scene_flatten", "hdcx_bvh_build_host", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
 )
@@ -132,6 +132,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_get_shading_mode.argtypes = [c_void_p, POINTER(c_int)]
     lib.hdcx_render.argtypes = [c_void_p, POINTER(Camera), c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, c_void_p]
     lib.hdcx_read_film.argtypes = [c_void_p, c_void_p]
+    lib.hdcx_set_sample_offset.argtypes = [c_void_p, c_uint32]
     lib.hdcx_has_geometry.argtypes = [c_void_p]
     lib.hdcx_trace_closest.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
     lib.hdcx_trace_occluded.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
@@ -298,6 +299,10 @@ class PathTracer:
         _check(code)
         return out if out is not None else np.empty((0,), dtype=np.float32)
 
+    def set_sample_offset(self, offset: int) -> None:
+        """Seeds sample number n as n + offset (independent sample streams)."""
+        _check(library().hdcx_set_sample_offset(self._h, offset))
+
     def read_film(self, width: int, height: int) -> np.ndarray:
         out = np.empty((height, width, 4), dtype=np.float32)
         _check(library().hdcx_read_film(self._h, out.ctypes.data))
@@ -346,15 +351,17 @@ HIT_DTYPE = np.dtype([("t", np.float32), ("primitive", np.uint32), ("u", np.floa
 class RadianceCacheConfig(ctypes.Structure):
     _fields_ = [("enabled", c_uint32), ("train_samples", c_uint32), ("terminate_bounce", c_uint32),
                 ("train_steps", c_uint32), ("learning_rate", c_float), ("beta1", c_float), ("beta2", c_float),
-                ("epsilon", c_float), ("ema_decay", c_float), ("spread_threshold", c_float), ("seed", c_uint64)]
+                ("epsilon", c_float), ("ema_decay", c_float), ("spread_threshold", c_float), ("seed", c_uint64),
+                ("calibration_decay", c_float), ("flags", c_uint32)]
 
 
 class RcStats(ctypes.Structure):
     _fields_ = [("frames", c_uint64), ("queries", c_uint64), ("train_samples", c_uint64), ("adam_steps", c_uint64),
-                ("last_loss", c_float), ("reserved", c_float), ("infer_ms", c_double), ("train_ms", c_double)]
+                ("last_loss", c_float), ("calibration_scale", c_float), ("infer_ms", c_double), ("train_ms", c_double)]
 
 
 RC_PARAMS = 24768
+RC_PLAIN_TRAINING_PATHS = 1
 RC_QUERY_DTYPE = np.dtype([("position", np.float32, 3), ("direction", np.float32, 3), ("lambda0", np.float32),
                            ("roughness", np.float32), ("albedo", np.float32), ("lane", np.uint32),
                            ("path", np.uint32), ("target", np.float32)])
@@ -372,9 +379,11 @@ def _bind_cache(lib: ctypes.CDLL) -> None:
 
 def _tracer_set_radiance_cache(self, enabled: bool = True, train_samples: int = 65536, terminate_bounce: int = 1,
                                train_steps: int = 4, learning_rate: float = 1e-2, seed: int = 0,
-                               ema_decay: float = 0.99, spread_threshold: float = 0.0) -> None:
+                               ema_decay: float = 0.99, spread_threshold: float = 0.0,
+                               calibration_decay: float = 0.0, plain_training_paths: bool = False) -> None:
     c = RadianceCacheConfig(int(enabled), train_samples, terminate_bounce, train_steps, learning_rate, 0.9, 0.99,
-                            1e-8, ema_decay, spread_threshold, seed)
+                            1e-8, ema_decay, spread_threshold, seed, calibration_decay,
+                            RC_PLAIN_TRAINING_PATHS if plain_training_paths else 0)
     _check(library().hdcx_set_radiance_cache(self._h, ctypes.byref(c)))
 
 
@@ -571,7 +580,7 @@ class BatchRenderer:
         """One cache per scene (hdcx_batch_set_radiance_cache); train_samples
         per scene and render."""
         c = RadianceCacheConfig(int(enabled), train_samples, terminate_bounce, train_steps, learning_rate, 0.9, 0.99,
-                                1e-8, ema_decay, spread_threshold, seed)
+                                1e-8, ema_decay, spread_threshold, seed, 0.0, 0)
         _check(library().hdcx_batch_set_radiance_cache(self._h, ctypes.byref(c)))
 
     def rc_stats(self) -> dict:

@@ -85,14 +85,13 @@ struct hdcx_batch : PathPool {
     // Radiance caches: the networks of all scenes side by side (scene s at
     // s * kRcParams; transposes at s * 4 * 4096), trained after each render
     // on the samples it gathered (sampleCapacity per scene).
-    static constexpr unsigned kRcGradBlocks = 256;
     static constexpr std::uint32_t kRecordVertices = 4;
     hdcx_radiance_cache_config rcConfig{};
     bool rcEnabled = false;
     std::uint64_t rcSeed = 0;
     std::uint32_t rcScenes = 0, rcFrame = 0;
     std::vector<int> rcStep;
-    DeviceBuffer rcMaster, rcM, rcV, rcEma, rcW, rcWt, rcWe, rcWte, rcGrad, rcLoss;
+    DeviceBuffer rcMaster, rcM, rcV, rcEma, rcW, rcWt, rcWe, rcWte, rcGrad, rcLoss, rcAct;
     DeviceBuffer rcQueries, rcSorted, rcCounters, rcRecords, rcRecordThroughput, rcTrainSlot, rcSamples, rcSceneSamples,
         rcSceneTrain, rcBucketCount, rcBucketStart, rcBucketBlocks, rcBucketCursor;
     hdcx_batch_rc_stats rcStats{};
@@ -145,7 +144,7 @@ struct hdcx_batch : PathPool {
         rcWe.Allocate(params * 2);
         rcWt.Allocate(static_cast<std::size_t>(n) * 4 * 4096 * 2);
         rcWte.Allocate(static_cast<std::size_t>(n) * 4 * 4096 * 2);
-        rcGrad.Allocate(static_cast<std::size_t>(kRcGradBlocks) * kRcParams * 4);
+        rcGrad.Allocate(static_cast<std::size_t>(kRcGradSplits) * kRcParams * 4);
         rcLoss.Allocate(8);
         std::vector<float> w(params);
         for (std::uint32_t k = 0; k < n; ++k) InitRcWeights(rcConfig.seed + k, w.data() + static_cast<std::size_t>(k) * kRcParams);
@@ -251,6 +250,7 @@ struct hdcx_batch : PathPool {
         Check(hipMemsetAsync(rcLoss.as<void>(), 0, 8, stream), "batch rc loss");
         const std::uint32_t cap = std::max(rcConfig.train_samples, 1U);
         const std::uint32_t steps = std::max(rcConfig.train_steps, 1U);
+        rcAct.Allocate(std::max(rcAct.bytes(), RcGradWorkspaceBytes(cap / steps + 1U)));
         std::uint32_t trained = 0;
         for (std::uint32_t k = 0; k < n; ++k) {
             const std::uint32_t count = std::min(counts[k], cap);
@@ -262,10 +262,10 @@ struct hdcx_batch : PathPool {
                 const auto b0 = static_cast<std::uint32_t>(static_cast<std::uint64_t>(count) * i / steps);
                 const auto b1 = static_cast<std::uint32_t>(static_cast<std::uint64_t>(count) * (i + 1) / steps);
                 if (b1 <= b0) continue;
-                Check(LaunchRcGradient(RcNetwork{st.w, st.wt}, samples + b0, nullptr, b1 - b0, st.grad, st.loss,
-                                       kRcGradBlocks, stream),
+                Check(LaunchRcGradient(RcNetwork{st.w, st.wt}, samples + b0, b1 - b0, rcAct.as<__bf16>(), st.grad, st.loss,
+                                       stream),
                       "batch rc gradient");
-                Check(LaunchRcAdamBlocks(st, static_cast<int>(kRcGradBlocks), ++rcStep[k], rcConfig.learning_rate,
+                Check(LaunchRcAdamBlocks(st, kRcGradSplits, ++rcStep[k], rcConfig.learning_rate,
                                          rcConfig.beta1, rcConfig.beta2, rcConfig.epsilon, stream),
                       "batch rc adam");
                 ++rcStats.adam_steps;

@@ -7,8 +7,12 @@ namespace hdcx {
 
 hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t paths, const CacheParams& cp,
                         unsigned maxBlocks, hipStream_t st);
+// Training samples from the chunk's records; then the records are
+// consumed and, with `outputs` (the cache's three outputs for every record
+// query), training paths that reached the terminate vertex take the cached
+// value there in the image, scaled by *calibration when given.
 hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, std::uint32_t records, unsigned maxBlocks,
-                                hipStream_t st);
+                                hipStream_t st, const float* outputs = nullptr, const float* calibration = nullptr);
 // Extend sharing: kShareNone traces every queued path; kShareCamera (the
 // launch after raygen) traces one camera ray per pixel sample for its
 // three lanes; kShareLanes traces the paths LaunchPair queued as tracing

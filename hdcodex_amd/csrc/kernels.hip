@@ -1275,7 +1275,6 @@ __global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < records; i += gridDim.x * blockDim.x) {
         RcQuery r = cp.records[i];
         if (r.lane > 2U) continue;
-        cp.records[i].lane = 0xffffffffU;  // consumed: never reused by a later chunk
         const float t = cp.recordThroughput[i];
         if (!(t > 0.0f)) continue;
         const float target = (pb.radiance[r.path] - r.target) / t;
@@ -1288,6 +1287,28 @@ __global__ void gatherTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32
         } else {
             const std::uint32_t k = atomicAdd(cp.sampleCount, 1U);
             if (k < cp.sampleCapacity) cp.samples[k] = r;
+        }
+    }
+}
+
+// After the samples are gathered: the records are consumed (never reused
+// by a later chunk), and with `outputs` (the cache's answers to the
+// records' queries) a training lane path whose record at the terminate
+// vertex exists takes, in the image, the value a path that stopped there
+// gets: the radiance up to that vertex plus its throughput times the
+// cache's answer (the plain rest of the path trained the cache only).
+__global__ void finishTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32_t records, const float* outputs,
+                                     const float* calibration)
+{
+    if (cp.recordCount) records = min(*cp.recordCount, cp.recordSlots) * cp.recordVertices;
+    const float scale = calibration ? *calibration : 1.0f;
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < records; i += gridDim.x * blockDim.x) {
+        const RcQuery r = cp.records[i];
+        if (r.lane > 2U) continue;
+        cp.records[i].lane = 0xffffffffU;
+        if (outputs && i % cp.recordVertices == cp.terminateBounce) {
+            const float y = outputs[3U * i + r.lane];
+            pb.radiance[r.path] = r.target + cp.recordThroughput[i] * (scale * fmaxf(y, 0.0f));
         }
     }
 }
@@ -1448,9 +1469,11 @@ hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t
     return hipGetLastError();
 }
 hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, std::uint32_t records, unsigned maxBlocks,
-                                hipStream_t st)
+                                hipStream_t st, const float* outputs, const float* calibration)
 {
     hipLaunchKernelGGL(gatherTrainingKernel, dim3(GridFor(records, maxBlocks)), dim3(kBlock), 0, st, pb, cp, records);
+    hipLaunchKernelGGL(finishTrainingKernel, dim3(GridFor(records, maxBlocks)), dim3(kBlock), 0, st, pb, cp, records, outputs,
+                       calibration);
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>

@@ -19,6 +19,8 @@ inline hdcx_radiance_cache_config NormalizedRcConfig(const hdcx_radiance_cache_c
     if (!(c.epsilon > 0.0F)) c.epsilon = 1e-8F;
     if (c.terminate_bounce >= HDCX_MAX_BOUNCES) throw std::invalid_argument("terminate_bounce out of range");
     if (!(c.ema_decay >= 0.0F && c.ema_decay < 1.0F)) throw std::invalid_argument("ema_decay outside [0, 1)");
+    if (!(c.calibration_decay >= 0.0F && c.calibration_decay < 1.0F))
+        throw std::invalid_argument("calibration_decay outside [0, 1)");
     return c;
 }
 

@@ -58,10 +58,26 @@ struct RcState {
 };
 
 // Inference: out[i * 3 + k] for every query (when out != nullptr), and
-// radiance[path] += throughput[path] * max(out[lane], 0) when radiance != nullptr.
+// radiance[path] += throughput[path] * scale * max(out[lane], 0) when
+// radiance != nullptr, with scale = *calibration (1 when it is null).
 hipError_t LaunchRcInfer(const RcNetwork& net, const RcQuery* queries, const std::uint32_t* count,
                          std::uint32_t maxCount, float* out, float* radiance, const float* throughput,
-                         unsigned maxBlocks, hipStream_t st);
+                         unsigned maxBlocks, hipStream_t st, const float* calibration = nullptr);
+
+// Brightness calibration of the cache against the plain estimator it learns
+// from. For the frame's training samples (targets t_i on lane k_i) and the
+// network's outputs y on them, sums[0] += sum t_i and sums[1] += sum
+// max(y_{k_i}, 0) (FP64). Then, in one thread: acc = decay * acc + sums,
+// sums cleared, and *scale = acc[0] / acc[1] (1 while acc[1] is 0).
+// Layout of `state`: double sums[2], double acc[2], float scale.
+struct RcCalibration {
+    double sums[2];
+    double acc[2];
+    float scale;
+    float pad[3];
+};
+hipError_t LaunchRcCalibrate(const RcQuery* samples, std::uint32_t count, const float* outputs, RcCalibration* state,
+                             float decay, hipStream_t st);
 // Buckets of cache queries by scene (batch renderer), device arrays:
 // count [scenes] (zero before the first use; the scan clears it), start and
 // blockStart [scenes + 1], cursor [scenes].
@@ -85,18 +101,35 @@ hipError_t LaunchRcInferScenes(const __bf16* weights, const RcQuery* queries, co
 // layer scaled by 0.5), FP32, kRcParams values.
 void InitRcWeights(std::uint64_t seed, float* w);
 
-// Forward + backward over `count` samples. Block b adds its share of
-// d(mean loss)/dW into partial row b (rows are kRcParams floats; at most
-// maxBlocks rows are used); loss[0] += summed loss, loss[1] += samples.
-hipError_t LaunchRcGradient(const RcNetwork& net, const RcQuery* samples, const std::uint32_t* count,
-                            std::uint32_t maxCount, float* partial, float* loss, unsigned maxBlocks, hipStream_t st);
+// Training step workspace: bf16 rows of RcGradPitch(count) samples each:
+// the encoded inputs (128 rows), h1..h5 (5 x 64), the deltas d1..d5
+// (5 x 64) and dy as hi and lo planes (6 rows).
+constexpr int kRcActX = 0;
+constexpr int kRcActH = 128;
+constexpr int kRcActD = 128 + 5 * 64;
+constexpr int kRcActDy = 128 + 10 * 64;
+constexpr int kRcActRows = kRcActDy + 6;
+// Partial gradient rows written by LaunchRcGradient (the Adam step sums them).
+constexpr int kRcGradSplits = 16;
+inline std::uint32_t RcGradPitch(std::uint32_t count) { return (count + 31U) / 32U * 32U; }
+inline std::size_t RcGradWorkspaceBytes(std::uint32_t count)
+{
+    return static_cast<std::size_t>(kRcActRows) * RcGradPitch(count) * sizeof(std::uint16_t);
+}
+
+// Forward + backward over `count` samples (workspace of
+// RcGradWorkspaceBytes(count)); writes d(mean loss)/dW as kRcGradSplits
+// partial rows of kRcParams floats (overwritten, no accumulation);
+// loss[0] += summed loss, loss[1] += samples.
+hipError_t LaunchRcGradient(const RcNetwork& net, const RcQuery* samples, std::uint32_t count, __bf16* workspace,
+                            float* partial, float* loss, hipStream_t st);
 // Adam step (bias-corrected) on all parameters from the sum of `blocks`
-// partial rows in s.grad (cleared afterwards); updates the weight average
-// and refreshes both BF16 copies.
+// partial rows in s.grad; updates the weight average and refreshes both
+// BF16 copies.
 hipError_t LaunchRcAdamBlocks(const RcState& s, int blocks, int step, float lr, float beta1, float beta2, float eps,
                               hipStream_t st);
-// out[i] = sum of the partial rows (cleared afterwards).
-hipError_t LaunchRcReduce(float* partial, int blocks, float* out, hipStream_t st);
+// out[i] = sum of the partial rows.
+hipError_t LaunchRcReduce(const float* partial, int blocks, float* out, hipStream_t st);
 // Resets the average to the FP32 master weights and publishes both BF16 copies.
 hipError_t LaunchRcPublish(const RcState& s, hipStream_t st);
 

@@ -193,7 +193,8 @@ __device__ void stageTiles(uint4* tiles, const __bf16* w)
 
 // One wave: queries [first, first + 32) below end.
 __device__ __forceinline__ void inferTile(const __bf16* w, const uint4* tiles, const RcQuery* queries, std::uint32_t first,
-                                          std::uint32_t end, int lane, float* out, float* radiance, const float* throughput)
+                                          std::uint32_t end, int lane, float* out, float* radiance, const float* throughput,
+                                          float scale = 1.0f)
 {
     const std::uint32_t qi = first + static_cast<std::uint32_t>(lane & 31);
     const bool valid = qi < end;
@@ -210,7 +211,7 @@ __device__ __forceinline__ void inferTile(const __bf16* w, const uint4* tiles, c
         }
         if (radiance) {
             const float v = q.lane == 0 ? y[0] : (q.lane == 1 ? y[1] : y[2]);
-            radiance[q.path] += throughput[q.path] * fmaxf(v, 0.0f);
+            radiance[q.path] += throughput[q.path] * (scale * fmaxf(v, 0.0f));
         }
     }
 }
@@ -222,10 +223,12 @@ template <bool kOut>
 __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(2))) void rcInferKernel(
     RcNetwork net, const RcQuery* queries,
                                                             const std::uint32_t* countPtr, std::uint32_t maxCount,
-                                                            float* out, float* radiance, const float* throughput)
+                                                            float* out, float* radiance, const float* throughput,
+                                                            const float* calibration)
 {
     __shared__ uint4 tiles[kWeightTiles * 64];  // 52 KB of MFMA-ready weight tiles
     const std::uint32_t count = countPtr ? min(*countPtr, maxCount) : maxCount;
+    const float scale = calibration ? *calibration : 1.0f;
     // Blocks without a tile of queries leave before staging the weights.
     if (blockIdx.x * (blockDim.x >> 6) >= (count + 31U) / 32U) return;
     stageTiles(tiles, net.w);
@@ -235,7 +238,50 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(2))
     const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const std::uint32_t tileCount = (count + 31U) / 32U;
     for (std::uint32_t tile = wave; tile < tileCount; tile += waves)
-        inferTile(net.w, tiles, queries, tile * 32U, count, lane, kOut ? out : nullptr, radiance, throughput);
+        inferTile(net.w, tiles, queries, tile * 32U, count, lane, kOut ? out : nullptr, radiance, throughput, scale);
+}
+
+// Calibration sums: one FP64 atomic pair per block.
+__global__ __launch_bounds__(256) void rcCalibrateSumKernel(const RcQuery* samples, std::uint32_t count,
+                                                            const float* outputs, RcCalibration* state)
+{
+    __shared__ double part[2][4];
+    double t = 0.0, y = 0.0;
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const RcQuery q = samples[i];
+        const std::uint32_t k = q.lane < 3U ? q.lane : 0U;
+        t += static_cast<double>(q.target);
+        y += static_cast<double>(fmaxf(outputs[3U * i + k], 0.0f));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        t += __shfl_xor(t, off);
+        y += __shfl_xor(y, off);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][wave] = t;
+        part[1][wave] = y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ts = 0.0, ys = 0.0;
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) {
+            ts += part[0][w];
+            ys += part[1][w];
+        }
+        atomicAdd(&state->sums[0], ts);
+        atomicAdd(&state->sums[1], ys);
+    }
+}
+
+__global__ void rcCalibrateFinishKernel(RcCalibration* state, float decay)
+{
+    const double d = static_cast<double>(decay);
+    state->acc[0] = d * state->acc[0] + state->sums[0];
+    state->acc[1] = d * state->acc[1] + state->sums[1];
+    state->sums[0] = 0.0;
+    state->sums[1] = 0.0;
+    state->scale = state->acc[1] > 0.0 ? static_cast<float>(state->acc[0] / state->acc[1]) : 1.0f;
 }
 
 // ---- One network per scene (batch renderer) ----
@@ -349,145 +395,163 @@ __global__ __launch_bounds__(kInferBlock) void rcInferScenesKernel(const __bf16*
     }
 }
 
-// Forward + backward of tiles of 32 training samples, one wave per block.
-// Loss per sample: (y_k - t)^2 / (y_k^2 + 0.01) on the sample's lane k with
-// the denominator held constant (relative L2), averaged over the batch.
-// Each block accumulates its weight gradient into its own partial row
-// partial[blockIdx.x][*] (plain read-modify-write: the row is private).
-__global__ __launch_bounds__(64) void rcGradKernel(RcNetwork net, const RcQuery* samples,
-                                                  const std::uint32_t* countPtr, std::uint32_t maxCount,
-                                                  float* partial, float* loss)
+// ---- Training: forward + backward, then the weight gradients as GEMMs ----
+//
+// Step 1 (rcForwardBackwardKernel, one wave per tile of 32 samples): the
+// encoded inputs, the activations h1..h5 and the deltas d1..d5 (the loss
+// gradient with respect to each layer's pre-activation, d5 = (W5^T dy) *
+// relu'(h5), d_L = (W_L^T d_{L+1}) * relu'(h_L)) stay in registers in the
+// MFMA operand layout (see the operand convention above) and are written
+// transposed, one row per unit and one column per sample, into the bf16
+// workspace; dy (3 outputs) as two bf16 planes, hi = bf16(dy) and lo =
+// bf16(dy - hi). Per sample: loss (y_k - t)^2 / (y_k^2 + 0.01) on the
+// sample's lane k, the denominator held constant (relative L2), averaged
+// over the batch.
+//
+// Step 2 (rcWeightGradKernel): dW_l[o][i] = sum_q d_{l+1}[o][q] in_l[i][q]
+// (in_0 = the encoded inputs, in_l = h_l) as MFMA tiles with the samples as
+// the reduction dimension: 26 output tiles of 32 x 32 (W0: 2 x 4, W1..W4:
+// 2 x 2 each, W5: 2 tiles of the 3 x 64 layer from dy's hi and lo planes),
+// each split over kRcGradSplits sample ranges. Every (tile, split) wave
+// writes its own rows of the partial gradient (no atomics: the sum is
+// deterministic); the Adam step adds the kRcGradSplits partial rows.
+
+// The workspace is tiled by 32 samples: tile t holds kRcActRows rows of 32
+// consecutive samples (64 bytes), so every row address of a tile is a
+// constant offset from the tile's base.
+__device__ __forceinline__ __bf16* actTile(__bf16* act, std::uint32_t tile)
 {
-    __shared__ __bf16 h0[32 * kP0];
-    __shared__ __bf16 hs[5 * 32 * kPH];  // h1..h5
-    __shared__ __bf16 ds[5 * 32 * kPH];  // delta1..delta5
-    __shared__ float dy[32 * 3];
-    const int lane = threadIdx.x, h = lane >> 5, q = lane & 31;
-    const std::uint32_t count = countPtr ? min(*countPtr, maxCount) : maxCount;
+    return act + static_cast<std::size_t>(tile) * (kRcActRows * 32);
+}
+
+// Stores a layer's 64 units held in B-operand layout (hc[2t + half][e] is
+// unit 32t + 16 half + 8h + e of sample q) as rows of the tile.
+__device__ __forceinline__ void storeUnits(__bf16* tileBase, int firstRow, int q, int h, const bf16x8 (&v)[4])
+{
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int half = 0; half < 2; ++half)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) tileBase[(firstRow + 32 * t + 16 * half + 8 * h + e) * 32 + q] = v[2 * t + half][e];
+}
+
+__global__ __launch_bounds__(256) void rcForwardBackwardKernel(RcNetwork net, const RcQuery* samples, std::uint32_t count,
+                                                               std::uint32_t pitch, __bf16* act, float* loss)
+{
+    const int lane = threadIdx.x & 63, h = lane >> 5, q = lane & 31;
     const float invN = count ? 1.0f / static_cast<float>(count) : 0.0f;
-    float* g = partial + static_cast<std::size_t>(blockIdx.x) * kRcParams;
-    const std::uint32_t tileCount = (count + 31U) / 32U;
+    const std::uint32_t tiles = pitch / 32U;
+    const std::uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     float lossSum = 0.0f, lossCount = 0.0f;
-    for (std::uint32_t tile = blockIdx.x; tile < tileCount; tile += gridDim.x) {
-        const std::uint32_t qi = tile * 32U + static_cast<std::uint32_t>(q);
-        const bool valid = qi < count;
+    for (std::uint32_t tile = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; tile < tiles; tile += waves) {
+        const std::uint32_t col = tile * 32U + static_cast<std::uint32_t>(q);
+        const bool valid = col < count;
         RcQuery smp{};
-        if (valid) smp = samples[qi];
+        if (valid) smp = samples[col];
+        __bf16* tb = actTile(act, tile);
         bf16x8 x[8];
         encode(smp, h, x);
 #pragma unroll
-        for (int s = 0; s < 8; ++s) *reinterpret_cast<bf16x8*>(h0 + q * kP0 + 16 * s + 8 * h) = x[s];
-        const f32x16 y = forward<true>(net.w, nullptr, x, lane, hs);
-        if (lane < 32) {
+        for (int s = 0; s < 8; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) tb[(kRcActX + 16 * s + 8 * h + e) * 32 + q] = x[s][e];
+        // Forward. Each hidden layer is stored, and kept as a 32-bit mask of
+        // its positive units (bit 8 (2t + half) + e: the relu' of the backward pass).
+        bf16x8 hc[4];
+        std::uint32_t mask[5];
+        {
+            f32x16 a0 = {}, a1 = {};
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                a0 = mfma(weightTile(net.w, 0, 0, s, lane), x[s], a0);
+                a1 = mfma(weightTile(net.w, 0, 1, s, lane), x[s], a1);
+            }
+            reluPack(a0, hc[0], hc[1]);
+            reluPack(a1, hc[2], hc[3]);
+        }
+#pragma unroll
+        for (int layer = 0; layer < 5; ++layer) {
+            if (layer > 0) {
+                f32x16 a0 = {}, a1 = {};
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    a0 = mfma(weightTile(net.w, layer, 0, s, lane), hc[s], a0);
+                    a1 = mfma(weightTile(net.w, layer, 1, s, lane), hc[s], a1);
+                }
+                reluPack(a0, hc[0], hc[1]);
+                reluPack(a1, hc[2], hc[3]);
+            }
+            std::uint32_t m = 0;
+#pragma unroll
+            for (int b = 0; b < 32; ++b) m |= (static_cast<float>(hc[b >> 3][b & 7]) > 0.0f ? 1U : 0U) << b;
+            mask[layer] = m;
+            storeUnits(tb, kRcActH + 64 * layer, q, h, hc);
+        }
+        f32x16 y = {};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) y = mfma(weightTile(net.w, 5, 0, s, lane), hc[s], y);
+        // dy on lanes 0..31 (outputs 0..2 are registers 0..2 there), shared
+        // with lanes 32..63 of the same sample.
+        float dy0 = 0.0f, dy1 = 0.0f, dy2 = 0.0f;
+        if (h == 0) {
             const std::uint32_t k = smp.lane < 3U ? smp.lane : 0U;
             const float yk = k == 0 ? y[0] : (k == 1 ? y[1] : y[2]);
             const float den = yk * yk + 0.01f;
             const float diff = yk - smp.target;
             const float gk = valid ? 2.0f * diff / den * invN : 0.0f;
-            dy[q * 3 + 0] = k == 0 ? gk : 0.0f;
-            dy[q * 3 + 1] = k == 1 ? gk : 0.0f;
-            dy[q * 3 + 2] = k == 2 ? gk : 0.0f;
+            dy0 = k == 0 ? gk : 0.0f;
+            dy1 = k == 1 ? gk : 0.0f;
+            dy2 = k == 2 ? gk : 0.0f;
             if (valid) {
                 lossSum += diff * diff / den;
                 lossCount += 1.0f;
             }
-        }
-        __syncthreads();
-        // delta5 = (W5^T dy) * relu'(h5): 3 x 64 product on the vector units.
-        {
-            const __bf16* w5 = net.w + kRcW5;
-            const float d0 = dy[q * 3 + 0], d1 = dy[q * 3 + 1], d2 = dy[q * 3 + 2];
-            for (int i = 0; i < 32; ++i) {
-                const int k = 32 * h + i;
-                float v = static_cast<float>(w5[k]) * d0 + static_cast<float>(w5[64 + k]) * d1 +
-                          static_cast<float>(w5[128 + k]) * d2;
-                if (!(static_cast<float>(hs[4 * 32 * kPH + q * kPH + k]) > 0.0f)) v = 0.0f;
-                ds[4 * 32 * kPH + q * kPH + k] = static_cast<__bf16>(v);
+            const float dv[3] = {dy0, dy1, dy2};
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                const __bf16 hi = static_cast<__bf16>(dv[o]);
+                tb[(kRcActDy + o) * 32 + q] = hi;
+                tb[(kRcActDy + 3 + o) * 32 + q] = static_cast<__bf16>(dv[o] - static_cast<float>(hi));
             }
         }
-        __syncthreads();
-        // delta_L = (W_L^T delta_{L+1}) * relu'(h_L), L = 4..1 (W_L: h_L -> h_{L+1}).
+        dy0 = __shfl(dy0, q);
+        dy1 = __shfl(dy1, q);
+        dy2 = __shfl(dy2, q);
+        // d5 = (W5^T dy) * relu'(h5) in FP32 on the vector units.
+        bf16x8 d[4];
+        {
+            const __bf16* w5 = net.w + kRcW5;
+#pragma unroll
+            for (int b = 0; b < 32; ++b) {
+                const int u = 32 * (b >> 4) + 16 * ((b >> 3) & 1) + 8 * h + (b & 7);
+                const float v = static_cast<float>(w5[u]) * dy0 + static_cast<float>(w5[64 + u]) * dy1 +
+                                static_cast<float>(w5[128 + u]) * dy2;
+                d[b >> 3][b & 7] = static_cast<__bf16>((mask[4] >> b) & 1U ? v : 0.0f);
+            }
+        }
+        storeUnits(tb, kRcActD + 64 * 4, q, h, d);
+        // d_L = (W_L^T d_{L+1}) * relu'(h_L), L = 4..1 (W_L maps h_L to h_{L+1}).
+#pragma unroll
         for (int L = 4; L >= 1; --L) {
             const __bf16* wt = net.wt + 4096 * (L - 1);
-            const __bf16* dnext = ds + L * 32 * kPH;
             f32x16 a0 = {}, a1 = {};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const bf16x8 b = *reinterpret_cast<const bf16x8*>(dnext + q * kPH + 16 * s + 8 * h);
                 const bf16x8 w0 = asBf16x8(*reinterpret_cast<const uint4*>(wt + swap23(q) * 64 + 16 * s + 8 * h));
                 const bf16x8 w1 = asBf16x8(*reinterpret_cast<const uint4*>(wt + (32 + swap23(q)) * 64 + 16 * s + 8 * h));
-                a0 = mfma(w0, b, a0);
-                a1 = mfma(w1, b, a1);
+                a0 = mfma(w0, d[s], a0);
+                a1 = mfma(w1, d[s], a1);
             }
-            const __bf16* hl = hs + (L - 1) * 32 * kPH + q * kPH;
-            __bf16* dl = ds + (L - 1) * 32 * kPH + q * kPH;
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const f32x16& a = t == 0 ? a0 : a1;
-#pragma unroll
-                for (int half = 0; half < 2; ++half) {
-                    const int k0 = 32 * t + 16 * half + 8 * h;
-                    const bf16x8 hv = *reinterpret_cast<const bf16x8*>(hl + k0);
-                    bf16x8 out;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const float v = a[8 * half + e];
-                        out[e] = static_cast<__bf16>(static_cast<float>(hv[e]) > 0.0f ? v : 0.0f);
-                    }
-                    *reinterpret_cast<bf16x8*>(dl + k0) = out;
-                }
+            for (int b = 0; b < 32; ++b) {
+                const float v = (b >> 4) == 0 ? a0[b & 15] : a1[b & 15];
+                d[b >> 3][b & 7] = static_cast<__bf16>((mask[L - 1] >> b) & 1U ? v : 0.0f);
             }
-            __syncthreads();
+            storeUnits(tb, kRcActD + 64 * (L - 1), q, h, d);
         }
-        // Weight gradients dW[o][k] = sum_q delta[q][o] in[q][k] with the 32
-        // samples as the MFMA reduction dimension (2 k-steps of 16).
-        for (int layer = 0; layer <= 4; ++layer) {
-            const __bf16* dsrc = ds + layer * 32 * kPH;                    // delta_{layer+1}
-            const __bf16* isrc = layer == 0 ? h0 : hs + (layer - 1) * 32 * kPH;
-            const int ipitch = layer == 0 ? kP0 : kPH;
-            const int kTiles = layer == 0 ? 4 : 2;
-            float* gl = g + (layer == 0 ? kRcW0 : kRcW1 + 4096 * (layer - 1));
-            const int width = layer == 0 ? kRcIn : kRcHidden;
-            for (int to = 0; to < 2; ++to) {
-                bf16x8 a[2];
-#pragma unroll
-                for (int s = 0; s < 2; ++s)
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) a[s][e] = dsrc[(16 * s + 8 * h + e) * kPH + 32 * to + q];
-                for (int tk = 0; tk < kTiles; ++tk) {
-                    f32x16 acc = {};
-#pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        bf16x8 b;
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) b[e] = isrc[(16 * s + 8 * h + e) * ipitch + 32 * tk + q];
-                        acc = mfma(a[s], b, acc);
-                    }
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int o = 32 * to + 8 * (j >> 2) + 4 * h + (j & 3);
-                        gl[o * width + 32 * tk + q] += acc[j];
-                    }
-                }
-            }
-        }
-        // dW5[o][k] = sum_q dy[q][o] h5[q][k] on the vector units.
-        {
-            const int k = lane;
-            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-            for (int i = 0; i < 32; ++i) {
-                const float hv = static_cast<float>(hs[4 * 32 * kPH + i * kPH + k]);
-                s0 += dy[i * 3 + 0] * hv;
-                s1 += dy[i * 3 + 1] * hv;
-                s2 += dy[i * 3 + 2] * hv;
-            }
-            g[kRcW5 + k] += s0;
-            g[kRcW5 + 64 + k] += s1;
-            g[kRcW5 + 128 + k] += s2;
-        }
-        __syncthreads();
     }
-    // Wave reduction of the loss statistics, one atomic pair per block.
+    // Wave reduction of the loss statistics, one atomic pair per wave.
     for (int off = 32; off > 0; off >>= 1) {
         lossSum += __shfl_xor(lossSum, off);
         lossCount += __shfl_xor(lossCount, off);
@@ -498,15 +562,86 @@ __global__ __launch_bounds__(64) void rcGradKernel(RcNetwork net, const RcQuery*
     }
 }
 
-// Sums the per-block partial gradients of one parameter and clears them.
-__device__ __forceinline__ float reducePartials(float* partial, int blocks, int i)
+// Output tiles of the weight gradient: 24 tiles of W0..W4, then 2 of W5.
+constexpr int kGradTiles = 26;
+
+__global__ __launch_bounds__(256) void rcWeightGradKernel(const __bf16* act, std::uint32_t pitch, float* partial)
+{
+    __shared__ float w5acc[4][6][32];
+    const int lane = threadIdx.x & 63, h = lane >> 5, r = lane & 31, waveInBlock = threadIdx.x >> 6;
+    const int wave = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int tile = wave % kGradTiles, split = wave / kGradTiles;
+    if (split >= kRcGradSplits) return;  // whole waves: no barrier below is shared across waves
+    int aRow, bRow, width, out, oBase, iBase;
+    bool w5 = false;
+    if (tile < 8) {  // W0: 64 x 128
+        oBase = 32 * (tile >> 2);
+        iBase = 32 * (tile & 3);
+        aRow = kRcActD + oBase;
+        bRow = kRcActX + iBase;
+        width = kRcIn;
+        out = kRcW0;
+    } else if (tile < 24) {  // W1..W4: 64 x 64
+        const int l = 1 + (tile - 8) / 4, t = (tile - 8) & 3;
+        oBase = 32 * (t >> 1);
+        iBase = 32 * (t & 1);
+        aRow = kRcActD + 64 * l + oBase;
+        bRow = kRcActH + 64 * (l - 1) + iBase;
+        width = kRcHidden;
+        out = kRcW1 + 4096 * (l - 1);
+    } else {  // W5: 3 x 64 from the hi and lo planes of dy
+        w5 = true;
+        oBase = 0;
+        iBase = 32 * (tile - 24);
+        aRow = kRcActDy;
+        bRow = kRcActH + 64 * 4 + iBase;
+        width = kRcHidden;
+        out = kRcW5;
+    }
+    const std::uint32_t steps = pitch / 16U;
+    const std::uint32_t s0 = steps * static_cast<std::uint32_t>(split) / kRcGradSplits;
+    const std::uint32_t s1 = steps * static_cast<std::uint32_t>(split + 1) / kRcGradSplits;
+    // Step s covers samples 16 s .. 16 s + 15: half (s & 1) of tile s / 2.
+    const bool aValid = !w5 || r < 6;
+    const std::size_t aOff = static_cast<std::size_t>(aRow + (aValid ? r : 0)) * 32 + 8 * h;
+    const std::size_t bOff = static_cast<std::size_t>(bRow + r) * 32 + 8 * h;
+    f32x16 acc = {};
+    for (std::uint32_t s = s0; s < s1; ++s) {
+        const __bf16* tb = act + static_cast<std::size_t>(s >> 1) * (kRcActRows * 32) + 16 * (s & 1U);
+        const bf16x8 a = aValid ? asBf16x8(*reinterpret_cast<const uint4*>(tb + aOff)) : bf16x8{};
+        const bf16x8 b = asBf16x8(*reinterpret_cast<const uint4*>(tb + bOff));
+        acc = mfma(a, b, acc);
+    }
+    float* g = partial + static_cast<std::size_t>(split) * kRcParams + out;
+    if (!w5) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int o = oBase + 8 * (j >> 2) + 4 * h + (j & 3);
+            g[o * width + iBase + r] = acc[j];
+        }
+        return;
+    }
+    // W5: rows 0..2 (hi) and 3..5 (lo) of the tile, combined through LDS
+    // (row 8 (j/4) + 4h + j%4 of register j).
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int o = 8 * (j >> 2) + 4 * h + (j & 3);
+        if (o < 6) w5acc[waveInBlock][o][r] = acc[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (h == 0)
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+            g[o * width + iBase + r] = w5acc[waveInBlock][o][r] + w5acc[waveInBlock][3 + o][r];
+}
+
+// Sums the partial gradient rows of one parameter (in row order).
+__device__ __forceinline__ float reducePartials(const float* partial, int blocks, int i)
 {
     float gsum = 0.0f;
-    for (int b = 0; b < blocks; ++b) {
-        float* p = partial + static_cast<std::size_t>(b) * kRcParams + i;
-        gsum += *p;
-        *p = 0.0f;
-    }
+    for (int b = 0; b < blocks; ++b) gsum += partial[static_cast<std::size_t>(b) * kRcParams + i];
     return gsum;
 }
 
@@ -541,7 +676,7 @@ __global__ void rcAdamKernel(RcState s, int blocks, int step, float lr, float be
     }
 }
 
-__global__ void rcReduceKernel(float* partial, int blocks, float* out)
+__global__ void rcReduceKernel(const float* partial, int blocks, float* out)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kRcParams; i += gridDim.x * blockDim.x)
         out[i] = reducePartials(partial, blocks, i);
@@ -577,17 +712,26 @@ void InitRcWeights(std::uint64_t seed, float* w)
 
 hipError_t LaunchRcInfer(const RcNetwork& net, const RcQuery* queries, const std::uint32_t* count,
                          std::uint32_t maxCount, float* out, float* radiance, const float* throughput,
-                         unsigned maxBlocks, hipStream_t st)
+                         unsigned maxBlocks, hipStream_t st, const float* calibration)
 {
     const unsigned tiles = (maxCount + 31U) / 32U;
     unsigned blocks = (tiles + 3U) / 4U;
     blocks = blocks == 0 ? 1U : (blocks < maxBlocks ? blocks : maxBlocks);
     if (out)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(rcInferKernel<true>), dim3(blocks), dim3(kInferBlock), 0, st, net, queries,
-                           count, maxCount, out, radiance, throughput);
+                           count, maxCount, out, radiance, throughput, calibration);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(rcInferKernel<false>), dim3(blocks), dim3(kInferBlock), 0, st, net, queries,
-                           count, maxCount, out, radiance, throughput);
+                           count, maxCount, out, radiance, throughput, calibration);
+    return hipGetLastError();
+}
+
+hipError_t LaunchRcCalibrate(const RcQuery* samples, std::uint32_t count, const float* outputs, RcCalibration* state,
+                             float decay, hipStream_t st)
+{
+    const unsigned blocks = count == 0 ? 1U : ((count + 255U) / 256U < 256U ? (count + 255U) / 256U : 256U);
+    hipLaunchKernelGGL(rcCalibrateSumKernel, dim3(blocks), dim3(256), 0, st, samples, count, outputs, state);
+    hipLaunchKernelGGL(rcCalibrateFinishKernel, dim3(1), dim3(1), 0, st, state, decay);
     return hipGetLastError();
 }
 
@@ -609,12 +753,15 @@ hipError_t LaunchRcInferScenes(const __bf16* weights, const RcQuery* queries, co
     return hipGetLastError();
 }
 
-hipError_t LaunchRcGradient(const RcNetwork& net, const RcQuery* samples, const std::uint32_t* count,
-                            std::uint32_t maxCount, float* grad, float* loss, unsigned maxBlocks, hipStream_t st)
+hipError_t LaunchRcGradient(const RcNetwork& net, const RcQuery* samples, std::uint32_t count, __bf16* workspace,
+                            float* partial, float* loss, hipStream_t st)
 {
-    const unsigned tiles = (maxCount + 31U) / 32U;
-    const unsigned blocks = tiles == 0 ? 1U : (tiles < maxBlocks ? tiles : maxBlocks);
-    hipLaunchKernelGGL(rcGradKernel, dim3(blocks), dim3(64), 0, st, net, samples, count, maxCount, grad, loss);
+    const std::uint32_t pitch = RcGradPitch(count);
+    const unsigned tiles = pitch / 32U;
+    const unsigned blocks = tiles == 0 ? 1U : (tiles + 3U) / 4U;
+    hipLaunchKernelGGL(rcForwardBackwardKernel, dim3(blocks), dim3(256), 0, st, net, samples, count, pitch, workspace, loss);
+    const unsigned waves = kGradTiles * kRcGradSplits;
+    hipLaunchKernelGGL(rcWeightGradKernel, dim3((waves + 3U) / 4U), dim3(256), 0, st, workspace, pitch, partial);
     return hipGetLastError();
 }
 
@@ -626,7 +773,7 @@ hipError_t LaunchRcAdamBlocks(const RcState& s, int blocks, int step, float lr, 
     return hipGetLastError();
 }
 
-hipError_t LaunchRcReduce(float* partial, int blocks, float* out, hipStream_t st)
+hipError_t LaunchRcReduce(const float* partial, int blocks, float* out, hipStream_t st)
 {
     hipLaunchKernelGGL(rcReduceKernel, dim3((kRcParams + 255) / 256), dim3(256), 0, st, partial, blocks, out);
     return hipGetLastError();

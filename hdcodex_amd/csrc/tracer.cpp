@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -57,19 +58,19 @@ struct hdcx_tracer : SceneStore, PathPool {
     // Ray queries.
     DeviceBuffer queryRays, queryHits;
     // Radiance cache.
-    static constexpr unsigned kRcGradBlocks = 256;
     hdcx_radiance_cache_config rcConfig{};
     bool rcEnabled = false;
     bool rcInitialised = false;
     std::uint64_t rcSeed = 0;
     int rcStep = 0;
     hdcx_rc_stats rcStats{};
-    DeviceBuffer rcMaster, rcM, rcV, rcGrad, rcW, rcWt, rcLoss, rcQueries, rcOut, rcEma, rcWe, rcWte;
-    DeviceBuffer rcChunkQueries, rcCounters, rcRecords, rcRecordThroughput, rcSamples;
+    DeviceBuffer rcMaster, rcM, rcV, rcGrad, rcW, rcWt, rcLoss, rcQueries, rcOut, rcEma, rcWe, rcWte, rcAct;
+    DeviceBuffer rcChunkQueries, rcCounters, rcRecords, rcRecordThroughput, rcSamples, rcCalib, rcCalibOut, rcRecordOut;
     std::uint32_t rcFrame = 0;
     bool rcCountersReady = false;
     // Instrumentation.
     std::uint32_t instrumentation = 0;
+    std::uint32_t sampleOffset = 0;  // added to sample numbers in the random seeds only
     hdcx_kernel_stats stats{};
     std::vector<hipEvent_t> eventPool;
     struct Timed { int kernel; hipEvent_t a, b; };
@@ -139,6 +140,14 @@ struct hdcx_tracer : SceneStore, PathPool {
         return r;
     }
 
+    // Device address of the brightness calibration factor, or null when
+    // calibration is off.
+    const float* CalibrationScale()
+    {
+        if (!(rcConfig.calibration_decay > 0.0F) || !rcCalib.bytes()) return nullptr;
+        return reinterpret_cast<const float*>(rcCalib.as<char>() + offsetof(RcCalibration, scale));
+    }
+
     // Weights of the training passes, and the (averaged) inference weights.
     RcNetwork RcNet() { return RcNetwork{rcW.as<__bf16>(), rcWt.as<__bf16>()}; }
     RcNetwork RcInferNet() { return RcNetwork{rcWe.as<__bf16>(), rcWte.as<__bf16>()}; }
@@ -153,13 +162,16 @@ struct hdcx_tracer : SceneStore, PathPool {
         rcMaster.Allocate(kRcParams * sizeof(float));
         rcM.Allocate(kRcParams * sizeof(float));
         rcV.Allocate(kRcParams * sizeof(float));
-        rcGrad.Allocate(static_cast<std::size_t>(kRcGradBlocks) * kRcParams * sizeof(float));
+        rcGrad.Allocate(static_cast<std::size_t>(kRcGradSplits) * kRcParams * sizeof(float));
         rcW.Allocate(kRcParams * sizeof(std::uint16_t));
         rcWt.Allocate(4 * 4096 * sizeof(std::uint16_t));
         rcEma.Allocate(kRcParams * sizeof(float));
         rcWe.Allocate(kRcParams * sizeof(std::uint16_t));
         rcWte.Allocate(4 * 4096 * sizeof(std::uint16_t));
         rcLoss.Allocate(2 * sizeof(float));
+        rcCalib.Allocate(sizeof(RcCalibration));
+        const RcCalibration calib{{0.0, 0.0}, {0.0, 0.0}, 1.0F, {0.0F, 0.0F, 0.0F}};
+        Check(hipMemcpyAsync(rcCalib.as<void>(), &calib, sizeof(calib), hipMemcpyHostToDevice, stream), "rc init");
         std::vector<float> w(kRcParams);
         InitRcWeights(seed, w.data());
         Check(hipMemcpyAsync(rcMaster.as<void>(), w.data(), kRcParams * sizeof(float), hipMemcpyHostToDevice, stream), "rc init");
@@ -178,6 +190,7 @@ struct hdcx_tracer : SceneStore, PathPool {
     {
         const hdcx_radiance_cache_config& c = rcConfig;
         steps = std::max(steps, 1U);
+        rcAct.Allocate(std::max(rcAct.bytes(), RcGradWorkspaceBytes(count / steps + 1U)));
         float lossHost[2] = {0.0F, 0.0F};
         for (std::uint32_t i = 0; i < steps; ++i) {
             const std::uint32_t b0 = static_cast<std::uint32_t>(static_cast<std::uint64_t>(count) * i / steps);
@@ -185,10 +198,10 @@ struct hdcx_tracer : SceneStore, PathPool {
             if (b1 <= b0) continue;
             (void)deviceCount;
             Check(hipMemsetAsync(rcLoss.as<void>(), 0, 2 * sizeof(float), stream), "rc loss");
-            Check(LaunchRcGradient(RcNet(), samples + b0, nullptr, b1 - b0, rcGrad.as<float>(), rcLoss.as<float>(),
-                                   kRcGradBlocks, stream),
+            Check(LaunchRcGradient(RcNet(), samples + b0, b1 - b0, rcAct.as<__bf16>(), rcGrad.as<float>(),
+                                   rcLoss.as<float>(), stream),
                   "rc gradient");
-            Check(LaunchRcAdamBlocks(Rc(), static_cast<int>(kRcGradBlocks), ++rcStep, c.learning_rate, c.beta1, c.beta2,
+            Check(LaunchRcAdamBlocks(Rc(), kRcGradSplits, ++rcStep, c.learning_rate, c.beta1, c.beta2,
                                      c.epsilon, stream),
                   "rc adam");
             ++rcStats.adam_steps;
@@ -303,7 +316,7 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     ChunkDesc c{};
     c.width = w;
     c.height = h;
-    c.sampleBase = sampleIndex;
+    c.sampleBase = sampleIndex + t->sampleOffset;
     c.maxBounces = bounces;
     std::memcpy(c.cam + 0, cam.origin, 12);
     for (int k = 0; k < 3; ++k)
@@ -344,7 +357,7 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                     // Queries of this bounce: answered, then the queue is emptied.
                     t->Timed(HDCX_KERNEL_CACHE, [&] {
                         return LaunchRcInfer(t->RcInferNet(), cp.queries, cp.queryCount, paths, nullptr, pb.radiance, pb.throughput,
-                                             t->maxBlocks, st);
+                                             t->maxBlocks, st, t->CalibrationScale());
                     });
                     Check(LaunchConsumeQueries(cp, st), "rc queries");
                 }
@@ -354,7 +367,21 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
             }
             if (cache) {
                 const std::uint32_t records = cp.recordSlots * 3U * kRecordVertices;
-                Check(LaunchGatherTraining(pb, cp, records, t->maxBlocks, st), "gather training samples");
+                // Training paths show the cached value in the image too
+                // (their plain rest of the path only trains the cache),
+                // unless asked to keep their plain radiance or the stop
+                // vertex depends on the spread rule.
+                const bool cachedImage = !(t->rcConfig.flags & HDCX_RC_PLAIN_TRAINING_PATHS) && !(cp.spreadThreshold > 0.0F);
+                const float* outputs = nullptr;
+                if (cachedImage) {
+                    t->rcRecordOut.Reserve(static_cast<std::size_t>(records) * 3 * sizeof(float));
+                    Check(LaunchRcInfer(t->RcInferNet(), cp.records, nullptr, records, t->rcRecordOut.as<float>(), nullptr,
+                                        nullptr, t->maxBlocks, st),
+                          "rc training-path answers");
+                    outputs = t->rcRecordOut.as<float>();
+                }
+                Check(LaunchGatherTraining(pb, cp, records, t->maxBlocks, st, outputs, t->CalibrationScale()),
+                      "gather training samples");
             }
             t->Timed(HDCX_KERNEL_FILM, [&] { return LaunchFilm(pb, c, t->filmBatch.as<double4>(), t->maxBlocks, st); });
             t->stats.rays[HDCX_KERNEL_FILM] += (p1 - p0);
@@ -378,7 +405,21 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
             b = t->NextEvent();
             Check(hipEventRecord(a, st), "hipEventRecord");
         }
-        if (n > 0) t->RcTrain(cp.samples, nullptr, n, t->rcConfig.train_steps);
+        if (n > 0) {
+            t->RcTrain(cp.samples, nullptr, n, t->rcConfig.train_steps);
+            if (t->CalibrationScale()) {
+                // The updated inference network on this frame's samples
+                // against their plain-estimator targets.
+                t->rcCalibOut.Allocate(std::max<std::size_t>(t->rcCalibOut.bytes(), static_cast<std::size_t>(n) * 3 * sizeof(float)));
+                Check(LaunchRcInfer(t->RcInferNet(), cp.samples, nullptr, n, t->rcCalibOut.as<float>(), nullptr, nullptr,
+                                    t->maxBlocks, st),
+                      "rc calibration outputs");
+                Check(LaunchRcCalibrate(cp.samples, n, t->rcCalibOut.as<float>(), t->rcCalib.as<RcCalibration>(),
+                                        t->rcConfig.calibration_decay, st),
+                      "rc calibration");
+                Check(hipStreamSynchronize(st), "rc calibration");
+            }
+        }
         if (t->instrumentation & 1U) {
             Check(hipEventRecord(b, st), "hipEventRecord");
             Check(hipEventSynchronize(b), "hipEventSynchronize");
@@ -494,6 +535,14 @@ int hdcx_set_scene(hdcx_tracer* tracer, const hdcx_scene_view* scene)
     return Guard([&] {
         if (!tracer) throw std::invalid_argument("tracer is null");
         SetScene(tracer, scene);
+    });
+}
+
+int hdcx_set_sample_offset(hdcx_tracer* tracer, uint32_t offset)
+{
+    return Guard([&] {
+        if (!tracer) throw std::invalid_argument("tracer is null");
+        tracer->sampleOffset = offset;
     });
 }
 
@@ -748,11 +797,11 @@ int hdcx_rc_gradient(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t
         tracer->rcQueries.Upload(samples, static_cast<std::size_t>(count) * sizeof(RcQuery), st);
         tracer->rcOut.Allocate(std::max<std::size_t>(static_cast<std::size_t>(count) * 3, kRcParams) * sizeof(float));
         Check(hipMemsetAsync(tracer->rcLoss.as<void>(), 0, 2 * sizeof(float), st), "rc loss");
-        Check(LaunchRcGradient(tracer->RcNet(), tracer->rcQueries.as<RcQuery>(), nullptr, count, tracer->rcGrad.as<float>(),
-                               tracer->rcLoss.as<float>(), hdcx_tracer::kRcGradBlocks, st),
+        tracer->rcAct.Allocate(std::max(tracer->rcAct.bytes(), RcGradWorkspaceBytes(count)));
+        Check(LaunchRcGradient(tracer->RcNet(), tracer->rcQueries.as<RcQuery>(), count, tracer->rcAct.as<__bf16>(),
+                               tracer->rcGrad.as<float>(), tracer->rcLoss.as<float>(), st),
               "rc gradient");
-        Check(LaunchRcReduce(tracer->rcGrad.as<float>(), static_cast<int>(hdcx_tracer::kRcGradBlocks), tracer->rcOut.as<float>(), st),
-              "rc reduce");
+        Check(LaunchRcReduce(tracer->rcGrad.as<float>(), kRcGradSplits, tracer->rcOut.as<float>(), st), "rc reduce");
         float loss[2];
         Check(hipMemcpyAsync(out_grad, tracer->rcOut.as<void>(), kRcParams * sizeof(float), hipMemcpyDeviceToHost, st), "rc grad");
         Check(hipMemcpyAsync(loss, tracer->rcLoss.as<void>(), sizeof(loss), hipMemcpyDeviceToHost, st), "rc grad");
@@ -788,6 +837,9 @@ int hdcx_rc_get_stats(hdcx_tracer* tracer, hdcx_rc_stats* out_stats)
                   "rc stats");
             tracer->rcStats.queries = queries;
         }
+        tracer->rcStats.calibration_scale = 1.0F;
+        if (const float* scale = tracer->CalibrationScale())
+            Check(hipMemcpy(&tracer->rcStats.calibration_scale, scale, sizeof(float), hipMemcpyDeviceToHost), "rc stats");
         *out_stats = tracer->rcStats;
     });
 }

@@ -175,6 +175,12 @@ int hdcx_render(hdcx_tracer* tracer, const hdcx_camera* camera,
                 uint32_t width, uint32_t height, uint32_t sample_index,
                 uint32_t max_bounces, uint32_t sample_count, float* out_rgba);
 
+/* Independent sample streams (no counterpart in the reference, whose random
+ * seeds are fixed by the pixel and the sample number, :1198-1207): sample
+ * number n of hdcx_render is seeded as n + offset, while the progressive
+ * mean still counts samples from 0. Default 0, the reference's streams. */
+int hdcx_set_sample_offset(hdcx_tracer* tracer, uint32_t offset);
+
 /* Copies the current device film (RGBA32F, width*height*4) to out_rgba. */
 int hdcx_read_film(hdcx_tracer* tracer, float* out_rgba);
 
@@ -361,7 +367,19 @@ typedef struct hdcx_radiance_cache_config {
                                 * times its primary footprint (Mueller et al. 2021);
                                 * <= 0: at terminate_bounce unconditionally */
     uint64_t seed;             /* weight initialisation */
+    float calibration_decay;   /* in (0, 1): cached radiance is scaled by the ratio of the
+                                * training targets' sum to the network's outputs on the same
+                                * samples, summed over frames with this decay per frame
+                                * (the plain estimator calibrates the cache's brightness);
+                                * 0 = off (default). hdcx_render only; hdcx_rc_infer
+                                * returns the raw network outputs */
+    uint32_t flags;            /* HDCX_RC_* bits */
 } hdcx_radiance_cache_config;
+
+/* Training pixel samples keep their plain radiance in the image (by default
+ * they show the cached value at the terminate vertex, like every other
+ * path, and their plain rest of the path only trains the cache). */
+#define HDCX_RC_PLAIN_TRAINING_PATHS 1u
 
 typedef struct hdcx_rc_stats {
     uint64_t frames;
@@ -369,7 +387,7 @@ typedef struct hdcx_rc_stats {
     uint64_t train_samples;    /* samples trained on */
     uint64_t adam_steps;
     float last_loss;           /* mean relative L2 loss of the last step */
-    float reserved;
+    float calibration_scale;   /* current brightness calibration factor (1 when off) */
     double infer_ms;           /* HIP-event time of inference launches (when timing is on) */
     double train_ms;
 } hdcx_rc_stats;

