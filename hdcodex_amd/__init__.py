@@ -374,6 +374,7 @@ def _bind_cache(lib: ctypes.CDLL) -> None:
     lib.hdcx_rc_infer.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p]
     lib.hdcx_rc_gradient.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p, POINTER(c_float)]
     lib.hdcx_rc_train.argtypes = [c_void_p, c_void_p, c_uint32, c_uint32, POINTER(c_float)]
+    lib.hdcx_rc_gradient_noalloc.argtypes = [c_void_p, c_void_p, c_uint32, c_void_p, c_void_p]
     lib.hdcx_rc_get_stats.argtypes = [c_void_p, POINTER(RcStats)]
 
 
@@ -414,6 +415,12 @@ def _tracer_rc_gradient(self, samples: np.ndarray):
     return float(loss.value), g
 
 
+def _tracer_rc_gradient_noalloc(self, samples_ptr: int, count: int, grad_ptr: int, loss_ptr: int = 0) -> None:
+    """hdcx_rc_gradient_noalloc on device pointers (samples: count x 48
+    bytes, grad: RC_PARAMS floats, loss: one float or 0)."""
+    _check(library().hdcx_rc_gradient_noalloc(self._h, samples_ptr, count, grad_ptr, loss_ptr or None))
+
+
 def _tracer_rc_train(self, samples: np.ndarray, steps: int = 1) -> float:
     q = np.ascontiguousarray(samples, dtype=RC_QUERY_DTYPE)
     loss = c_float()
@@ -433,10 +440,11 @@ PathTracer.rc_set_params = _tracer_rc_set_params
 PathTracer.rc_infer = _tracer_rc_infer
 PathTracer.rc_gradient = _tracer_rc_gradient
 PathTracer.rc_train = _tracer_rc_train
+PathTracer.rc_gradient_noalloc = _tracer_rc_gradient_noalloc
 PathTracer.rc_stats = _tracer_rc_stats
 EXPORTED_SYMBOLS = EXPORTED_SYMBOLS + (
     "hdcx_set_radiance_cache", "hdcx_rc_get_params", "hdcx_rc_set_params", "hdcx_rc_infer", "hdcx_rc_gradient",
-    "hdcx_rc_train", "hdcx_rc_get_stats")
+    "hdcx_rc_train", "hdcx_rc_get_stats", "hdcx_rc_gradient_noalloc")
 
 
 # ---- BVH builders and the tracer's BVH policy ----------------------------

@@ -111,8 +111,8 @@ constexpr int kRcActDy = 128 + 10 * 64;
 constexpr int kRcActRows = kRcActDy + 6;
 // Partial gradient rows written by LaunchRcGradient (the Adam step sums them).
 constexpr int kRcGradSplits = 16;
-inline std::uint32_t RcGradPitch(std::uint32_t count) { return (count + 31U) / 32U * 32U; }
-inline std::size_t RcGradWorkspaceBytes(std::uint32_t count)
+constexpr std::uint32_t RcGradPitch(std::uint32_t count) { return (count + 31U) / 32U * 32U; }
+constexpr std::size_t RcGradWorkspaceBytes(std::uint32_t count)
 {
     return static_cast<std::size_t>(kRcActRows) * RcGradPitch(count) * sizeof(std::uint16_t);
 }
@@ -130,6 +130,10 @@ hipError_t LaunchRcAdamBlocks(const RcState& s, int blocks, int step, float lr, 
                               hipStream_t st);
 // out[i] = sum of the partial rows.
 hipError_t LaunchRcReduce(const float* partial, int blocks, float* out, hipStream_t st);
+// loss[0] = loss[1] = 0 (a kernel, not a memset).
+hipError_t LaunchRcClearLoss(float* loss, hipStream_t st);
+// *out = loss[0] / loss[1] (0 without samples).
+hipError_t LaunchRcMeanLoss(const float* loss, float* out, hipStream_t st);
 // Resets the average to the FP32 master weights and publishes both BF16 copies.
 hipError_t LaunchRcPublish(const RcState& s, hipStream_t st);
 

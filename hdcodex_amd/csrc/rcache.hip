@@ -682,6 +682,14 @@ __global__ void rcReduceKernel(const float* partial, int blocks, float* out)
         out[i] = reducePartials(partial, blocks, i);
 }
 
+__global__ void rcClearLossKernel(float* loss)
+{
+    loss[0] = 0.0f;
+    loss[1] = 0.0f;
+}
+
+__global__ void rcMeanLossKernel(const float* loss, float* out) { *out = loss[1] > 0.0f ? loss[0] / loss[1] : 0.0f; }
+
 __global__ void rcPublishKernel(RcState s)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kRcParams; i += gridDim.x * blockDim.x) {
@@ -776,6 +784,18 @@ hipError_t LaunchRcAdamBlocks(const RcState& s, int blocks, int step, float lr, 
 hipError_t LaunchRcReduce(const float* partial, int blocks, float* out, hipStream_t st)
 {
     hipLaunchKernelGGL(rcReduceKernel, dim3((kRcParams + 255) / 256), dim3(256), 0, st, partial, blocks, out);
+    return hipGetLastError();
+}
+
+hipError_t LaunchRcClearLoss(float* loss, hipStream_t st)
+{
+    hipLaunchKernelGGL(rcClearLossKernel, dim3(1), dim3(1), 0, st, loss);
+    return hipGetLastError();
+}
+
+hipError_t LaunchRcMeanLoss(const float* loss, float* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(rcMeanLossKernel, dim3(1), dim3(1), 0, st, loss, out);
     return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -807,6 +808,67 @@ int hdcx_rc_gradient(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t
         Check(hipMemcpyAsync(loss, tracer->rcLoss.as<void>(), sizeof(loss), hipMemcpyDeviceToHost, st), "rc grad");
         Check(hipStreamSynchronize(st), "rc gradient");
         if (out_loss) *out_loss = loss[1] > 0.0F ? loss[0] / loss[1] : 0.0F;
+    });
+}
+
+} // extern "C"
+
+// ---- Scratch of hdcx_rc_gradient_noalloc ----
+// The training workspace for up to kNoAllocMaxSamples samples, the
+// partial gradient rows and the loss sums live in one file-scope device
+// array (every GPU the library runs on holds its own copy for the life of
+// the process). Host code never passes `scratch` itself to a kernel: its
+// address on a device comes from hipGetSymbolAddress, taken once per
+// device with that device current.
+namespace {
+constexpr std::uint32_t kNoAllocMaxSamples = 4096;
+constexpr std::size_t kNoAllocActBytes = hdcx::RcGradWorkspaceBytes(kNoAllocMaxSamples);
+constexpr std::size_t kNoAllocPartialBytes = static_cast<std::size_t>(hdcx::kRcGradSplits) * hdcx::kRcParams * sizeof(float);
+constexpr std::size_t kNoAllocScratchBytes = kNoAllocActBytes + kNoAllocPartialBytes + 256;  // + the loss sums
+static_assert(kNoAllocActBytes % 256 == 0 && kNoAllocPartialBytes % 256 == 0, "aligned scratch regions");
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex gScratchMutex;
+void* gScratchAddress[64] = {};
+
+// Device address of `scratch` on the current device `device`.
+unsigned char* ScratchAddress(int device)
+{
+    if (device < 0 || device >= 64) throw std::invalid_argument("device ordinal out of range");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchAddress[device]) {
+        void* ptr = nullptr;
+        Check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch address");
+        gScratchAddress[device] = ptr;
+    }
+    return static_cast<unsigned char*>(gScratchAddress[device]);
+}
+} // namespace
+
+extern "C" {
+
+int hdcx_rc_gradient_noalloc(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, float* out_grad,
+                             float* out_loss)
+{
+    return Guard([&] {
+        if (!tracer || !samples || !out_grad) throw std::invalid_argument("null argument");
+        if (count == 0 || count > kNoAllocMaxSamples)
+            throw std::invalid_argument("hdcx_rc_gradient_noalloc takes 1 to 4096 samples per call (its scratch size)");
+        if (!tracer->rcInitialised)
+            throw std::invalid_argument("the radiance cache has no network yet: call hdcx_set_radiance_cache first");
+        tracer->Activate();
+        unsigned char* base = ScratchAddress(tracer->device);
+        auto* act = reinterpret_cast<__bf16*>(base);
+        auto* partial = reinterpret_cast<float*>(base + kNoAllocActBytes);
+        auto* loss = reinterpret_cast<float*>(base + kNoAllocActBytes + kNoAllocPartialBytes);
+        hipStream_t st = tracer->stream;
+        Check(LaunchRcClearLoss(loss, st), "rc loss");
+        Check(LaunchRcGradient(tracer->RcNet(), reinterpret_cast<const RcQuery*>(samples), count, act, partial, loss, st),
+              "rc gradient");
+        Check(LaunchRcReduce(partial, kRcGradSplits, out_grad, st), "rc reduce");
+        if (out_loss) Check(LaunchRcMeanLoss(loss, out_loss, st), "rc loss");
+        Check(hipStreamSynchronize(st), "rc gradient");
     });
 }
 

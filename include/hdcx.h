@@ -405,6 +405,18 @@ int hdcx_rc_infer(hdcx_tracer* tracer, const hdcx_rc_query* queries, uint32_t co
  * update; out_loss receives the mean loss. */
 int hdcx_rc_gradient(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, float* out_grad,
                      float* out_loss);
+/* hdcx_rc_gradient for callers that cannot allocate (code under HIP graph
+ * capture, tight serving or training loops, harnesses that forbid
+ * hipMalloc); no counterpart in the reference. samples, out_grad
+ * (HDCX_RC_PARAMS floats) and out_loss (one float, may be NULL) are DEVICE
+ * pointers of the tracer's device. The training workspace lives in a static
+ * device array of about 8 MB that each GPU holds until the process ends, so
+ * a call takes at most 4096 samples (more: HDCX_ERR_INVALID_ARGUMENT); the
+ * cache must already have a network (hdcx_set_radiance_cache). Calls on one
+ * GPU share that array: calls that overlap on one GPU (several threads or
+ * tracers) race. Synchronous, like hdcx_rc_gradient. */
+int hdcx_rc_gradient_noalloc(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, float* out_grad,
+                             float* out_loss);
 /* `steps` Adam steps over consecutive mini-batches of the samples. */
 int hdcx_rc_train(hdcx_tracer* tracer, const hdcx_rc_query* samples, uint32_t count, uint32_t steps,
                   float* out_loss);
