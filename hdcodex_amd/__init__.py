@@ -31,7 +31,7 @@ SHADING_FUSED = 0
 SHADING_MODULAR = 1
 SHADING_RASTER_PREVIEW = 2
 
-KERNELS = ("raygen", "extend", "shade", "shadow", "film", "cache", "walk", "pair")
+KERNELS = ("raygen", "extend", "shade", "shadow", "film", "cache", "walk", "pair", "miss")
 
 # Every symbol include/hdcx.h declares.
 EXPORTED_SYMBOLS = (
@@ -39,6 +39,7 @@ EXPORTED_SYMBOLS = (
     "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_set_sample_offset", "hdcx_has_geometry", "hdcx_trace_closest",
     "hdcx_trace_occluded", "hdcx_scene_create_procedural", "hdcx_scene_view_get", "hdcx_scene_camera_get",
     "hdcx_scene_destroy", "hdcx_scene_flatten", "hdcx_bvh_build_host", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
+    "hdcx_read_emissive_table", "hdcx_read_dome_importance",
 )
 
 
@@ -75,12 +76,13 @@ class SceneView(ctypes.Structure):
         ("textures", POINTER(TextureView)),
         ("prototype_count", c_uint32), ("prototypes", c_void_p),
         ("instance_count", c_uint32), ("instances", c_void_p),
+        ("material_emission_power", POINTER(c_float)), ("light_triangle_count", c_uint32), ("reserved", c_uint32),
     ]
 
 
 class KernelStats(ctypes.Structure):
     _fields_ = [
-        ("launches", c_uint64 * 8), ("milliseconds", c_double * 8), ("rays", c_uint64 * 8),
+        ("launches", c_uint64 * len(KERNELS)), ("milliseconds", c_double * len(KERNELS)), ("rays", c_uint64 * len(KERNELS)),
         ("nodes_visited", c_uint64), ("triangles_tested", c_uint64), ("extend_rays_counted", c_uint64),
         ("shadow_rays_counted", c_uint64), ("shadow_nodes_visited", c_uint64), ("shadow_triangles_tested", c_uint64),
         ("node_bytes", c_uint64), ("triangle_bytes", c_uint64), ("bvh_nodes", c_uint64), ("bvh_triangles", c_uint64),
@@ -144,6 +146,9 @@ def library() -> ctypes.CDLL:
     lib.hdcx_bvh_build_host.argtypes = [POINTER(SceneView), POINTER(BvhInfo), c_void_p, c_void_p]
     lib.hdcx_set_instrumentation.argtypes = [c_void_p, c_uint32]
     lib.hdcx_get_kernel_stats.argtypes = [c_void_p, POINTER(KernelStats)]
+    lib.hdcx_read_emissive_table.argtypes = [c_void_p, c_void_p, c_uint32]
+    lib.hdcx_read_dome_importance.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_void_p,
+                                              c_void_p, c_void_p]
     lib.hdcx_reset_kernel_stats.argtypes = [c_void_p]
     _bind_cache(lib)
     _bind_bvh(lib)
@@ -203,7 +208,11 @@ class ProceduralScene:
             "triangle_materials": np.ctypeslib.as_array(v.triangle_materials, shape=(t,)).copy(),
             "texcoords": np.ctypeslib.as_array(v.texcoords, shape=(t, 3, 2)).copy(),
             "normals": np.ctypeslib.as_array(v.normals, shape=(t, 3, 4)).copy(),
-            "emissive": np.ctypeslib.as_array(v.emissive, shape=(t, 4)).copy(),
+            # None: the emissive-triangle table is built on the device.
+            "emissive": np.ctypeslib.as_array(v.emissive, shape=(t, 4)).copy() if v.emissive else None,
+            "material_emission_power": (np.ctypeslib.as_array(v.material_emission_power, shape=(v.material_count,)).copy()
+                                        if v.material_emission_power else None),
+            "light_triangle_count": int(v.light_triangle_count),
             "materials": np.frombuffer(ctypes.string_at(v.materials, 368 * v.material_count), dtype=np.uint8)
             .reshape(v.material_count, 368).copy(),
             "lights": np.frombuffer(ctypes.string_at(v.lights, 224 * max(v.light_count, 1)), dtype=np.uint8)
@@ -307,6 +316,28 @@ class PathTracer:
         out = np.empty((height, width, 4), dtype=np.float32)
         _check(library().hdcx_read_film(self._h, out.ctypes.data))
         return out
+
+    def emissive_table(self, triangle_count: int) -> np.ndarray:
+        """The emissive-triangle table the device samples: (n, 4) float32
+        rows (cdf, pdf / area, area, 0)."""
+        out = np.empty((triangle_count, 4), dtype=np.float32)
+        _check(library().hdcx_read_emissive_table(self._h, out.ctypes.data, triangle_count))
+        return out
+
+    def dome_importance(self, light: int):
+        """(marginal[h], conditional[h, w], pdf[h, w]) of a dome light, or
+        None when it is sampled uniformly."""
+        w, h = c_uint32(), c_uint32()
+        lib = library()
+        _check(lib.hdcx_read_dome_importance(self._h, light, ctypes.byref(w), ctypes.byref(h), None, None, None))
+        if w.value == 0:
+            return None
+        marginal = np.empty(h.value, dtype=np.float32)
+        conditional = np.empty((h.value, w.value), dtype=np.float32)
+        pdf = np.empty((h.value, w.value), dtype=np.float32)
+        _check(lib.hdcx_read_dome_importance(self._h, light, ctypes.byref(w), ctypes.byref(h), marginal.ctypes.data,
+                                             conditional.ctypes.data, pdf.ctypes.data))
+        return marginal, conditional, pdf
 
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
         """rays: (N, 8) float32 -> structured hits (t, primitive, u, v)."""

@@ -48,7 +48,11 @@ HIP_FLAGS = [
 
 HOST_SOURCES = ["packer.cpp", "procedural.cpp", "procedural_showcase.cpp", "procedural_mixed.cpp", "bvh_host.cpp",
                 "lbvh_host.cpp"]
-HIP_SOURCES = ["kernels.hip", "rcache.hip", "lbvh.hip", "scene_store.cpp", "tracer.cpp", "batch.cpp"]
+HIP_SOURCES = ["kernels.hip", "shade.hip", "rcache.hip", "lbvh.hip", "tables.hip", "scene_store.cpp", "tracer.cpp",
+               "batch.cpp"]
+# The shading stages divide and take square roots at the hardware rate
+# (within 2.5 ulp); the other files keep IEEE division (bit-exact hits).
+EXTRA_FLAGS = {"shade.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 
 
 def _run(cmd: list[str]) -> None:
@@ -69,7 +73,7 @@ def build_library() -> Path:
     for src in HIP_SOURCES:
         obj = BUILD / (src + ".o")
         objects.append(obj)
-        jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)])
     with futures.ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as pool:
         list(pool.map(_run, jobs))
     tmp = LIB.with_suffix(".so.tmp")

@@ -388,12 +388,13 @@ struct hdcx_batch : PathPool {
         bw.wavePixels = wavePixels.as<std::uint32_t>();
         bw.cursor = cursor.as<std::uint32_t>();
         const DeviceScene* table = sceneTable.as<DeviceScene>();
-        bool twoLevel = false, subsurface = false;
+        bool twoLevel = false, subsurface = false, simple = true;
         for (const DeviceScene& d : sceneHost) {
             twoLevel = twoLevel || d.instanceCount > 0;
             subsurface = subsurface || d.hasSubsurface;
+            simple = simple && d.simpleMaterials;
         }
-        const int walk = subsurface ? kShadeWalkDefer : kShadeWalkInline;
+        const int walk = simple ? kShadeSimple : subsurface ? kShadeWalkDefer : kShadeWalkInline;
         const PathBuffers pb = Paths();
         const std::uint32_t maxPaths = plan.maxPaths, maxPixels = plan.maxPixels;
         const bool cache = rcEnabled;
@@ -407,6 +408,7 @@ struct hdcx_batch : PathPool {
                                         nullptr, st),
                       "batch extend");
                 const int last = b + 1 == bounces ? 1 : 0;
+                Check(LaunchMissBatch(table, pb, in, maxPaths, maxBlocks, st), "batch miss");
                 Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, walk, maxPaths, maxBlocks, st),
                       "batch shade");
                 if (walk == kShadeWalkDefer && !last) {

@@ -56,6 +56,7 @@ struct DeviceScene {
     std::uint32_t textureCount;
     std::uint32_t nodeCount;
     std::uint32_t hasSubsurface;       // some material can take the subsurface random walk
+    std::uint32_t simpleMaterials;     // no material has coat, sheen, transmission, subsurface or translucency
     float boundsLo[3];                 // world box of the scene (radiance-cache position normalisation)
     float boundsInvExtent[3];
     // Two-level scenes (instancing); instanceCount == 0 for one level.
@@ -82,6 +83,8 @@ constexpr int kShadowSlots = 3;
 constexpr std::uint32_t kShadowMeshLight = 0xfffffffeU;  // boolean visibility, no shadow controls
 
 constexpr std::uint32_t kCounterStride = 32;  // 32 x 4 bytes: one queue size per 128-byte line
+constexpr std::uint32_t kHitCounter = 7;      // counter slot of the hit queue
+constexpr std::uint32_t kCounterSlots = 8;
 
 struct PathBuffers {
     float4* rayO;            // origin xyz
@@ -106,10 +109,13 @@ struct PathBuffers {
     // Queue sizes, one per 128-byte line (index k at counters[k * kCounterStride],
     // so that the per-wave atomics on different queues do not share a line):
     // [0],[1] path queue sizes; [2] shadow queue size; [3] walk queue size;
-    // [4],[5] extendQueue sizes; [6] shadowLeaders size.
+    // [4],[5] extendQueue sizes; [6] shadowLeaders size; [7] hit queue size
+    // (the hit queue of bounce `in` reuses extendQueue[in] once extend has
+    // read it).
     std::uint32_t* counters;
     unsigned long long* totals;  // [0] paths extended, [1] shadow-queued paths, [2] shadow rays traced,
-                                 // [3] extend rays traced (lanes of a pixel sample trace an equal ray once)
+                                 // [3] extend rays traced (lanes of a pixel sample trace an equal ray once),
+                                 // [4] paths shaded at a surface hit
     float* lambda0;          // wavelength of lane 0 of the path's hero triple
     float2* spread;          // radiance cache: primary footprint a0, accumulated sqrt spread
     std::uint32_t* pathScene;  // batch renderer: scene of each path

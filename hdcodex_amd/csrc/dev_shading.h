@@ -268,7 +268,10 @@ struct Closure {
     float specE, diffE, translucentE, diffuseContribution, translucentContribution;
 };
 
-// evaluateDirectSurface (:1013-1069): BSDF x cos toward l.
+// evaluateDirectSurface (:1013-1069): BSDF x cos toward l. kFull = false:
+// the closure has no sheen (c.sheen == 0), whose layer is then left out
+// (it is 0 x a finite response).
+template <bool kFull = true>
 __device__ inline float evalDirect(const Closure& c, V3 l, float diffuseScale, float specularScale)
 {
     const V3 n = c.normal, v = c.view;
@@ -294,16 +297,20 @@ __device__ inline float evalDirect(const Closure& c, V3 l, float diffuseScale, f
         coatSpec = specularScale * c.coat * nl * evalGgx(n, c.tangent, v, l, c.crx, c.cry, c.coatF0);
     }
     const float baseLayer = (1.0f - c.coat * coatF) * (diffuse + translucent + specular) + coatSpec;
-    const float sheenLayer = specularScale * c.sheen * c.sheenColor * sheenResponse(n, v, l, c.sheenRoughness, c.sheenZeltner);
+    const float sheenLayer =
+        kFull ? specularScale * c.sheen * c.sheenColor * sheenResponse(n, v, l, c.sheenRoughness, c.sheenZeltner) : 0.0f;
     return baseLayer * (1.0f - clampf(c.sheen * c.sheenAlbedo, 0.0f, 1.0f)) + sheenLayer;
 }
 
 // surfaceReflectionPdf (:392-415)
+// kFull = false: the closure has no coat (c.pCoat == 0), whose pdf term is
+// then left out (it is 0 x a finite pdf).
+template <bool kFull = true>
 __device__ inline float surfacePdf(const Closure& c, V3 l)
 {
     const float refl = fmaxf(dot(c.normal, l), 0.0f) / kPi;
     const float trans = fmaxf(-dot(c.normal, l), 0.0f) / kPi;
-    const float coatPdf = ggxPdf(c.normal, c.tangent, c.view, l, c.crx, c.cry);
+    const float coatPdf = kFull ? ggxPdf(c.normal, c.tangent, c.view, l, c.crx, c.cry) : 0.0f;
     const float specPdf = ggxPdf(c.normal, c.tangent, c.view, l, c.rx, c.ry);
     const float diffPdf = mixf(refl, trans, c.pTranslucent);
     const float base = c.pCoat * coatPdf + (1.0f - c.pCoat) * (c.pSpec * specPdf + (1.0f - c.pSpec) * diffPdf);

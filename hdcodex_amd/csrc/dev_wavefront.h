@@ -1,0 +1,94 @@
+// Wavefront building blocks shared by the stage kernels (kernels.hip,
+// shade.hip): block size, queue appends, the opacity of a hit.
+#pragma once
+
+#include "dev_shading.h"
+#include "dev_traverse2.h"
+
+#include <hip/hip_runtime.h>
+
+namespace hdcx {
+
+constexpr int kBlock = 128;
+
+__device__ __forceinline__ std::uint32_t laneId() { return __lane_id(); }
+
+// Appends `item` for lanes with `keep` to `queue` with one atomic per wave.
+__device__ __forceinline__ void waveAppend(bool keep, std::uint32_t item, std::uint32_t* queue, std::uint32_t* counter)
+{
+    const unsigned long long mask = __ballot(keep);
+    if (mask == 0ULL) return;
+    const unsigned long long active = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(active)) - 1;
+    const std::uint32_t lane = laneId();
+    const std::uint32_t total = static_cast<std::uint32_t>(__popcll(mask));
+    const std::uint32_t before = static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
+    std::uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    if (keep) queue[base + before] = item;
+}
+
+// Index of this lane's entry in a compacted array (one atomic per wave), or
+// 0xffffffff when it does not want one.
+__device__ __forceinline__ std::uint32_t waveSlot(bool want, std::uint32_t* counter)
+{
+    const unsigned long long mask = __ballot(want);
+    if (mask == 0ULL) return 0xffffffffU;
+    const unsigned long long active = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(active)) - 1;
+    const std::uint32_t lane = laneId();
+    std::uint32_t base = 0;
+    if (static_cast<int>(lane) == leader) base = atomicAdd(counter, static_cast<std::uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    return want ? base + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL))) : 0xffffffffU;
+}
+
+constexpr unsigned kPairBlock = 1024;
+
+
+// Appends pid for threads with `want` to `queue`: one atomic per block of
+// kPairBlock threads. Every thread of the block must call it.
+__device__ __forceinline__ void pairAppend(bool want, std::uint32_t pid, std::uint32_t* queue, std::uint32_t* counter,
+                                           std::uint32_t* waveCount, std::uint32_t* waveBase)
+{
+    constexpr unsigned kWaves = kPairBlock / 64U;
+    const std::uint32_t lane = laneId(), wave = threadIdx.x >> 6;
+    const unsigned long long mask = __ballot(want);
+    if (lane == 0U) waveCount[wave] = static_cast<std::uint32_t>(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0U) {
+        std::uint32_t total = 0U;
+        for (unsigned k = 0; k < kWaves; ++k) total += waveCount[k];
+        std::uint32_t at = total ? atomicAdd(counter, total) : 0U;
+        for (unsigned k = 0; k < kWaves; ++k) {
+            waveBase[k] = at;
+            at += waveCount[k];
+        }
+    }
+    __syncthreads();
+    if (want) queue[waveBase[wave] + static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)))] = pid;
+    __syncthreads();
+}
+
+template <bool k2L>
+__device__ __forceinline__ float materialOpacity(const DeviceScene& s, std::uint32_t prim, float bu, float bv)
+{
+    const TriRef tr = resolvePrim<k2L>(s, prim);
+    const GpuMaterialRecord& m = s.materials[triangleMaterial(s, tr)];
+    const std::uint32_t tri = tr.tri;
+    const float2 t0 = s.texcoords[tri * 3], t1 = s.texcoords[tri * 3 + 1], t2 = s.texcoords[tri * 3 + 2];
+    const float w0 = 1.0f - bu - bv;
+    const float u = t0.x * w0 + t1.x * bu + t2.x * bv, v = t0.y * w0 + t1.y * bu + t2.y * bv;
+    return clampf(matScalar(s, m.textureIndices1.x, u, v, m.transmissionOpacityIor.y), 0.0f, 1.0f);
+}
+
+// Blocks for `work` items, at most maxBlocks (the stage loops are
+// grid-stride).
+inline unsigned GridFor(std::uint32_t work, unsigned maxBlocks)
+{
+    const unsigned blocks = (work + kBlock - 1U) / kBlock;
+    return blocks == 0 ? 1U : (blocks < maxBlocks ? blocks : maxBlocks);
+}
+
+} // namespace hdcx

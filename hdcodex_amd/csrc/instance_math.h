@@ -39,7 +39,11 @@ HDCX_HD inline void TransformPoint(const float* m, float x, float y, float z, fl
 }
 
 // Normal matrix applied to a corner normal, then normalised; a zero normal
-// (meaning "use the geometric normal") stays zero.
+// (meaning "use the geometric normal") stays zero. The square root and the
+// divisions are the correctly rounded FP32 ones in every build (host
+// flattening, and device code built with hardware-rate FP32 division):
+// each is evaluated in FP64 and rounded once to FP32, which gives the
+// correctly rounded FP32 result for sqrt and division of FP32 operands.
 HDCX_HD inline void TransformNormal(const float* nm, float x, float y, float z, float out[3])
 {
     HDCX_NO_CONTRACT
@@ -50,8 +54,8 @@ HDCX_HD inline void TransformNormal(const float* nm, float x, float y, float z, 
         out[0] = out[1] = out[2] = 0.0f;
         return;
     }
-    const float l = sqrtf(l2);
-    for (int r = 0; r < 3; ++r) out[r] = v[r] / l;
+    const float l = static_cast<float>(sqrt(static_cast<double>(l2)));
+    for (int r = 0; r < 3; ++r) out[r] = static_cast<float>(static_cast<double>(v[r]) / static_cast<double>(l));
 }
 
 // Host: inverse of the 3x4 affine transform and the normal matrix, in FP64,

@@ -10,9 +10,14 @@ hipError_t LaunchRaygen(const PathBuffers& pb, const ChunkDesc& c, std::uint32_t
 // Training samples from the chunk's records; then the records are
 // consumed and, with `outputs` (the cache's three outputs for every record
 // query), training paths that reached the terminate vertex take the cached
-// value there in the image, scaled by *calibration when given.
+// value there in the image, scaled by *calibration when given. With
+// `scratch` (GatherScratchWords(records) words; one scene, records counted
+// on the host) the samples follow record order; otherwise their order is
+// that of the appends.
 hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, std::uint32_t records, unsigned maxBlocks,
-                                hipStream_t st, const float* outputs = nullptr, const float* calibration = nullptr);
+                                hipStream_t st, const float* outputs = nullptr, const float* calibration = nullptr,
+                                std::uint32_t* scratch = nullptr);
+std::size_t GatherScratchWords(std::uint32_t records);
 // Extend sharing: kShareNone traces every queued path; kShareCamera (the
 // launch after raygen) traces one camera ray per pixel sample for its
 // three lanes; kShareLanes traces the paths LaunchPair queued as tracing
@@ -28,8 +33,14 @@ hipError_t LaunchPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t
 // walk: kShadeWalkInline runs subsurface walks inside shade;
 // kShadeWalkDefer queues them (then LaunchWalk, then shade again with
 // kShadeWalkResume over the walk queue: same results, the walks' traversal
-// runs in a kernel of its own).
-constexpr int kShadeWalkInline = 0, kShadeWalkDefer = 1, kShadeWalkResume = 2;
+// runs in a kernel of its own). kShadeSimple: the variant without coat,
+// sheen, transmission, subsurface and translucent lobes, for scenes whose
+// materials have none (DeviceScene::simpleMaterials); same images.
+constexpr int kShadeWalkInline = 0, kShadeWalkDefer = 1, kShadeWalkResume = 2, kShadeSimple = 3;
+// Ends the paths of queue[in] whose ray missed (environment added) and
+// queues the others for shade; run between extend and shade.
+hipError_t LaunchMiss(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
+                      unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
                        const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
@@ -48,6 +59,8 @@ hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const Path
                              int sharing, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
                              hipStream_t st);
 hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
+                           unsigned maxBlocks, hipStream_t st);
+hipError_t LaunchMissBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
                             int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,

@@ -103,59 +103,6 @@ float TextureAverageLuminance(const SceneTexture& source)
     return static_cast<float>(total / static_cast<double>(texels));
 }
 
-SceneTexture BuildLatLongImportanceTexture(const SceneTexture& source)
-{
-    // Luminance x texel solid angle, FP64 marginal (row) and conditional
-    // (column) CDFs; b = luminance / total = pdf per steradian (:1998-2066).
-    SceneTexture result;
-    if (!TextureUsable(source)) return result;
-    const std::uint32_t w = source.width;
-    const std::uint32_t h = source.height;
-    const std::size_t texels = static_cast<std::size_t>(w) * h;
-    std::vector<float> lum(texels);
-    std::vector<double> weight(texels);
-    std::vector<double> rowWeight(h, 0.0);
-    const double pi = 3.14159265358979323846;
-    const double dphi = 2.0 * pi / static_cast<double>(w);
-    double total = 0.0;
-    for (std::uint32_t y = 0; y < h; ++y) {
-        const double c0 = std::cos(pi * (1.0 - static_cast<double>(y) / h));
-        const double c1 = std::cos(pi * (1.0 - static_cast<double>(y + 1U) / h));
-        const double solidAngle = dphi * std::abs(c1 - c0);
-        for (std::uint32_t x = 0; x < w; ++x) {
-            const std::size_t p = static_cast<std::size_t>(y) * w + x;
-            float rgb[3];
-            LinearTexel(source, p, rgb);
-            lum[p] = 0.2126F * rgb[0] + 0.7152F * rgb[1] + 0.0722F * rgb[2];
-            weight[p] = static_cast<double>(lum[p]) * solidAngle;
-            rowWeight[y] += weight[p];
-            total += weight[p];
-        }
-    }
-    if (!(total > 0.0) || !std::isfinite(total)) return result;
-    result.id = source.id + "#latlong-importance";
-    result.sourcePath = source.sourcePath;
-    result.width = w;
-    result.height = h;
-    result.rgbaFloat.resize(texels * 4);
-    double marginal = 0.0;
-    for (std::uint32_t y = 0; y < h; ++y) {
-        marginal += rowWeight[y] / total;
-        double conditional = 0.0;
-        for (std::uint32_t x = 0; x < w; ++x) {
-            const std::size_t p = static_cast<std::size_t>(y) * w + x;
-            conditional += rowWeight[y] > 0.0 ? weight[p] / rowWeight[y]
-                                              : 1.0 / static_cast<double>(w);
-            float* out = &result.rgbaFloat[p * 4];
-            out[0] = static_cast<float>(x + 1U == w ? 1.0 : conditional);
-            out[1] = static_cast<float>(y + 1U == h ? 1.0 : marginal);
-            out[2] = static_cast<float>(static_cast<double>(lum[p]) / total);
-            out[3] = 1.0F;
-        }
-    }
-    return result;
-}
-
 PackedScene FlattenInstances(const hdcx_scene_view& v)
 {
     PackedScene out;
@@ -165,7 +112,14 @@ PackedScene FlattenInstances(const hdcx_scene_view& v)
     out.triangleMaterials.assign(v.triangle_materials, v.triangle_materials + drawn);
     out.texcoords.assign(v.texcoords, v.texcoords + static_cast<std::size_t>(drawn) * 6);
     out.normals.assign(v.normals, v.normals + static_cast<std::size_t>(drawn) * 12);
-    out.emissive.assign(v.emissive, v.emissive + static_cast<std::size_t>(drawn) * 4);
+    if (v.emissive) {
+        out.emissive.assign(v.emissive, v.emissive + static_cast<std::size_t>(drawn) * 4);
+    } else {
+        // The device builds the table; the instances' copies stay out of it.
+        out.lightTriangleCount = v.light_triangle_count ? v.light_triangle_count : drawn;
+        if (v.material_emission_power)
+            out.materialEmissionPower.assign(v.material_emission_power, v.material_emission_power + v.material_count);
+    }
     for (std::uint32_t i = 0; i < v.instance_count; ++i) {
         const hdcx_instance& inst = v.instances[i];
         const hdcx_prototype& proto = v.prototypes[inst.prototype];
@@ -190,7 +144,8 @@ PackedScene FlattenInstances(const hdcx_scene_view& v)
             out.texcoords.insert(out.texcoords.end(), v.texcoords + static_cast<std::size_t>(t) * 6,
                                  v.texcoords + static_cast<std::size_t>(t) * 6 + 6);
             // Never sampled as a light (as on the device): the final CDF value, zero pdf.
-            out.emissive.insert(out.emissive.end(), {v.emissive[static_cast<std::size_t>(t) * 4], 0.0F, 0.0F, 0.0F});
+            if (v.emissive)
+                out.emissive.insert(out.emissive.end(), {v.emissive[static_cast<std::size_t>(t) * 4], 0.0F, 0.0F, 0.0F});
         }
     }
     const auto* mats = static_cast<const GpuMaterialRecord*>(v.materials);
@@ -237,7 +192,9 @@ hdcx_scene_view PackedScene::View()
     v.triangle_materials = triangleMaterials.data();
     v.texcoords = texcoords.data();
     v.normals = normals.data();
-    v.emissive = emissive.data();
+    v.emissive = emissive.empty() ? nullptr : emissive.data();
+    v.material_emission_power = materialEmissionPower.empty() ? nullptr : materialEmissionPower.data();
+    v.light_triangle_count = lightTriangleCount;
     v.material_count = static_cast<std::uint32_t>(materials.size());
     v.light_count = lightCount;
     v.materials = materials.data();
@@ -312,8 +269,14 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
         luminanceCache.emplace(id, value);
         return value;
     };
-    // Importance textures are appended as dome lights first reference them (:2681-2702).
-    std::map<std::string, std::uint32_t, std::less<>> importanceIndex;
+    // A lat-long (or automatic) dome with a usable plain texture is
+    // importance sampled. The reference appends one importance texture per
+    // distinct dome texture, in the order the domes first reference them,
+    // while fewer than 256 textures exist (:2681-2702); those slots are
+    // counted here, and the dome's importance slot names its own texture:
+    // the tables are built on the device at hdcx_set_scene.
+    std::map<std::string, std::uint32_t, std::less<>> importanceSlot;
+    std::size_t textureSlots = out.textures.size();
     const auto domeImportance = [&](const SceneLight& light) {
         if (light.type != SceneLightType::Dome ||
             (light.textureFormat != DomeTextureFormat::Automatic &&
@@ -321,14 +284,12 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
             indexOf(light.texture) == kMissingTexture) {
             return kMissingTexture;
         }
-        if (const auto c = importanceIndex.find(light.texture); c != importanceIndex.end()) return c->second;
+        if (const auto c = importanceSlot.find(light.texture); c != importanceSlot.end()) return c->second;
         const auto s = plainSource.find(light.texture);
-        if (s == plainSource.end() || out.textures.size() >= kMaxTextures) return kMissingTexture;
-        SceneTexture importance = BuildLatLongImportanceTexture(*s->second);
-        if (importance.rgbaFloat.empty()) return kMissingTexture;
-        const std::uint32_t index = static_cast<std::uint32_t>(out.textures.size());
-        out.textures.push_back(UploadFormat(importance));
-        importanceIndex.emplace(light.texture, index);
+        if (s == plainSource.end() || textureSlots >= kMaxTextures || !TextureUsable(*s->second)) return kMissingTexture;
+        ++textureSlots;
+        const std::uint32_t index = indexOf(light.texture);
+        importanceSlot.emplace(light.texture, index);
         return index;
     };
 
@@ -547,48 +508,9 @@ PackedScene PackScene(const SceneSnapshot& snapshot)
         return out;
     }
 
-    // Emissive-triangle CDF over all drawn triangles, weight = area x
-    // emission power, accumulated in FP64; last CDF entry forced to 1
-    // (:2944-2993). Prototype triangles are never sampled as lights: their
-    // entries repeat the final CDF value with zero pdf.
-    const std::size_t allTriangles = out.triangleMaterials.size();
-    const std::size_t triangles = out.prototypes.empty() ? allTriangles : out.prototypes.front().first_triangle;
-    out.emissive.assign(allTriangles * 4, 0.0F);
-    std::vector<double> weights(triangles, 0.0);
-    std::vector<float> areas(triangles, 0.0F);
-    double total = 0.0;
-    for (std::size_t p = 0; p < triangles; ++p) {
-        const float* a = &out.positions[out.indices[p * 3] * 4];
-        const float* b = &out.positions[out.indices[p * 3 + 1] * 4];
-        const float* c = &out.positions[out.indices[p * 3 + 2] * 4];
-        const double e1[3] = {static_cast<double>(b[0]) - a[0], static_cast<double>(b[1]) - a[1],
-                              static_cast<double>(b[2]) - a[2]};
-        const double e2[3] = {static_cast<double>(c[0]) - a[0], static_cast<double>(c[1]) - a[1],
-                              static_cast<double>(c[2]) - a[2]};
-        const double cx = e1[1] * e2[2] - e1[2] * e2[1];
-        const double cy = e1[2] * e2[0] - e1[0] * e2[2];
-        const double cz = e1[0] * e2[1] - e1[1] * e2[0];
-        const float area = static_cast<float>(0.5 * std::sqrt(cx * cx + cy * cy + cz * cz));
-        areas[p] = std::isfinite(area) ? area : 0.0F;
-        const std::uint32_t material = out.triangleMaterials[p];
-        const float power = material < emissionPower.size() ? emissionPower[material] : 0.0F;
-        const double w = static_cast<double>(areas[p]) * static_cast<double>(power);
-        if (w > 0.0 && std::isfinite(w)) {
-            weights[p] = w;
-            total += w;
-        }
-    }
-    if (total > 0.0 && std::isfinite(total)) {
-        double cdf = 0.0;
-        for (std::size_t p = 0; p < triangles; ++p) {
-            const double probability = weights[p] / total;
-            cdf += probability;
-            out.emissive[p * 4 + 0] = static_cast<float>(p + 1 == triangles ? 1.0 : cdf);
-            out.emissive[p * 4 + 1] = areas[p] > 0.0F ? static_cast<float>(probability) / areas[p] : 0.0F;
-            out.emissive[p * 4 + 2] = areas[p];
-        }
-        for (std::size_t p = triangles; p < allTriangles; ++p) out.emissive[p * 4 + 0] = 1.0F;
-    }
+    // The emissive-triangle table (:2944-2993) is built on the device from
+    // these powers; prototype triangles are never sampled as lights.
+    out.materialEmissionPower = std::move(emissionPower);
     return out;
 }
 

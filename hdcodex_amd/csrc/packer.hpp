@@ -25,7 +25,9 @@ struct PackedScene {
     std::vector<std::uint32_t> triangleMaterials;
     std::vector<float> texcoords;            // 6 per triangle
     std::vector<float> normals;              // 12 per triangle
-    std::vector<float> emissive;             // 4 per triangle
+    std::vector<float> emissive;             // 4 per triangle; empty: built on the device
+    std::vector<float> materialEmissionPower;  // 1 per material (device-built emissive table)
+    std::uint32_t lightTriangleCount{0};     // mesh-light triangles [0, n); 0 = all drawn
     std::vector<GpuMaterialRecord> materials;
     std::vector<GpuLightRecord> lights;      // >= 1 (dummy when none authored)
     std::uint32_t lightCount{0};             // authored, visible, <= 64
@@ -49,12 +51,15 @@ struct PackedScene {
 PackedScene FlattenInstances(const hdcx_scene_view& view);
 
 // Restates the reference's BuildScene host flattening and packing
-// (src/gpu/vulkan_path_tracer.cpp:2604-2993).
+// (src/gpu/vulkan_path_tracer.cpp:2604-2940). The light tables that the
+// reference builds on the host next (emissive-triangle CDF, :2944-2993;
+// lat-long dome importance, :1998-2066) are left to hdcx_set_scene, which
+// builds them on the device: the view carries a NULL emissive table plus
+// each material's emission power, and each importance-sampled dome names
+// its own texture in its importance slot.
 PackedScene PackScene(const SceneSnapshot& snapshot);
 
 // src/gpu/vulkan_path_tracer.cpp:1968-1996
 float TextureAverageLuminance(const SceneTexture& source);
-// src/gpu/vulkan_path_tracer.cpp:1998-2066; empty result when unusable.
-SceneTexture BuildLatLongImportanceTexture(const SceneTexture& source);
 
 } // namespace hdcx

@@ -5,9 +5,11 @@
 #include "instance_math.h"
 #include "lbvh_host.hpp"
 #include "records.h"
+#include "tables.h"
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -32,7 +34,7 @@ void ValidateTextureRef(std::uint32_t ti, std::uint32_t textureCount, const char
 
 void ValidateScene(const hdcx_scene_view& v)
 {
-    if (v.triangle_count && (!v.indices || !v.triangle_materials || !v.texcoords || !v.normals || !v.emissive))
+    if (v.triangle_count && (!v.indices || !v.triangle_materials || !v.texcoords || !v.normals))
         throw Failure(HDCX_ERR_INVALID_SCENE, "triangle arrays missing");
     if (v.vertex_count && !v.positions) throw Failure(HDCX_ERR_INVALID_SCENE, "positions missing");
     if (v.material_count == 0 || !v.materials) throw Failure(HDCX_ERR_INVALID_SCENE, "no materials");
@@ -90,14 +92,21 @@ void ValidateScene(const hdcx_scene_view& v)
         }
         if (flat >= 0xffffffffULL) throw Failure(HDCX_ERR_INVALID_SCENE, "more than 2^32 - 1 flattened triangles");
     }
+    const std::uint32_t drawn = v.prototype_count ? v.prototypes[0].first_triangle : v.triangle_count;
+    if (!v.emissive && v.light_triangle_count > drawn)
+        throw Failure(HDCX_ERR_INVALID_SCENE, "light_triangle_count beyond the drawn triangles");
     const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
     for (std::uint32_t l = 0; l < v.light_count; ++l) {
         ValidateTextureRef(ls[l].textureInfo.x, v.texture_count, "light");
-        if (ls[l].textureInfo.w != kMissingTexture) {
-            if (ls[l].textureInfo.w >= v.texture_count ||
-                v.textures[ls[l].textureInfo.w].format != HDCX_TEXTURE_RGBA32F)
-                throw Failure(HDCX_ERR_INVALID_SCENE, "light importance texture invalid");
+        if (ls[l].textureInfo.w == kMissingTexture) continue;
+        if (ls[l].textureInfo.z == 0U && ls[l].textureInfo.w == ls[l].textureInfo.x) {
+            // Built on the device from the dome's own (plain) texture.
+            if (ls[l].textureInfo.x & kUdimTextureFlag)
+                throw Failure(HDCX_ERR_INVALID_SCENE, "dome importance from a UDIM set");
+            continue;
         }
+        if (ls[l].textureInfo.w >= v.texture_count || v.textures[ls[l].textureInfo.w].format != HDCX_TEXTURE_RGBA32F)
+            throw Failure(HDCX_ERR_INVALID_SCENE, "light importance texture invalid");
     }
 }
 
@@ -388,7 +397,9 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     t->triMaterial.Upload(v.triangle_materials, static_cast<std::size_t>(v.triangle_count) * 4, st);
     t->texcoords.Upload(v.texcoords, static_cast<std::size_t>(v.triangle_count) * 24, st);
     t->normals.Upload(v.normals, static_cast<std::size_t>(v.triangle_count) * 48, st);
-    t->emissive.Upload(v.emissive, static_cast<std::size_t>(v.triangle_count) * 16, st);
+    const bool deviceEmissive = v.emissive == nullptr;
+    if (!deviceEmissive) t->emissive.Upload(v.emissive, static_cast<std::size_t>(v.triangle_count) * 16, st);
+    t->emissiveEntries = v.triangle_count;
     t->materials.Upload(v.materials, static_cast<std::size_t>(v.material_count) * HDCX_MATERIAL_BYTES, st);
     const std::uint32_t lightRecords = std::max<std::uint32_t>(v.light_count, 1);
     t->lights.Upload(v.lights, static_cast<std::size_t>(lightRecords) * HDCX_LIGHT_BYTES, st);
@@ -411,8 +422,30 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     t->textures.Upload(descs.data(), descs.size() * sizeof(TexDesc), st);
     t->texels.Upload(pool.data(), pool.size(), st);
 
-    // Dome importance tables re-laid out per light: marginal CDF column,
-    // conditional CDF and pdf planes of the packed importance texture.
+    if (deviceEmissive) {
+        // Emissive-triangle distribution on the device (:2944-2993).
+        const std::uint32_t lightTriangles = v.light_triangle_count ? v.light_triangle_count : drawn;
+        t->emissive.Allocate(static_cast<std::size_t>(v.triangle_count) * 16);
+        t->materialPower.Allocate(static_cast<std::size_t>(v.material_count) * 4);
+        if (v.material_emission_power) {
+            t->materialPower.Upload(v.material_emission_power, static_cast<std::size_t>(v.material_count) * 4, st);
+        } else {
+            Check(LaunchMaterialPower(t->materials.as<GpuMaterialRecord>(), v.material_count, t->textures.as<TexDesc>(),
+                                      v.texture_count, t->texels.as<unsigned char>(), t->materialPower.as<float>(), st),
+                  "material power");
+        }
+        t->tableScratch.Reserve(EmissiveScratchDoubles(lightTriangles) * sizeof(double));
+        Check(LaunchEmissiveTable(t->positions.as<float4>(), t->indices.as<std::uint32_t>(),
+                                  t->triMaterial.as<std::uint32_t>(), lightTriangles, v.triangle_count,
+                                  t->materialPower.as<float>(), v.material_count, t->emissive.as<float4>(),
+                                  t->tableScratch.as<double>(), st),
+              "emissive table");
+    }
+
+    // Dome importance tables per light: marginal CDF, conditional CDFs, pdf
+    // plane and guide tables in one pool. Host-built importance textures
+    // are re-laid out here; domes whose importance slot names their own
+    // texture get theirs built on the device after the upload.
     const auto* ls = static_cast<const GpuLightRecord*>(v.lights);
     std::vector<ImportanceDesc> imp(lightRecords, ImportanceDesc{});
     std::vector<float> impPool;
@@ -437,9 +470,14 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
         while ((1U << k) < n && k < 20U) ++k;
         return k;
     };
+    std::vector<std::uint32_t> deviceDomes;
     for (std::uint32_t l = 0; l < v.light_count; ++l) {
         const std::uint32_t ti = ls[l].textureInfo.w;
         if (ls[l].textureInfo.z != 0U || ti == kMissingTexture) continue;
+        if (ti == ls[l].textureInfo.x) {
+            deviceDomes.push_back(l);
+            continue;
+        }
         const hdcx_texture_view& tv = v.textures[ti];
         const float* px = static_cast<const float*>(tv.texels);
         ImportanceDesc d{};
@@ -474,10 +512,70 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
         }
         imp[l] = d;
     }
-    if (impPool.empty()) impPool.push_back(0.0F);
+    // Device-built tables follow the host part of the pool. Their CDFs are
+    // monotone by construction (prefix sums of non-negative terms), so both
+    // guide tables are always present.
+    std::size_t poolFloats = std::max<std::size_t>(impPool.size(), 1);
+    std::size_t scratchDoubles = 0;
+    for (std::uint32_t l : deviceDomes) {
+        const hdcx_texture_view& tv = v.textures[ls[l].textureInfo.x];
+        ImportanceDesc d{};
+        d.width = tv.width;
+        d.height = tv.height;
+        d.rowShift = shiftFor(tv.height);
+        d.colShift = shiftFor(tv.width);
+        const std::size_t texels = static_cast<std::size_t>(tv.width) * tv.height;
+        d.marginal = poolFloats;
+        d.conditional = d.marginal + tv.height;
+        d.pdf = d.conditional + texels;
+        d.rowGuide = d.pdf + texels;
+        d.colGuide = d.rowGuide + (1U << d.rowShift) + 1U;
+        poolFloats = d.colGuide + static_cast<std::size_t>(tv.height) * ((1U << d.colShift) + 1U);
+        scratchDoubles = std::max(scratchDoubles, ImportanceScratchDoubles(tv.width, tv.height));
+        imp[l] = d;
+    }
     t->importance.Upload(imp.data(), imp.size() * sizeof(ImportanceDesc), st);
-    t->importancePool.Upload(impPool.data(), impPool.size() * sizeof(float), st);
+    t->importancePool.Allocate(poolFloats * sizeof(float));
+    if (!impPool.empty())
+        Check(hipMemcpyAsync(t->importancePool.as<float>(), impPool.data(), impPool.size() * sizeof(float),
+                             hipMemcpyHostToDevice, st), "importance upload");
+    if (!deviceDomes.empty()) {
+        t->tableScratch.Reserve(scratchDoubles * sizeof(double));
+        t->domeOk.Reserve(deviceDomes.size() * sizeof(std::uint32_t));
+        for (std::size_t k = 0; k < deviceDomes.size(); ++k) {
+            const std::uint32_t l = deviceDomes[k];
+            const std::uint32_t ti = ls[l].textureInfo.x;
+            Check(LaunchImportanceTables(descs[ti], t->texels.as<unsigned char>(), imp[l], t->importancePool.as<float>(),
+                                         t->tableScratch.as<double>(), t->domeOk.as<std::uint32_t>() + k, st),
+                  "dome importance tables");
+        }
+    }
     Check(hipStreamSynchronize(st), "scene upload");
+    if (!deviceDomes.empty()) {
+        // A dome without positive weight is sampled uniformly: its record's
+        // importance slot becomes missing on the device.
+        std::vector<std::uint32_t> ok(deviceDomes.size());
+        Check(hipMemcpy(ok.data(), t->domeOk.as<std::uint32_t>(), ok.size() * 4, hipMemcpyDeviceToHost), "dome flags");
+        for (std::size_t k = 0; k < deviceDomes.size(); ++k) {
+            if (ok[k]) continue;
+            const std::uint32_t l = deviceDomes[k];
+            const std::uint32_t missing = kMissingTexture;
+            Check(hipMemcpy(t->lights.as<unsigned char>() + static_cast<std::size_t>(l) * HDCX_LIGHT_BYTES +
+                                offsetof(GpuLightRecord, textureInfo) + 3 * sizeof(std::uint32_t),
+                            &missing, 4, hipMemcpyHostToDevice),
+                  "dome record");
+            imp[l] = ImportanceDesc{};
+        }
+    }
+    t->importanceLayout.assign(imp.begin(), imp.begin() + v.light_count);
+    float lastCdf = 0.0F;
+    if (drawn > 0) {
+        if (deviceEmissive)
+            Check(hipMemcpy(&lastCdf, t->emissive.as<float>() + (static_cast<std::size_t>(drawn) - 1) * 4, 4,
+                            hipMemcpyDeviceToHost), "emissive readback");
+        else
+            lastCdf = v.emissive[(static_cast<std::size_t>(drawn) - 1) * 4];
+    }
 
     DeviceScene& s = t->scene;
     s.nodes = t->nodes.as<Bvh4Node>();
@@ -497,15 +595,26 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     s.primitiveCount = drawn;  // emissive-triangle sampling covers the drawn triangles
     s.lightCount = v.light_count;
     s.opaque = v.opaque ? 1U : 0U;
-    s.hasEmissive = drawn > 0 && v.emissive[(static_cast<std::size_t>(drawn) - 1) * 4] > 0.0F ? 1U : 0U;
+    s.hasEmissive = lastCdf > 0.0F ? 1U : 0U;
     for (int k = 0; k < 3; ++k) {
         s.boundsLo[k] = t->sceneLo[k];
         s.boundsInvExtent[k] = t->sceneInvExtent[k];
     }
     s.hasSubsurface = 0U;
+    s.simpleMaterials = 1U;
     const auto* mats = static_cast<const GpuMaterialRecord*>(v.materials);
-    for (std::uint32_t m = 0; m < v.material_count; ++m)
-        if (mats[m].subsurfaceWeightScale.x > 0.0F || mats[m].textureIndices2.x != kMissingTexture) s.hasSubsurface = 1U;
+    for (std::uint32_t m = 0; m < v.material_count; ++m) {
+        const GpuMaterialRecord& r = mats[m];
+        if (r.subsurfaceWeightScale.x > 0.0F || r.textureIndices2.x != kMissingTexture) s.hasSubsurface = 1U;
+        // Every lobe the simple shading variant leaves out has weight 0
+        // here (the weights are clamped to [0, 1]; NaN counts as 0).
+        const bool plain = !(r.subsurfaceWeightScale.x > 0.0F) && r.textureIndices2.x == kMissingTexture &&
+                           !(r.coatWeightRoughnessIor.x > 0.0F) && r.textureIndices3.z == kMissingTexture &&
+                           !(r.coatColor.w > 0.0F) && r.textureIndices1.w == kMissingTexture &&
+                           !(r.translucentColorWeight.w > 0.0F) && r.textureIndices5.x == kMissingTexture &&
+                           !(r.transmissionOpacityIor.x > 0.0F);
+        if (!plain) s.simpleMaterials = 0U;
+    }
     s.textureCount = v.texture_count;
     s.nodeCount = t->bvhInfo.node_count;
     t->geometry = true;
@@ -514,7 +623,7 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
 void PathPool::EnsurePaths(std::uint32_t paths)
 {
     if (!counters.bytes()) {
-        counters.Allocate(7 * kCounterStride * sizeof(std::uint32_t));
+        counters.Allocate(kCounterSlots * kCounterStride * sizeof(std::uint32_t));
         totals.Allocate(64);
         Check(hipMemset(totals.as<void>(), 0, 64), "memset totals");
     }

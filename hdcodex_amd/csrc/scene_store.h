@@ -21,6 +21,13 @@ struct SceneStore {
     bool geometry = false;
     DeviceBuffer nodes, tris, positions, indices, triMaterial, texcoords, normals, emissive, materials, lights,
         textures, texels, importance, importancePool;
+    // Light tables built on the device: per-material emission power, FP64
+    // scan workspace, one success flag per dome light.
+    DeviceBuffer materialPower, tableScratch, domeOk;
+    // Host copies for inspection: entries of the emissive table, each
+    // light's importance layout (width 0: sampled uniformly).
+    std::uint32_t emissiveEntries = 0;
+    std::vector<ImportanceDesc> importanceLayout;
     // BVH: builder policy, the device builder's workspace, refit order and
     // the triangles (vertex count, index buffer) the BVH was built over.
     hdcx_bvh_options bvhOptions{HDCX_BVH_DEVICE_LBVH, 2048U, 1U, 0U};

@@ -66,7 +66,8 @@ struct hdcx_tracer : SceneStore, PathPool {
     int rcStep = 0;
     hdcx_rc_stats rcStats{};
     DeviceBuffer rcMaster, rcM, rcV, rcGrad, rcW, rcWt, rcLoss, rcQueries, rcOut, rcEma, rcWe, rcWte, rcAct;
-    DeviceBuffer rcChunkQueries, rcCounters, rcRecords, rcRecordThroughput, rcSamples, rcCalib, rcCalibOut, rcRecordOut;
+    DeviceBuffer rcChunkQueries, rcCounters, rcRecords, rcRecordThroughput, rcSamples, rcCalib, rcCalibOut, rcRecordOut,
+        rcGather;
     std::uint32_t rcFrame = 0;
     bool rcCountersReady = false;
     // Instrumentation.
@@ -311,7 +312,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
         Check(hipMemsetAsync(t->rcCounters.as<void>(), 0, 8, st), "rc counters");
     }
     TraversalCounters* ctr = (t->instrumentation & 2U) ? t->traversalCounters.as<TraversalCounters>() : nullptr;
-    const int walk = !t->scene.hasSubsurface || (t->instrumentation & 4U) ? kShadeWalkInline : kShadeWalkDefer;
+    const int walk = t->scene.simpleMaterials && !(t->instrumentation & 16U) ? kShadeSimple
+                     : !t->scene.hasSubsurface || (t->instrumentation & 4U) ? kShadeWalkInline
+                                                                             : kShadeWalkDefer;
     const bool share = !(t->instrumentation & 8U);  // lanes of a pixel sample trace an equal ray once
     TraversalCounters* shadowCtr = ctr ? ctr + 1 : nullptr;
     ChunkDesc c{};
@@ -341,6 +344,7 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                 t->Timed(HDCX_KERNEL_EXTEND,
                          [&] { return LaunchExtend(t->scene, pb, in, sharing, paths, t->maxBlocks, ctr, st); });
                 const int last = b + 1 == bounces ? 1 : 0;
+                t->Timed(HDCX_KERNEL_MISS, [&] { return LaunchMiss(t->scene, pb, in, paths, t->maxBlocks, st); });
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
                     return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, walk, paths, t->maxBlocks, st);
                 });
@@ -381,7 +385,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                           "rc training-path answers");
                     outputs = t->rcRecordOut.as<float>();
                 }
-                Check(LaunchGatherTraining(pb, cp, records, t->maxBlocks, st, outputs, t->CalibrationScale()),
+                t->rcGather.Reserve(GatherScratchWords(records) * sizeof(std::uint32_t));
+                Check(LaunchGatherTraining(pb, cp, records, t->maxBlocks, st, outputs, t->CalibrationScale(),
+                                           t->rcGather.as<std::uint32_t>()),
                       "gather training samples");
             }
             t->Timed(HDCX_KERNEL_FILM, [&] { return LaunchFilm(pb, c, t->filmBatch.as<double4>(), t->maxBlocks, st); });
@@ -733,6 +739,40 @@ int hdcx_get_bvh_info(hdcx_tracer* tracer, hdcx_bvh_info* out_info)
     });
 }
 
+int hdcx_read_emissive_table(hdcx_tracer* tracer, float* out, uint32_t count)
+{
+    return Guard([&] {
+        if (!tracer || !out) throw std::invalid_argument("null argument");
+        if (!tracer->geometry) throw Failure(HDCX_ERR_NO_GEOMETRY, "no scene");
+        if (count != tracer->emissiveEntries) throw std::invalid_argument("count is not the scene's triangle count");
+        tracer->Activate();
+        Check(hipMemcpy(out, tracer->emissive.as<float>(), static_cast<std::size_t>(count) * 16, hipMemcpyDeviceToHost),
+              "emissive readback");
+    });
+}
+
+int hdcx_read_dome_importance(hdcx_tracer* tracer, uint32_t light, uint32_t* out_width, uint32_t* out_height,
+                              float* out_marginal, float* out_conditional, float* out_pdf)
+{
+    return Guard([&] {
+        if (!tracer || !out_width || !out_height) throw std::invalid_argument("null argument");
+        if (!tracer->geometry) throw Failure(HDCX_ERR_NO_GEOMETRY, "no scene");
+        if (light >= tracer->importanceLayout.size()) throw std::invalid_argument("light index out of range");
+        const ImportanceDesc& d = tracer->importanceLayout[light];
+        *out_width = d.width;
+        *out_height = d.height;
+        tracer->Activate();
+        const float* pool = tracer->importancePool.as<float>();
+        const std::size_t texels = static_cast<std::size_t>(d.width) * d.height;
+        if (out_marginal && d.height)
+            Check(hipMemcpy(out_marginal, pool + d.marginal, d.height * sizeof(float), hipMemcpyDeviceToHost), "readback");
+        if (out_conditional && texels)
+            Check(hipMemcpy(out_conditional, pool + d.conditional, texels * sizeof(float), hipMemcpyDeviceToHost), "readback");
+        if (out_pdf && texels)
+            Check(hipMemcpy(out_pdf, pool + d.pdf, texels * sizeof(float), hipMemcpyDeviceToHost), "readback");
+    });
+}
+
 int hdcx_set_radiance_cache(hdcx_tracer* tracer, const hdcx_radiance_cache_config* config)
 {
     return Guard([&] {
@@ -921,12 +961,13 @@ int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats)
         tracer->Activate();
         hdcx_kernel_stats s = tracer->stats;
         if (tracer->totals.bytes()) {
-            unsigned long long totals[4] = {0, 0, 0, 0};
+            unsigned long long totals[5] = {0, 0, 0, 0, 0};
             TraversalCounters ctr[2] = {};
             Check(hipMemcpy(totals, tracer->totals.as<void>(), sizeof(totals), hipMemcpyDeviceToHost), "stats");
             Check(hipMemcpy(ctr, tracer->traversalCounters.as<void>(), sizeof(ctr), hipMemcpyDeviceToHost), "stats");
             s.rays[HDCX_KERNEL_EXTEND] = totals[3];
-            s.rays[HDCX_KERNEL_SHADE] = totals[0];
+            s.rays[HDCX_KERNEL_MISS] = totals[0];
+            s.rays[HDCX_KERNEL_SHADE] = totals[4];
             s.rays[HDCX_KERNEL_SHADOW] = totals[2];
             s.nodes_visited = ctr[0].nodes;
             s.triangles_tested = ctr[0].triangles;

@@ -104,7 +104,8 @@ typedef struct hdcx_scene_view {
     const uint32_t* triangle_materials;/* 1 per triangle */
     const float* texcoords;            /* 6 per triangle (uv per corner) */
     const float* normals;              /* 12 per triangle (float4 per corner) */
-    const float* emissive;             /* 4 per triangle: cdf, pdf/area, area, 0 */
+    const float* emissive;             /* 4 per triangle: cdf, pdf/area, area, 0;
+                                          NULL: built on the device (below) */
     uint32_t material_count;
     uint32_t light_count;              /* authored lights, <= HDCX_MAX_LIGHTS */
     const void* materials;             /* material_count * HDCX_MATERIAL_BYTES */
@@ -116,7 +117,34 @@ typedef struct hdcx_scene_view {
     const hdcx_prototype* prototypes;
     uint32_t instance_count;
     const hdcx_instance* instances;
+    /* Used only when emissive is NULL. Emission power per material
+     * (max(emissionWeight, 0) x emission luminance, the reference's
+     * materialEmissionPowers, vulkan_path_tracer.cpp:2873-2880), or NULL:
+     * then derived from the records (luminance 0.2126 r + 0.7152 g +
+     * 0.0722 b of max(emission, 0), or the mean texel luminance of the
+     * emission texture, of a UDIM set the mean of its tiles' means). */
+    const float* material_emission_power;
+    /* Used only when emissive is NULL: triangles [0, n) are sampled as mesh
+     * lights, the others are not (cdf 1, pdf 0); 0 = every drawn triangle. */
+    uint32_t light_triangle_count;
+    uint32_t reserved;
 } hdcx_scene_view;
+
+/* Light tables built on the device by hdcx_set_scene (no host pass):
+ * - emissive == NULL: the emissive-triangle distribution of the reference's
+ *   BuildScene (vulkan_path_tracer.cpp:2944-2993): weight = area x power of
+ *   the triangle's material, area from the FP64 cross product, FP64 prefix
+ *   sum of weight / total, the last sampled triangle's cdf 1; every entry 0
+ *   when no weight is positive;
+ * - a dome light record (textureInfo.z == 0) whose textureInfo.w equals
+ *   textureInfo.x: the lat-long importance tables of its own texture
+ *   (BuildLatLongImportanceTexture, :1998-2066: luminance x row solid
+ *   angle, FP64 marginal and conditional CDFs, pdf = luminance / total);
+ *   a dome with no positive weight is sampled uniformly (as if w were
+ *   0xffffffff). Otherwise w names a host-built RGBA32F importance texture
+ *   (r conditional CDF, g marginal CDF, b pdf), as the reference packs it.
+ * The device sums run in a fixed tree order: deterministic, and within a
+ * few FP64 ulps of the reference's sequential sums before the FP32 store. */
 
 /* One query ray: origin, tmin, direction, tmax (32 bytes). */
 typedef struct hdcx_ray {
@@ -152,7 +180,7 @@ int hdcx_set_scene(hdcx_tracer* tracer, const hdcx_scene_view* scene);
 
 /* The checks hdcx_set_scene runs before touching the device: indices and
  * material ids in range, finite positions, texture references (plain and
- * UDIM) in range, <= 64 lights, dome importance tables RGBA32F. Returns
+ * UDIM) in range, <= 64 lights, host dome importance tables RGBA32F. Returns
  * HDCX_ERR_INVALID_SCENE with the first violation. Needs no device; the
  * reference has no such check (the driver consumes the buffers as given). */
 int hdcx_validate_scene(const hdcx_scene_view* scene);
@@ -277,6 +305,17 @@ typedef struct hdcx_bvh_options {
 
 int hdcx_set_bvh_options(hdcx_tracer* tracer, const hdcx_bvh_options* options);
 int hdcx_get_bvh_info(hdcx_tracer* tracer, hdcx_bvh_info* out_info);
+
+/* Inspection of the current scene's light tables (no counterpart in the
+ * reference, whose tables are host vectors). hdcx_read_emissive_table
+ * copies the table the device samples (4 floats per triangle of the view,
+ * count = the view's triangle_count). hdcx_read_dome_importance writes the
+ * importance-table size of light `light` (0 x 0: sampled uniformly) and,
+ * for each non-NULL pointer, its marginal CDF (height floats), conditional
+ * CDFs (width x height) and pdf (width x height). */
+int hdcx_read_emissive_table(hdcx_tracer* tracer, float* out, uint32_t count);
+int hdcx_read_dome_importance(hdcx_tracer* tracer, uint32_t light, uint32_t* out_width, uint32_t* out_height,
+                              float* out_marginal, float* out_conditional, float* out_pdf);
 
 /* ---- Batched multi-scene rendering (no counterpart in the reference: one
  * VulkanPathTracer renders one scene per Render call). Many independent
@@ -450,7 +489,8 @@ int hdcx_batch_get_rc_stats(hdcx_batch* batch, hdcx_batch_rc_stats* out_stats);
 #define HDCX_KERNEL_CACHE 5 /* radiance-cache inference */
 #define HDCX_KERNEL_WALK 6  /* subsurface random walks deferred by shade */
 #define HDCX_KERNEL_PAIR 7  /* finds the lanes of a pixel sample with equal next rays */
-#define HDCX_KERNEL_COUNT 8
+#define HDCX_KERNEL_MISS 8  /* environment at misses; queues the hits for shade */
+#define HDCX_KERNEL_COUNT 9
 
 typedef struct hdcx_kernel_stats {
     uint64_t launches[HDCX_KERNEL_COUNT];
@@ -458,7 +498,8 @@ typedef struct hdcx_kernel_stats {
     uint64_t rays[HDCX_KERNEL_COUNT];       /* work items processed: extend counts the
                                                closest-hit rays traced (lanes of a pixel
                                                sample with equal rays trace them once),
-                                               shade the path vertices */
+                                               miss the path vertices, shade the
+                                               vertices on a surface */
     uint64_t nodes_visited;                 /* counted only when counting is on */
     uint64_t triangles_tested;
     uint64_t extend_rays_counted;
@@ -482,7 +523,11 @@ typedef struct hdcx_kernel_stats {
  * inside the shade kernel instead of deferring them to the walk kernel
  * (the same results; kept to check the split against); bit 3: every lane
  * traces its own rays (by default the lanes of a pixel sample trace an
- * equal ray once: the same results; kept to check the sharing against). */
+ * equal ray once: the same results; kept to check the sharing against);
+ * bit 4: shade scenes whose materials have no coat, sheen, transmission,
+ * subsurface or translucency with the full shading kernel (by default a
+ * variant without those lobes runs: the same results; kept to check the
+ * variant against). */
 int hdcx_set_instrumentation(hdcx_tracer* tracer, uint32_t flags);
 int hdcx_get_kernel_stats(hdcx_tracer* tracer, hdcx_kernel_stats* out_stats);
 int hdcx_reset_kernel_stats(hdcx_tracer* tracer);

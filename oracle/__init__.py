@@ -119,6 +119,9 @@ def library() -> ctypes.CDLL:
     lib.oracle_random_walk.argtypes = [c_void_p, c_void_p, c_void_p, c_double, c_double, c_double, POINTER(c_uint32),
                                        c_void_p, c_void_p, c_void_p]
     lib.oracle_random_walk.restype = c_int
+    lib.oracle_emissive_table.argtypes = [c_void_p, c_void_p]
+    lib.oracle_dome_importance.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_void_p,
+                                           c_void_p, c_void_p]
     _lib = lib
     return lib
 
@@ -132,8 +135,29 @@ class OracleScene:
 
     def __init__(self, view):
         self._h = c_void_p()
+        self.triangle_count = int(view.triangle_count)
         if library().oracle_scene_create(ctypes.byref(view), ctypes.byref(self._h)) != 0:
             raise RuntimeError(library().oracle_last_error().decode())
+
+    def emissive_table(self) -> np.ndarray:
+        """(n, 4) float32 rows (cdf, pdf / area, area, 0): the view's table,
+        or the FP64 restatement of the reference's host build."""
+        out = np.empty((self.triangle_count, 4), dtype=np.float32)
+        library().oracle_emissive_table(self._h, out.ctypes.data)
+        return out
+
+    def dome_importance(self, light: int):
+        """(marginal[h], conditional[h, w], pdf[h, w]) or None (uniform)."""
+        w, h = c_uint32(), c_uint32()
+        library().oracle_dome_importance(self._h, light, ctypes.byref(w), ctypes.byref(h), None, None, None)
+        if w.value == 0:
+            return None
+        marginal = np.empty(h.value, dtype=np.float32)
+        conditional = np.empty((h.value, w.value), dtype=np.float32)
+        pdf = np.empty((h.value, w.value), dtype=np.float32)
+        library().oracle_dome_importance(self._h, light, ctypes.byref(w), ctypes.byref(h), marginal.ctypes.data,
+                                         conditional.ctypes.data, pdf.ctypes.data)
+        return marginal, conditional, pdf
 
     def render(self, camera, width: int, height: int, sample_index: int, max_bounces: int, sample_count: int,
                precision: int = 64, rows=None, col_step: int = 1, threads: int = 8) -> np.ndarray:

@@ -96,7 +96,7 @@ public:
         triangleMaterials.assign(v.triangle_materials, v.triangle_materials + triangleCount);
         texcoords.assign(v.texcoords, v.texcoords + static_cast<std::size_t>(triangleCount) * 6);
         normals.assign(v.normals, v.normals + static_cast<std::size_t>(triangleCount) * 12);
-        emissive.assign(v.emissive, v.emissive + static_cast<std::size_t>(triangleCount) * 4);
+        if (v.emissive) emissive.assign(v.emissive, v.emissive + static_cast<std::size_t>(triangleCount) * 4);
         materials.resize(v.material_count);
         std::memcpy(materials.data(), v.materials, sizeof(Material) * v.material_count);
         lightCount = v.light_count;
@@ -122,6 +122,8 @@ public:
             }
             textures.push_back(std::move(t));
         }
+        if (!v.emissive) BuildEmissive(v);
+        BuildDomeImportance();
         BuildBvh();
     }
 
@@ -142,6 +144,158 @@ public:
     }
 
 private:
+    static void LinearRgb(const Texture& t, std::size_t texel, float rgb[3])
+    {
+        for (int c = 0; c < 3; ++c) {
+            const float x = t.texels[texel * 4 + c];
+            rgb[c] = std::isfinite(x) ? std::max(x, 0.0F) : 0.0F;
+        }
+    }
+    // TextureAverageLuminance (:1968-1996): FP64 mean of the texels'
+    // luminance, returned as float.
+    float AverageLuminance(std::uint32_t ti) const
+    {
+        const Texture& t = textures[ti];
+        const std::size_t n = static_cast<std::size_t>(t.width) * t.height;
+        double total = 0.0;
+        for (std::size_t p = 0; p < n; ++p) {
+            float rgb[3];
+            LinearRgb(t, p, rgb);
+            total += 0.2126 * rgb[0] + 0.7152 * rgb[1] + 0.0722 * rgb[2];
+        }
+        return n ? static_cast<float>(total / static_cast<double>(n)) : 0.0F;
+    }
+    // materialEmissionPowers (:2873-2880) from a packed record: the
+    // emission luminance, or the emission texture's mean luminance (a UDIM
+    // set: the mean of its tiles' means), times max(emissionWeight, 0).
+    float MaterialPower(const Material& m) const
+    {
+        const std::uint32_t slot = m.textureIndices0.w;
+        float luminance;
+        if (slot == kMissing) {
+            luminance = 0.2126F * std::max(m.emissionRoughness.x, 0.0F) + 0.7152F * std::max(m.emissionRoughness.y, 0.0F) +
+                        0.0722F * std::max(m.emissionRoughness.z, 0.0F);
+        } else {
+            const bool udim = (slot & 0x80000000U) != 0U;
+            const std::uint32_t first = udim ? (slot & 0xffU) : slot;
+            const std::uint32_t tiles = udim ? static_cast<std::uint32_t>(__builtin_popcount((slot >> 8) & 0x7fffffU)) : 1U;
+            double total = 0.0;
+            for (std::uint32_t k = 0; k < tiles; ++k) total += AverageLuminance(first + k);
+            luminance = static_cast<float>(total / static_cast<double>(tiles));
+        }
+        return std::max(m.transmissionOpacityIor.w, 0.0F) * luminance;
+    }
+    // The emissive-triangle distribution of BuildScene (:2944-2993): weight
+    // = area x material power (area from the FP64 cross product of the FP32
+    // positions), FP64 sequential sums, cdf of the last sampled triangle 1;
+    // triangles past the sampled ones: cdf 1, pdf 0; all zero without a
+    // positive weight.
+    void BuildEmissive(const hdcx_scene_view& v)
+    {
+        const std::uint32_t drawn = v.prototype_count ? v.prototypes[0].first_triangle : triangleCount;
+        const std::uint32_t n = v.light_triangle_count ? v.light_triangle_count : drawn;
+        std::vector<float> power(materials.size());
+        for (std::size_t m = 0; m < materials.size(); ++m)
+            power[m] = v.material_emission_power ? v.material_emission_power[m] : MaterialPower(materials[m]);
+        emissive.assign(static_cast<std::size_t>(triangleCount) * 4, 0.0F);
+        std::vector<double> weight(n, 0.0);
+        std::vector<float> area(n, 0.0F);
+        double total = 0.0;
+        for (std::uint32_t p = 0; p < n; ++p) {
+            const float* a = &positions[static_cast<std::size_t>(indices[3 * p]) * 4];
+            const float* b = &positions[static_cast<std::size_t>(indices[3 * p + 1]) * 4];
+            const float* c = &positions[static_cast<std::size_t>(indices[3 * p + 2]) * 4];
+            double e1[3], e2[3];
+            for (int k = 0; k < 3; ++k) {
+                e1[k] = static_cast<double>(b[k]) - a[k];
+                e2[k] = static_cast<double>(c[k]) - a[k];
+            }
+            const double x = e1[1] * e2[2] - e1[2] * e2[1], y = e1[2] * e2[0] - e1[0] * e2[2],
+                         z = e1[0] * e2[1] - e1[1] * e2[0];
+            const float ar = static_cast<float>(0.5 * std::sqrt(x * x + y * y + z * z));
+            area[p] = std::isfinite(ar) ? ar : 0.0F;
+            const std::uint32_t m = triangleMaterials[p];
+            const double w = static_cast<double>(area[p]) * static_cast<double>(m < power.size() ? power[m] : 0.0F);
+            if (w > 0.0 && std::isfinite(w)) {
+                weight[p] = w;
+                total += w;
+            }
+        }
+        if (!(total > 0.0 && std::isfinite(total))) return;
+        double cdf = 0.0;
+        for (std::uint32_t p = 0; p < n; ++p) {
+            const double q = weight[p] / total;
+            cdf += q;
+            emissive[static_cast<std::size_t>(p) * 4] = p + 1 == n ? 1.0F : static_cast<float>(cdf);
+            emissive[static_cast<std::size_t>(p) * 4 + 1] = area[p] > 0.0F ? static_cast<float>(q) / area[p] : 0.0F;
+            emissive[static_cast<std::size_t>(p) * 4 + 2] = area[p];
+        }
+        for (std::uint32_t p = n; p < triangleCount; ++p) emissive[static_cast<std::size_t>(p) * 4] = 1.0F;
+    }
+    // Lat-long importance tables (:1998-2066) of the texture of every dome
+    // whose importance slot names its own texture, appended as a texture
+    // (r conditional CDF, g marginal CDF, b pdf per steradian) that the
+    // dome's slot then names; without positive weight the dome is sampled
+    // uniformly.
+    void BuildDomeImportance()
+    {
+        std::vector<std::pair<std::uint32_t, std::uint32_t>> built;  // source texture -> table texture
+        for (std::uint32_t l = 0; l < lightCount; ++l) {
+            U4& info = lights[l].textureInfo;
+            if (info.z != 0U || info.w == kMissing || info.w != info.x) continue;
+            std::uint32_t table = kMissing;
+            bool found = false;
+            for (const auto& b : built)
+                if (b.first == info.x) { table = b.second; found = true; }
+            if (!found) {
+                table = ImportanceTable(textures[info.x]);
+                built.emplace_back(info.x, table);
+            }
+            info.w = table;
+        }
+    }
+    std::uint32_t ImportanceTable(const Texture& src)
+    {
+        const std::uint32_t w = static_cast<std::uint32_t>(src.width), h = static_cast<std::uint32_t>(src.height);
+        const std::size_t n = static_cast<std::size_t>(w) * h;
+        std::vector<float> lum(n);
+        std::vector<double> weight(n), rowWeight(h, 0.0);
+        const double pi = 3.14159265358979323846;
+        double total = 0.0;
+        for (std::uint32_t y = 0; y < h; ++y) {
+            const double solidAngle = 2.0 * pi / static_cast<double>(w) *
+                std::abs(std::cos(pi * (1.0 - static_cast<double>(y + 1U) / h)) - std::cos(pi * (1.0 - static_cast<double>(y) / h)));
+            for (std::uint32_t x = 0; x < w; ++x) {
+                const std::size_t p = static_cast<std::size_t>(y) * w + x;
+                float rgb[3];
+                LinearRgb(src, p, rgb);
+                lum[p] = 0.2126F * rgb[0] + 0.7152F * rgb[1] + 0.0722F * rgb[2];
+                weight[p] = static_cast<double>(lum[p]) * solidAngle;
+                rowWeight[y] += weight[p];
+                total += weight[p];
+            }
+        }
+        if (!(total > 0.0 && std::isfinite(total))) return kMissing;
+        Texture t;
+        t.width = src.width;
+        t.height = src.height;
+        t.texels.assign(n * 4, 1.0F);
+        double marginal = 0.0;
+        for (std::uint32_t y = 0; y < h; ++y) {
+            marginal += rowWeight[y] / total;
+            double conditional = 0.0;
+            for (std::uint32_t x = 0; x < w; ++x) {
+                const std::size_t p = static_cast<std::size_t>(y) * w + x;
+                conditional += rowWeight[y] > 0.0 ? weight[p] / rowWeight[y] : 1.0 / static_cast<double>(w);
+                t.texels[p * 4] = x + 1U == w ? 1.0F : static_cast<float>(conditional);
+                t.texels[p * 4 + 1] = y + 1U == h ? 1.0F : static_cast<float>(marginal);
+                t.texels[p * 4 + 2] = static_cast<float>(static_cast<double>(lum[p]) / total);
+            }
+        }
+        textures.push_back(std::move(t));
+        return static_cast<std::uint32_t>(textures.size() - 1);
+    }
+
     // Binned-SAH BVH2 (16 bins per axis over the centroid bounds, node and
     // triangle costs 1, leaves of at most 4 triangles; a range whose
     // centroids coincide is halved). Only a culling structure: which
@@ -2042,6 +2196,36 @@ int oracle_random_walk(void* sp, const double* entry, const double* out_n, doubl
     exit_d[0] = ed.x; exit_d[1] = ed.y; exit_d[2] = ed.z;
     *att = a;
     return exited ? 1 : 0;
+}
+
+// The scene's emissive-triangle table (4 floats per triangle: cdf,
+// pdf / area, area, 0), as given in the view or built by BuildEmissive.
+void oracle_emissive_table(void* sp, float* out)
+{
+    const auto& sc = static_cast<OracleScene*>(sp)->scene;
+    std::copy(sc.emissive.begin(), sc.emissive.end(), out);
+}
+// Importance-table size of light `light` (0 x 0: sampled uniformly) and,
+// for each non-NULL pointer, its marginal CDF (height floats), conditional
+// CDFs and pdf (width x height floats).
+void oracle_dome_importance(void* sp, std::uint32_t light, std::uint32_t* width, std::uint32_t* height,
+                            float* marginal, float* conditional, float* pdf)
+{
+    const auto& sc = static_cast<OracleScene*>(sp)->scene;
+    const std::uint32_t ti = light < sc.lightCount && sc.lights[light].textureInfo.z == 0U
+                                 ? sc.lights[light].textureInfo.w : oracle::kMissing;
+    *width = *height = 0;
+    if (ti == oracle::kMissing) return;
+    const auto& t = sc.textures[ti];
+    *width = static_cast<std::uint32_t>(t.width);
+    *height = static_cast<std::uint32_t>(t.height);
+    const std::size_t n = static_cast<std::size_t>(t.width) * t.height;
+    for (std::size_t p = 0; p < n; ++p) {
+        if (conditional) conditional[p] = t.texels[p * 4];
+        if (pdf) pdf[p] = t.texels[p * 4 + 2];
+    }
+    if (marginal)
+        for (int y = 0; y < t.height; ++y) marginal[y] = t.texels[static_cast<std::size_t>(y) * t.width * 4 + 1];
 }
 
 } // extern "C"

@@ -82,3 +82,46 @@ def cpu_threads() -> int:
     if omp.isdigit() and int(omp) > 0:
         n = min(n, int(omp))
     return max(1, n)
+
+
+def texture_mean_luminance(view, i: int) -> float:
+    """Mean linear luminance of texture i of a view (numpy, FP64)."""
+    import ctypes
+
+    t = view.textures[i]
+    n = t.width * t.height * 4
+    if t.format == 2:
+        x = np.ctypeslib.as_array(ctypes.cast(t.texels, ctypes.POINTER(ctypes.c_float)), shape=(n,)).astype(np.float64)
+    else:
+        b = np.ctypeslib.as_array(ctypes.cast(t.texels, ctypes.POINTER(ctypes.c_uint8)), shape=(n,)) / 255.0
+        x = np.where(b <= 0.04045, b / 12.92, ((b + 0.055) / 1.055) ** 2.4) if t.format == 1 else b
+    rgb = np.clip(x.reshape(-1, 4)[:, :3], 0, None)
+    return float((rgb @ np.array([0.2126, 0.7152, 0.0722])).mean())
+
+
+def with_emission_texture(scene, slot: int):
+    """A copy of the scene's view whose emissive materials take their
+    emission from texture slot `slot` (a plain index or a UDIM set), with no
+    per-material powers: the device and the oracle derive them from the
+    records. Returns (view, keep-alive buffers, expected powers)."""
+    import ctypes
+
+    import hdcodex_amd as hx
+
+    v = scene.view
+    view = hx.SceneView()
+    ctypes.memmove(ctypes.byref(view), ctypes.byref(v), ctypes.sizeof(view))
+    mats = np.frombuffer(ctypes.string_at(v.materials, 368 * v.material_count), dtype=np.uint8).copy()
+    f = mats.view(np.float32).reshape(-1, 92)
+    u = mats.view(np.uint32).reshape(-1, 92)
+    emitters = f[:, 11] > 0
+    u[emitters, 71] = slot
+    if slot & 0x80000000:
+        tiles = [(slot & 0xFF) + k for k in range(bin((slot >> 8) & 0x7FFFFF).count("1"))]
+    else:
+        tiles = [slot]
+    lum = np.float32(np.mean([np.float32(texture_mean_luminance(v, t)) for t in tiles]))
+    power = np.where(emitters, np.maximum(f[:, 11], 0) * lum, 0.0).astype(np.float32)
+    view.materials = mats.ctypes.data
+    view.material_emission_power = None
+    return view, mats, power

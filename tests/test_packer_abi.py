@@ -13,7 +13,7 @@ import pytest
 
 import hdcodex_amd as hx
 import oracle
-from helpers import rel_mse
+from helpers import rel_mse, with_emission_texture
 
 ROOT = Path(__file__).resolve().parent.parent
 MISSING = 0xFFFFFFFF
@@ -91,7 +91,10 @@ def test_cornell_packing_follows_id_order():
     assert Lu[0, 54] == 1  # type Rect
     assert L[0, 51] == pytest.approx(1.0 * 100.0 * 0.0625)
     assert L[0, 7] == pytest.approx(0.0625)  # controls.w = area
-    assert np.all(a["emissive"] == 0.0)
+    # No emissive material: the device-built table (restated by the oracle)
+    # is all zero, so no triangle is sampled as a light.
+    assert a["emissive"] is None and np.all(a["material_emission_power"] == 0.0)
+    assert np.all(oracle.OracleScene(hx.ProceduralScene("cornell", 0).view).emissive_table() == 0.0)
 
 
 def test_texture_order_plain_then_udim_sets():
@@ -110,33 +113,110 @@ def test_texture_order_plain_then_udim_sets():
     assert (x >> 8) & 0x7FFFFF == 0b11 and (x & 0xFF) < len(m["textures"])
 
 
-def test_dome_importance_texture_is_a_normalised_distribution():
+def test_dome_importance_slot_names_the_dome_texture():
+    # The packer leaves the importance tables to the device: the dome's
+    # importance slot (textureInfo.w) names its own texture (textureInfo.x).
     s = hx.ProceduralScene("heightfield", 0, 16, 64)
-    v = s.view
-    fmt = [(v.textures[i].width, v.textures[i].height, v.textures[i].format) for i in range(v.texture_count)]
-    L = np.frombuffer(ctypes.string_at(v.lights, 224), dtype=np.uint32)
-    imp = int(L[55])
-    assert imp != MISSING and fmt[imp][2] == 2
-    w, h = fmt[imp][:2]
-    t = np.ctypeslib.as_array(ctypes.cast(v.textures[imp].texels, ctypes.POINTER(ctypes.c_float)), shape=(h, w, 4))
-    assert np.all(t[:, -1, 0] == 1.0) and t[-1, 0, 1] == 1.0
-    assert np.all(np.diff(t[:, :, 0], axis=1) >= 0) and np.all(np.diff(t[:, 0, 1]) >= 0)
-    # Sum of pdf x texel solid angle = 1 (luminance x solid angle weights).
+    L = np.frombuffer(ctypes.string_at(s.view.lights, 224), dtype=np.uint32)
+    assert L[54] == 0 and L[55] == L[52] != MISSING
+    assert s.view.emissive is None or not s.view.emissive
+
+
+def _row_solid_angles(h: int, w: int) -> np.ndarray:
     v0 = np.arange(h) / h
-    solid = 2 * math.pi / w * np.abs(np.cos(math.pi * (1 - (v0 + 1 / h))) - np.cos(math.pi * (1 - v0)))
-    assert float((t[:, :, 2] * solid[:, None]).sum()) == pytest.approx(1.0, rel=1e-5)
+    return 2 * math.pi / w * np.abs(np.cos(math.pi * (1 - (v0 + 1 / h))) - np.cos(math.pi * (1 - v0)))
 
 
-def test_emissive_cdf_is_a_distribution():
-    a = hx.ProceduralScene("materials", 0).arrays()
-    e = a["emissive"]
-    assert e[-1, 0] == 1.0
-    assert np.all(np.diff(e[:, 0]) >= -1e-7)
-    assert float((e[:, 1].astype(np.float64) * e[:, 2]).sum()) == pytest.approx(1.0, rel=1e-5)
-    mats = a["triangle_materials"]
+def test_dome_importance_tables_are_one_distribution():
+    """The oracle's restatement of the importance build (:1998-2066) on the
+    2048 x 1024 benchmark dome: the three planes describe ONE distribution.
+    pdf x texel solid angle sums to 1; each marginal step is its row's pdf
+    mass; each conditional step is the texel's share of its row; the pdf is
+    proportional to the texel luminance recomputed here from the texture."""
+    s = hx.ProceduralScene("heightfield", 0, 16, 2048)
+    o = oracle.OracleScene(s.view)
+    marginal, cond, pdf = o.dome_importance(0)
+    h, w = pdf.shape
+    assert (w, h) == (2048, 1024)
+    solid = _row_solid_angles(h, w)
+    mass = pdf.astype(np.float64) * solid[:, None]
+    assert float(mass.sum()) == pytest.approx(1.0, rel=1e-6)
+    assert marginal[-1] == 1.0 and np.all(cond[:, -1] == 1.0)
+    assert np.all(np.diff(marginal) >= 0) and np.all(np.diff(cond, axis=1) >= 0)
+    row_mass = mass.sum(axis=1)
+    assert np.allclose(np.diff(np.concatenate([[0.0], marginal.astype(np.float64)])), row_mass, atol=2e-7)
+    share = mass / np.where(row_mass > 0, row_mass, 1.0)[:, None]
+    assert np.allclose(np.diff(np.concatenate([np.zeros((h, 1)), cond.astype(np.float64)], axis=1), axis=1), share,
+                       atol=2e-7)
+    v = s.view
+    tex = np.ctypeslib.as_array(ctypes.cast(v.textures[int(np.frombuffer(ctypes.string_at(v.lights, 224),
+                                                                          dtype=np.uint32)[52])].texels,
+                                            ctypes.POINTER(ctypes.c_float)), shape=(h, w, 4))
+    lum = np.clip(tex[..., :3].astype(np.float64), 0, None) @ np.array([0.2126, 0.7152, 0.0722])
+    ratio = pdf / np.where(lum > 0, lum, 1.0)
+    assert np.allclose(ratio[lum > 1e-3], ratio[lum > 1e-3].mean(), rtol=1e-5)
+
+
+def test_emissive_table_is_the_area_power_distribution():
+    """The oracle's restatement of the emissive-triangle build (:2944-2993):
+    probability = area x material power / total, recomputed here from the
+    positions and the packer's per-material powers; the cdf is the running
+    sum of those probabilities with its last entry 1."""
+    s = hx.ProceduralScene("materials", 0)
+    a = s.arrays()
+    e = oracle.OracleScene(s.view).emissive_table()
+    p = a["positions"][:, :3].astype(np.float64)[a["indices"]]
+    area = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+    power = a["material_emission_power"][a["triangle_materials"]].astype(np.float64)
+    weight = area.astype(np.float32).astype(np.float64) * power
+    prob = weight / weight.sum()
+    assert np.count_nonzero(prob) > 0 and np.count_nonzero(prob) < len(prob)
+    assert np.allclose(e[:, 2], area, rtol=1e-6)
+    assert np.allclose(e[:, 1].astype(np.float64) * e[:, 2], prob, rtol=1e-6, atol=1e-12)
+    assert e[-1, 0] == 1.0 and np.allclose(e[:, 0], np.cumsum(prob), atol=2e-7)
     f, _ = _materials(a)
-    emissive_mat = f[mats, 11] > 0  # emissionWeight
-    assert np.all((e[:, 1] > 0) == emissive_mat)
+    assert np.all((e[:, 1] > 0) == (f[a["triangle_materials"], 11] * power > 0))
+
+
+@pytest.mark.parametrize("slot_kind", ["color", "plain", "udim"])
+def test_material_power_rule_from_records(slot_kind):
+    """A view without per-material powers derives them from its records:
+    max(emissionWeight, 0) x the luminance of the emission color, or the
+    emission texture's mean texel luminance (a UDIM set: mean of its tiles'
+    means), recomputed here with numpy from the texels."""
+    s = hx.ProceduralScene("materials", 0)
+    a = s.arrays()
+    _, mu = _materials(a)
+    if slot_kind == "color":
+        view = hx.SceneView()
+        ctypes.memmove(ctypes.byref(view), ctypes.byref(s.view), ctypes.sizeof(view))
+        view.material_emission_power = None
+        keep, power = None, a["material_emission_power"]
+    else:
+        slots = [int(x) for x in mu[:, 68:92].reshape(-1) if x != MISSING]
+        slot = next(x for x in slots if bool(x & 0x80000000) == (slot_kind == "udim"))
+        view, keep, power = with_emission_texture(s, slot)
+    got = oracle.OracleScene(view).emissive_table()
+    want_view = hx.SceneView()
+    ctypes.memmove(ctypes.byref(want_view), ctypes.byref(view), ctypes.sizeof(view))
+    p = np.ascontiguousarray(power, dtype=np.float32)
+    want_view.material_emission_power = p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    want = oracle.OracleScene(want_view).emissive_table()
+    assert np.any(want[:, 1] > 0)
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=0)
+    del keep
+
+
+def test_validation_of_device_table_inputs():
+    s = hx.ProceduralScene("cornell", 0)
+    view = hx.SceneView()
+    ctypes.memmove(ctypes.byref(view), ctypes.byref(s.view), ctypes.sizeof(view))
+    view.light_triangle_count = view.triangle_count + 1
+    with pytest.raises(hx.HdcxError) as e:
+        hx.validate_scene(view)
+    assert e.value.code == hx.ERR_INVALID_SCENE and "light_triangle_count" in str(e.value)
+    view.light_triangle_count = view.triangle_count
+    hx.validate_scene(view)
 
 
 def _corrupted(scene, field, value, index=0):
