@@ -78,12 +78,16 @@ def algorithmic_bytes(stats: dict) -> dict:
     return out
 
 
-# Bytes the shade kernel moves per path vertex (single-level scene, no
+# Bytes the miss kernel moves per path vertex (every vertex of the bounce):
+MISS_VERTEX_READ = 20     # queue entry 4, hit 16
+MISS_READ = 40            # on a miss: flags 4, MIS pdfs 8, throughput 4, radiance 4, direction 16, wavelength 4
+MISS_WRITE = 4            # on a miss: radiance (environment added)
+MISS_HIT_WRITE = 4        # on a hit: hit-queue entry
+# Bytes the shade kernel moves per surface hit (single-level scene, no
 # radiance cache), from its loads and stores:
-SHADE_VERTEX_READ = 96    # queue entry 4, hit 16, ray origin 16 + direction 16, rng 4, wavelength 4,
+SHADE_VERTEX_READ = 96    # hit-queue entry 4, hit 16, ray origin 16 + direction 16, rng 4, wavelength 4,
                           # throughput 4, radiance 4, flags 4, MIS pdfs 8, medium 16
 SHADE_VERTEX_WRITE = 12   # the three shadow-slot light ids cleared
-SHADE_MISS_WRITE = 4      # radiance (environment on a miss)
 SHADE_HIT_READ = 136      # three vertex indices 12, positions 3 x 16, corner normals 3 x 16, UVs 3 x 8,
                           # triangle material id 4
 SHADE_HIT_WRITE = 12      # rng, throughput, radiance
@@ -91,23 +95,31 @@ SHADE_KEPT_WRITE = 64     # next ray origin + direction 32, flags 4, MIS pdfs 8,
 SHADE_SHADOW_WRITE = 40   # shadow slot: origin + tmax 16, direction + contribution 16, light 4; queue entry 4
 
 
-def shade_bytes(timed: dict, counted: dict) -> dict:
+def miss_bytes(timed: dict) -> dict:
+    """Algorithmic bytes of the miss launches of the timed steps (path state
+    of every vertex; the dome texels a miss reads are a scene-wide table in
+    L2/MALL and are not counted)."""
+    items = timed["miss"]["items"]
+    hits = timed["shade"]["items"]
+    misses = max(items - hits, 0)
+    total = items * MISS_VERTEX_READ + misses * (MISS_READ + MISS_WRITE) + hits * MISS_HIT_WRITE
+    return {"bytes": total, "bytes_per_vertex": total / max(items, 1), "vertices": items,
+            "hit_fraction": hits / max(items, 1)}
+
+
+def shade_bytes(timed: dict) -> dict:
     """Algorithmic bytes of the shade launches of the timed steps: the bytes
-    unique to each path vertex (its path state, hit record, triangle
+    unique to each surface hit (its path state, hit record, triangle
     attributes and the state, queue and shadow-ray records it writes).
     Scene-wide tables that every vertex reads (material records, light
     records, dome texels and importance tables, textures: a few MB in
-    L2/MALL) are not counted per vertex. The hit fraction comes from the
-    counting pass over the same frame."""
-    items = timed["shade"]["items"]
-    hit_frac = counted["extend_hits_counted"] / max(counted["shade"]["items"], 1)
-    hits = items * hit_frac
-    kept = max(items - timed["raygen"]["items"], 0)  # every later vertex was kept by the one before
+    L2/MALL) are not counted per vertex."""
+    hits = timed["shade"]["items"]
+    kept = max(timed["miss"]["items"] - timed["raygen"]["items"], 0)  # every later vertex was kept by the one before
     queued = timed["shadow_queued_paths"]
-    total = (items * (SHADE_VERTEX_READ + SHADE_VERTEX_WRITE) + (items - hits) * SHADE_MISS_WRITE
-             + hits * (SHADE_HIT_READ + SHADE_HIT_WRITE) + kept * SHADE_KEPT_WRITE + queued * SHADE_SHADOW_WRITE)
-    return {"bytes": total, "bytes_per_vertex": total / max(items, 1), "vertices": items, "hit_fraction": hit_frac,
-            "shadow_queued_paths": queued}
+    total = (hits * (SHADE_VERTEX_READ + SHADE_VERTEX_WRITE + SHADE_HIT_READ + SHADE_HIT_WRITE)
+             + kept * SHADE_KEPT_WRITE + queued * SHADE_SHADOW_WRITE)
+    return {"bytes": total, "bytes_per_hit": total / max(hits, 1), "hits": hits, "shadow_queued_paths": queued}
 
 
 def cpu_share() -> dict:
@@ -573,7 +585,7 @@ def instanced_workload(hx, frames: int = 2) -> dict:
             "instances": scene.instance_count, "instanced_triangles": 10000 * 20480,
             "stored_triangles": scene.triangle_count, "ms_per_frame": ms, "mpaths_per_s": paths / (ms * 1e-3) / 1e6,
             "mrays_per_s": (timed["extend"]["items"] + timed["shadow"]["items"]) / (sum(times)) / 1e6,
-            "kernels_ms_per_frame": {k: timed[k]["ms"] / frames for k in ("extend", "shade", "walk", "shadow")},
+            "kernels_ms_per_frame": {k: timed[k]["ms"] / frames for k in ("extend", "miss", "shade", "walk", "shadow")},
             "traversal": traversal,
             "set_scene_s": set_scene_s, "bvh": {k: info[k] for k in ("node_count", "max_stack", "depth", "build_ms")},
             "frame_mean_rgb": [float(x) for x in frame[..., :3].reshape(-1, 3).mean(axis=0)]}
@@ -657,14 +669,17 @@ def main() -> int:
     other.set_scene(scene)
     other_bvh = other.bvh_info()
     other.close()
-    # set_scene again on a fresh tracer of the same builder: the process's
-    # one-time HIP set-up is paid; upload, BVH build and light tables remain.
-    again = hx.PathTracer(0)
-    again.set_bvh_options(hx.BVH_DEVICE_LBVH if args.bvh == "lbvh" else hx.BVH_HOST_SAH, 2048, True)
-    t0 = time.perf_counter()
-    again.set_scene(scene)
-    set_scene_repeat_s = time.perf_counter() - t0
-    again.close()
+    # set_scene on fresh tracers, the process's one-time HIP set-up paid:
+    # upload, validation, BVH build and the device light tables, with each
+    # builder.
+    set_scene_warm_s = {}
+    for name, builder in (("device_lbvh", hx.BVH_DEVICE_LBVH), ("host_sah", hx.BVH_HOST_SAH)):
+        again = hx.PathTracer(0)
+        again.set_bvh_options(builder, 2048, True)
+        t0 = time.perf_counter()
+        again.set_scene(scene)
+        set_scene_warm_s[name] = time.perf_counter() - t0
+        again.close()
     log(f"set_scene {set_scene_s:.2f} s, {scene.triangle_count} triangles")
 
     for i in range(args.warmup):
@@ -712,8 +727,8 @@ def main() -> int:
             avg_s = timed[k]["ms"] * 1e-3 / timed[k]["launches"]
             kernels[k]["hbm_GBs"] = traffic / avg_s / 1e9
             kernels[k]["hbm_frac"] = kernels[k]["hbm_GBs"] / HBM_PEAK_GBS
-    dominant = max(("extend", "shade", "shadow"), key=lambda k: timed[k]["ms"])
-    # Rooflines of the three path kernels: algorithmic bytes of the timed
+    dominant = max(("extend", "miss", "shade", "shadow"), key=lambda k: timed[k]["ms"])
+    # Rooflines of the four path kernels: algorithmic bytes of the timed
     # launches over their HIP-event time.
     roof = {}
     for k in ("extend", "shadow"):
@@ -722,8 +737,11 @@ def main() -> int:
             roof[k] = {**alg[k], "alg_bytes": rays * alg[k]["bytes_per_ray"], "rays_per_launch":
                        rays / max(timed[k]["launches"], 1)}
     if timed["shade"]["ms"] > 0:
-        roof["shade"] = shade_bytes(timed, counted)
+        roof["shade"] = shade_bytes(timed)
         roof["shade"]["alg_bytes"] = roof["shade"].pop("bytes")
+    if timed["miss"]["ms"] > 0:
+        roof["miss"] = miss_bytes(timed)
+        roof["miss"]["alg_bytes"] = roof["miss"].pop("bytes")
     for k, r in roof.items():
         launches = max(timed[k]["launches"], 1)
         r["achieved_GBs"] = r["alg_bytes"] / (timed[k]["ms"] * 1e-3) / 1e9
@@ -762,7 +780,7 @@ def main() -> int:
                            "note": "same frame with the 33 MB RGBA32F readback to host memory"},
         "lane_paths_per_s_M": 3 * mpaths,
         "set_scene_s": set_scene_s,
-        "set_scene_repeat_s": set_scene_repeat_s,
+        "set_scene_warm_s": set_scene_warm_s,
         "bvh": {"builder": args.bvh, "build_ms": bvh["build_ms"], "nodes": bvh["node_count"],
                 "max_stack": bvh["max_stack"], "depth": bvh["depth"], "top_clusters": bvh["top_clusters"],
                 "other_builder": "sah" if args.bvh == "lbvh" else "lbvh", "other_build_ms": other_bvh["build_ms"],
