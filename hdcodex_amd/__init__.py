@@ -39,7 +39,7 @@ EXPORTED_SYMBOLS = (
     "hdcx_get_shading_mode", "hdcx_render", "hdcx_read_film", "hdcx_set_sample_offset", "hdcx_has_geometry", "hdcx_trace_closest",
     "hdcx_trace_occluded", "hdcx_scene_create_procedural", "hdcx_scene_view_get", "hdcx_scene_camera_get",
     "hdcx_scene_destroy", "hdcx_scene_flatten", "hdcx_bvh_build_host", "hdcx_set_instrumentation", "hdcx_get_kernel_stats", "hdcx_reset_kernel_stats",
-    "hdcx_read_emissive_table", "hdcx_read_dome_importance",
+    "hdcx_read_emissive_table", "hdcx_read_dome_importance", "hdcx_read_film_upscaled",
 )
 
 
@@ -147,6 +147,7 @@ def library() -> ctypes.CDLL:
     lib.hdcx_set_instrumentation.argtypes = [c_void_p, c_uint32]
     lib.hdcx_get_kernel_stats.argtypes = [c_void_p, POINTER(KernelStats)]
     lib.hdcx_read_emissive_table.argtypes = [c_void_p, c_void_p, c_uint32]
+    lib.hdcx_read_film_upscaled.argtypes = [c_void_p, c_uint32, c_uint32, c_void_p, c_int]
     lib.hdcx_read_dome_importance.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_void_p,
                                               c_void_p, c_void_p]
     lib.hdcx_reset_kernel_stats.argtypes = [c_void_p]
@@ -315,6 +316,12 @@ class PathTracer:
     def read_film(self, width: int, height: int) -> np.ndarray:
         out = np.empty((height, width, 4), dtype=np.float32)
         _check(library().hdcx_read_film(self._h, out.ctypes.data))
+        return out
+
+    def read_film_upscaled(self, width: int, height: int) -> np.ndarray:
+        """The last film upscaled by nearest neighbour on the device."""
+        out = np.empty((height, width, 4), dtype=np.float32)
+        _check(library().hdcx_read_film_upscaled(self._h, width, height, out.ctypes.data, 0))
         return out
 
     def emissive_table(self, triangle_count: int) -> np.ndarray:

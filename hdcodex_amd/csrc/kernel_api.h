@@ -83,6 +83,10 @@ hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch,
 hipError_t LaunchConsumeQueries(const CacheParams& cp, hipStream_t st);
 hipError_t LaunchResolve(double4* mean, const double4* batch, float4* out, std::uint32_t pixels, std::uint32_t sampleIndex,
                          std::uint32_t sampleCount, unsigned maxBlocks, hipStream_t st);
+// Nearest-neighbour upscale of a w0 x h0 RGBA32F film to w x h
+// (src/hydra/render_pass.cpp:45-68).
+hipError_t LaunchUpscaleNearest(const float4* src, std::uint32_t sw, std::uint32_t sh, float4* dst, std::uint32_t w,
+                                std::uint32_t h, unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchZero(double4* p, std::uint32_t n, unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchQueryClosest(const DeviceScene& s, const float4* rays, std::uint32_t n, float4* hits, unsigned maxBlocks,
                               hipStream_t st);

@@ -690,6 +690,21 @@ __global__ void resolveKernel(double4* mean, const double4* batch, float4* out, 
     }
 }
 
+// UpscaleNearest (src/hydra/render_pass.cpp:45-68) of an RGBA32F film: one
+// thread per destination pixel, the reference's unsigned 32-bit index
+// arithmetic.
+__global__ void upscaleNearestKernel(const float4* src, std::uint32_t sw, std::uint32_t sh, float4* dst, std::uint32_t w,
+                                     std::uint32_t h)
+{
+    const std::size_t n = static_cast<std::size_t>(w) * h;
+    for (std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<std::size_t>(gridDim.x) * blockDim.x) {
+        const std::uint32_t y = static_cast<std::uint32_t>(i / w), x = static_cast<std::uint32_t>(i % w);
+        const std::uint32_t sy = min(sh - 1U, y * sh / h), sx = min(sw - 1U, x * sw / w);
+        dst[i] = src[static_cast<std::size_t>(sy) * sw + sx];
+    }
+}
+
 __global__ void zeroDoubleKernel(double4* p, std::uint32_t n)
 {
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -873,6 +888,12 @@ hipError_t LaunchResolve(double4* mean, const double4* batch, float4* out, std::
 {
     hipLaunchKernelGGL(resolveKernel, dim3(GridFor(pixels, maxBlocks)), dim3(kBlock), 0, st, mean, batch, out, pixels,
                        sampleIndex, sampleCount);
+    return hipGetLastError();
+}
+hipError_t LaunchUpscaleNearest(const float4* src, std::uint32_t sw, std::uint32_t sh, float4* dst, std::uint32_t w,
+                                std::uint32_t h, unsigned maxBlocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(upscaleNearestKernel, dim3(GridFor(w * h, maxBlocks)), dim3(kBlock), 0, st, src, sw, sh, dst, w, h);
     return hipGetLastError();
 }
 hipError_t LaunchZero(double4* p, std::uint32_t n, unsigned maxBlocks, hipStream_t st)

@@ -54,7 +54,7 @@ struct hdcx_tracer : SceneStore, PathPool {
     bool fallbackZUp = false;
     DeviceBuffer traversalCounters;
     // Film.
-    DeviceBuffer filmMean, filmBatch, filmOut;
+    DeviceBuffer filmMean, filmBatch, filmOut, filmUpscaled;
     std::uint32_t filmWidth = 0, filmHeight = 0;
     // Ray queries.
     DeviceBuffer queryRays, queryHits;
@@ -590,6 +590,32 @@ int hdcx_read_film(hdcx_tracer* tracer, float* out_rgba)
         if (!tracer->filmOut.bytes()) throw Failure(HDCX_ERR_NO_GEOMETRY, "nothing rendered yet");
         tracer->Activate();
         Check(hipMemcpy(out_rgba, tracer->filmOut.as<void>(), tracer->filmOut.bytes(), hipMemcpyDeviceToHost), "readback");
+    });
+}
+
+int hdcx_read_film_upscaled(hdcx_tracer* tracer, uint32_t width, uint32_t height, float* out_rgba, int out_on_device)
+{
+    return Guard([&] {
+        if (!tracer || !out_rgba) throw std::invalid_argument("null argument");
+        if (width == 0 || height == 0) throw std::invalid_argument("empty upscale target");
+        if (static_cast<std::uint64_t>(width) * height >= (1ULL << 32) ||
+            static_cast<std::uint64_t>(width) * tracer->filmWidth >= (1ULL << 32) ||
+            static_cast<std::uint64_t>(height) * tracer->filmHeight >= (1ULL << 32))
+            throw std::invalid_argument("upscale size beyond 32-bit index arithmetic");
+        if (!tracer->filmOut.bytes()) throw Failure(HDCX_ERR_NO_GEOMETRY, "nothing rendered yet");
+        tracer->Activate();
+        const std::size_t bytes = static_cast<std::size_t>(width) * height * sizeof(float4);
+        float4* dst = reinterpret_cast<float4*>(out_rgba);
+        if (!out_on_device) {
+            tracer->filmUpscaled.Reserve(bytes);
+            dst = tracer->filmUpscaled.as<float4>();
+        }
+        Check(LaunchUpscaleNearest(tracer->filmOut.as<float4>(), tracer->filmWidth, tracer->filmHeight, dst, width, height,
+                                   tracer->maxBlocks, tracer->stream),
+              "upscale");
+        if (!out_on_device)
+            Check(hipMemcpyAsync(out_rgba, dst, bytes, hipMemcpyDeviceToHost, tracer->stream), "upscaled readback");
+        Check(hipStreamSynchronize(tracer->stream), "upscale");
     });
 }
 

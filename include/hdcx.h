@@ -212,6 +212,18 @@ int hdcx_set_sample_offset(hdcx_tracer* tracer, uint32_t offset);
 /* Copies the current device film (RGBA32F, width*height*4) to out_rgba. */
 int hdcx_read_film(hdcx_tracer* tracer, float* out_rgba);
 
+/* The current film upscaled to width x height by nearest neighbour on the
+ * device: replaces the render pass's host UpscaleNearest
+ * (src/hydra/render_pass.cpp:45-68) of the interactive half-resolution
+ * frame (:191-208), pixel for pixel (destination (x, y) takes source
+ * (min(w0-1, x*w0/width), min(h0-1, y*h0/height)), unsigned 32-bit
+ * arithmetic). out_rgba holds width*height*4 floats: host memory, or, with
+ * out_on_device != 0, device memory of the tracer's device (written by the
+ * kernel, no host copy). Sizes whose index products reach 2^32 are
+ * rejected (HDCX_ERR_INVALID_ARGUMENT). */
+int hdcx_read_film_upscaled(hdcx_tracer* tracer, uint32_t width, uint32_t height, float* out_rgba,
+                            int out_on_device);
+
 /* Replaces VulkanPathTracer::HasGeometry (vulkan_path_tracer.h:47).
  * Returns 1 or 0; 0 for a NULL tracer. */
 int hdcx_has_geometry(const hdcx_tracer* tracer);

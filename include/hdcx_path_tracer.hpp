@@ -98,6 +98,31 @@ public:
         return pixels;
     }
 
+    // Render at traceWidth x traceHeight, then the film upscaled by nearest
+    // neighbour to width x height on the device: what the render pass's
+    // interactive frame (src/hydra/render_pass.cpp:191-208) computes with
+    // Render + UpscaleNearest. Empty as Render is.
+    [[nodiscard]] std::vector<float> RenderUpscaled(const PathTracerCamera& camera, std::uint32_t traceWidth,
+                                                    std::uint32_t traceHeight, std::uint32_t sampleIndex,
+                                                    std::uint32_t maxBounces, std::uint32_t sampleCount,
+                                                    std::uint32_t width, std::uint32_t height)
+    {
+        if (!HasGeometry() || traceWidth == 0 || traceHeight == 0 || width == 0 || height == 0) return {};
+        hdcx_camera c{};
+        for (int k = 0; k < 3; ++k) {
+            c.origin[k] = camera.origin[k];
+            c.lower_left[k] = camera.lowerLeft[k];
+            c.horizontal[k] = camera.horizontal[k];
+            c.vertical[k] = camera.vertical[k];
+        }
+        const int code = hdcx_render(tracer_, &c, traceWidth, traceHeight, sampleIndex, maxBounces, sampleCount, nullptr);
+        if (code == HDCX_ERR_NO_GEOMETRY) return {};
+        Check(code);
+        std::vector<float> pixels(static_cast<std::size_t>(width) * height * 4U);
+        Check(hdcx_read_film_upscaled(tracer_, width, height, pixels.data(), 0));
+        return pixels;
+    }
+
     [[nodiscard]] bool HasGeometry() const noexcept { return hdcx_has_geometry(tracer_) != 0; }
 
     [[nodiscard]] hdcx_tracer* Native() const noexcept { return tracer_; }

@@ -120,6 +120,7 @@ def library() -> ctypes.CDLL:
                                        c_void_p, c_void_p, c_void_p]
     lib.oracle_random_walk.restype = c_int
     lib.oracle_emissive_table.argtypes = [c_void_p, c_void_p]
+    lib.oracle_upscale_nearest.argtypes = [c_void_p, c_uint32, c_uint32, c_uint32, c_uint32, c_void_p]
     lib.oracle_dome_importance.argtypes = [c_void_p, c_uint32, POINTER(c_uint32), POINTER(c_uint32), c_void_p,
                                            c_void_p, c_void_p]
     _lib = lib
@@ -354,6 +355,14 @@ def transmission(rd, n, front: bool, ior: float, thin: bool, state: int):
     crossed = library().oracle_transmission(a.ctypes.data, b.ctypes.data, int(front), ior, int(thin), ctypes.byref(st),
                                             out.ctypes.data)
     return bool(crossed), out, st.value
+
+
+def upscale_nearest(source: np.ndarray, width: int, height: int) -> np.ndarray:
+    """UpscaleNearest of an (h0, w0, 4) float32 film to (height, width, 4)."""
+    src = np.ascontiguousarray(source, dtype=np.float32)
+    out = np.empty((height, width, 4), dtype=np.float32)
+    library().oracle_upscale_nearest(src.ctypes.data, src.shape[1], src.shape[0], width, height, out.ctypes.data)
+    return out
 
 
 def lat_long_direction(u: float, v: float) -> np.ndarray:

@@ -2228,4 +2228,20 @@ void oracle_dome_importance(void* sp, std::uint32_t light, std::uint32_t* width,
         for (int y = 0; y < t.height; ++y) marginal[y] = t.texels[static_cast<std::size_t>(y) * t.width * 4 + 1];
 }
 
+// UpscaleNearest (src/hydra/render_pass.cpp:45-68): destination pixel
+// (x, y) copies source pixel (min(sw - 1, x * sw / w), min(sh - 1, y * sh / h)),
+// unsigned 32-bit arithmetic; RGBA floats.
+void oracle_upscale_nearest(const float* source, std::uint32_t sw, std::uint32_t sh, std::uint32_t w, std::uint32_t h,
+                            float* out)
+{
+    for (std::uint32_t y = 0; y < h; ++y) {
+        const std::uint32_t sy = std::min(sh - 1U, y * sh / h);
+        for (std::uint32_t x = 0; x < w; ++x) {
+            const std::uint32_t sx = std::min(sw - 1U, x * sw / w);
+            std::memcpy(out + (static_cast<std::size_t>(y) * w + x) * 4, source + (static_cast<std::size_t>(sy) * sw + sx) * 4,
+                        4 * sizeof(float));
+        }
+    }
+}
+
 } // extern "C"
