@@ -47,6 +47,7 @@ struct Plan {
 
 struct GraphKey {
     const void* pool = nullptr;
+    const void* optional[3] = {};  // walk slots, path spread, scene index (allocated on demand)
     const void* tables[7] = {};
     std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0, twoLevel = 0, walk = 0;
     std::uint32_t scenes = 0;  // also keeps the struct free of implicit padding (keys compare with memcmp)
@@ -55,7 +56,7 @@ struct GraphKey {
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 static_assert(sizeof(CacheParams) == 10 * 4 + 10 * 8, "CacheParams has implicit padding");
-static_assert(sizeof(GraphKey) == 8 * 8 + 8 * 4 + sizeof(CacheParams) + 6 * 8, "GraphKey has implicit padding");
+static_assert(sizeof(GraphKey) == 11 * 8 + 8 * 4 + sizeof(CacheParams) + 6 * 8, "GraphKey has implicit padding");
 
 // Capacity-preserving upload (stable device addresses for a captured graph).
 void Put(DeviceBuffer& b, const void* host, std::size_t bytes, hipStream_t st)
@@ -339,7 +340,6 @@ struct hdcx_batch : PathPool {
         stats = hdcx_batch_stats{};
         if (plan.tiles.empty()) return;
         hipStream_t st = stream;
-        EnsurePaths(plan.maxPaths);
 
         viewHost.assign(views.size(), BatchView{});
         sceneHost.assign(scenes.size(), DeviceScene{});
@@ -395,6 +395,8 @@ struct hdcx_batch : PathPool {
             simple = simple && d.simpleMaterials;
         }
         const int walk = simple ? kShadeSimple : subsurface ? kShadeWalkDefer : kShadeWalkInline;
+        EnsurePaths(plan.maxPaths, kNeedSceneIndex | (walk == kShadeWalkDefer ? kNeedWalks : 0U) |
+                                       (rcEnabled ? kNeedSpread : 0U));
         const PathBuffers pb = Paths();
         const std::uint32_t maxPaths = plan.maxPaths, maxPixels = plan.maxPixels;
         const bool cache = rcEnabled;
@@ -437,6 +439,9 @@ struct hdcx_batch : PathPool {
         if (tuning.use_graph) {
             GraphKey key{};
             key.pool = rayO.as<void>();
+            key.optional[0] = this->walk.as<void>();
+            key.optional[1] = spread.as<void>();
+            key.optional[2] = pathScene.as<void>();
             const void* t[7] = {table, bw.views, bw.tiles, bw.waveTiles, bw.wavePaths, bw.wavePixels, bw.cursor};
             std::memcpy(key.tables, t, sizeof(t));
             key.capacity = capacity;

@@ -81,6 +81,8 @@ hipError_t LaunchAdvanceWave(const BatchWaves& bw, std::uint32_t* clear, hipStre
 hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st);
 // Counts the answered cache queries of a bounce and empties the queue.
 hipError_t LaunchConsumeQueries(const CacheParams& cp, hipStream_t st);
+// Marks n training records empty (before a single render's chunk).
+hipError_t LaunchClearRecords(RcQuery* records, std::uint32_t n, unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchResolve(double4* mean, const double4* batch, float4* out, std::uint32_t pixels, std::uint32_t sampleIndex,
                          std::uint32_t sampleCount, unsigned maxBlocks, hipStream_t st);
 // Nearest-neighbour upscale of a w0 x h0 RGBA32F film to w x h

@@ -46,15 +46,10 @@ __device__ __forceinline__ void startPath(const PathBuffers& pb, const CachePara
     }
     pb.lambda0[pid] = wl0;
     if (cp.enabled) pb.spread[pid] = make_float2(0.0f, 0.0f);
-    // A single render clears the training slot this path will write. (In a
-    // batch, slots are handed out per wave by raygenBatchKernel and the
-    // gather of the previous wave left every handed-out record empty.)
-    if (cp.enabled && !cp.trainSlot && (ps + cp.trainOffset) % cp.trainStride == 0U) {
-        const std::uint32_t slot = (ps + cp.trainOffset) / cp.trainStride;
-        if (slot < cp.recordSlots)
-            for (std::uint32_t j = 0; j < cp.recordVertices; ++j)
-                cp.records[(slot * 3U + lane) * cp.recordVertices + j].lane = 0xffffffffU;
-    }
+    // (A single render empties every training record before raygen,
+    // LaunchClearRecords; in a batch, slots are handed out per wave by
+    // raygenBatchKernel and the gather of the previous wave left every
+    // handed-out record empty.)
     const float jx = randomFloat(rng), jy = randomFloat(rng);
     const float u = (static_cast<float>(px) + jx) / static_cast<float>(width);
     const float v = (static_cast<float>(py) + jy) / static_cast<float>(height);
@@ -590,6 +585,14 @@ __global__ void finishTrainingKernel(PathBuffers pb, CacheParams cp, std::uint32
 
 // After the queries of a bounce are answered: counts them (queryCount + 2,
 // 64-bit) and empties the queue for the next bounce.
+// Marks `records` training records empty (lane ~0): records that no
+// training path of the chunk writes are then never gathered.
+__global__ void clearRecordsKernel(RcQuery* records, std::uint32_t n)
+{
+    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        records[i].lane = 0xffffffffU;
+}
+
 __global__ void consumeQueriesKernel(CacheParams cp)
 {
     *reinterpret_cast<unsigned long long*>(cp.queryCount + 2) += min(*cp.queryCount, cp.queryCapacity);
@@ -876,6 +879,11 @@ hipError_t LaunchAdvanceWave(const BatchWaves& bw, std::uint32_t* clear, hipStre
 hipError_t LaunchFilm(const PathBuffers& pb, const ChunkDesc& c, double4* batch, unsigned maxBlocks, hipStream_t st)
 {
     hipLaunchKernelGGL(filmKernel, dim3(GridFor(c.p1 - c.p0, maxBlocks)), dim3(kBlock), 0, st, pb, c, batch);
+    return hipGetLastError();
+}
+hipError_t LaunchClearRecords(RcQuery* records, std::uint32_t n, unsigned maxBlocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(clearRecordsKernel, dim3(GridFor(n, maxBlocks)), dim3(kBlock), 0, st, records, n);
     return hipGetLastError();
 }
 hipError_t LaunchConsumeQueries(const CacheParams& cp, hipStream_t st)

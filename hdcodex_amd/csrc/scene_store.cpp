@@ -620,42 +620,59 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     t->geometry = true;
 }
 
-void PathPool::EnsurePaths(std::uint32_t paths)
+void PathPool::EnsurePaths(std::uint32_t paths, std::uint32_t needs)
 {
     if (!counters.bytes()) {
         counters.Allocate(kCounterSlots * kCounterStride * sizeof(std::uint32_t));
         totals.Allocate(64);
         Check(hipMemset(totals.as<void>(), 0, 64), "memset totals");
     }
-    if (paths <= capacity) return;
-    const std::size_t n = paths;
-    rayO.Allocate(n * 16);
-    rayD.Allocate(n * 16);
-    hit.Allocate(n * 16);
-    rng.Allocate(n * 4);
-    wavelength.Allocate(n * 4);
-    throughput.Allocate(n * 4);
-    radiance.Allocate(n * 4);
-    flags.Allocate(n * 4);
-    mis.Allocate(n * 8);
-    medium.Allocate(n * 16);
-    shadowO.Allocate(n * 16 * kShadowSlots);
-    shadowD.Allocate(n * 16 * kShadowSlots);
-    shadowLight.Allocate(n * 4 * kShadowSlots);
-    queue0.Allocate(n * 4);
-    queue1.Allocate(n * 4);
-    shadowQueue.Allocate(n * 4);
-    lambda0.Allocate(n * 4);
-    spread.Allocate(n * 8);
-    pathScene.Allocate(n * 4);
-    walk.Allocate(n * 48);
-    walkQueue.Allocate(n * 4);
-    extendQueue0.Allocate(n * 4);
-    extendQueue1.Allocate(n * 4);
-    follow.Allocate(n * 4);
-    shadowLeaders.Allocate(n * 4);
-    shadowFollow.Allocate(n * 4);
-    capacity = paths;
+    if (paths > capacity) {
+        const std::size_t n = paths;
+        rayO.Allocate(n * 16);
+        rayD.Allocate(n * 16);
+        hit.Allocate(n * 16);
+        rng.Allocate(n * 4);
+        wavelength.Allocate(n * 4);
+        throughput.Allocate(n * 4);
+        radiance.Allocate(n * 4);
+        flags.Allocate(n * 4);
+        mis.Allocate(n * 8);
+        medium.Allocate(n * 16);
+        shadowO.Allocate(n * 16 * kShadowSlots);
+        shadowD.Allocate(n * 16 * kShadowSlots);
+        shadowLight.Allocate(n * 4 * kShadowSlots);
+        queue0.Allocate(n * 4);
+        queue1.Allocate(n * 4);
+        shadowQueue.Allocate(n * 4);
+        lambda0.Allocate(n * 4);
+        extendQueue0.Allocate(n * 4);
+        extendQueue1.Allocate(n * 4);
+        follow.Allocate(n * 4);
+        shadowLeaders.Allocate(n * 4);
+        shadowFollow.Allocate(n * 4);
+        // The optional arrays are re-made at the new capacity when needed.
+        walk.Reset();
+        walkQueue.Reset();
+        spread.Reset();
+        pathScene.Reset();
+        have = 0;
+        capacity = paths;
+    }
+    const std::size_t n = capacity;
+    if ((needs & kNeedWalks) && !(have & kNeedWalks)) {
+        walk.Allocate(n * 48);
+        walkQueue.Allocate(n * 4);
+        have |= kNeedWalks;
+    }
+    if ((needs & kNeedSpread) && !(have & kNeedSpread)) {
+        spread.Allocate(n * 8);
+        have |= kNeedSpread;
+    }
+    if ((needs & kNeedSceneIndex) && !(have & kNeedSceneIndex)) {
+        pathScene.Allocate(n * 4);
+        have |= kNeedSceneIndex;
+    }
 }
 
 PathBuffers PathPool::Paths()

@@ -57,9 +57,16 @@ struct PathPool {
         shadowLight, queue0, queue1, shadowQueue, lambda0, spread, pathScene, walk, walkQueue, extendQueue0, extendQueue1,
         follow, shadowLeaders, shadowFollow, counters, totals;
     std::uint32_t capacity = 0;
+    std::uint32_t have = 0;  // optional arrays allocated at `capacity` (kNeed* bits)
 
-    // Room for `paths` lane paths (grows only).
-    void EnsurePaths(std::uint32_t paths);
+    // Optional per-path arrays, allocated only for renders that use them:
+    // deferred subsurface walks (52 B per path), the radiance cache's path
+    // spread (8 B), the batch renderer's scene index (4 B).
+    static constexpr std::uint32_t kNeedWalks = 1U, kNeedSpread = 2U, kNeedSceneIndex = 4U;
+
+    // Room for `paths` lane paths (grows only) and the optional arrays in
+    // `needs`.
+    void EnsurePaths(std::uint32_t paths, std::uint32_t needs);
     PathBuffers Paths();
 };
 

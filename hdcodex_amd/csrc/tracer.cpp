@@ -273,7 +273,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
         const std::uint32_t chunks = (samples + samplesPerChunk - 1U) / samplesPerChunk;
         samplesPerChunk = (samples + chunks - 1U) / chunks;
     }
-    t->EnsurePaths(pixelsPerChunk * samplesPerChunk * 3U);
+    const bool deferWalks = t->scene.hasSubsurface && !(t->instrumentation & 4U);
+    t->EnsurePaths(pixelsPerChunk * samplesPerChunk * 3U,
+                   (deferWalks ? PathPool::kNeedWalks : 0U) | (t->rcEnabled ? PathPool::kNeedSpread : 0U));
     PathBuffers pb = t->Paths();
     // Radiance cache: training pixel samples are spread evenly over the
     // frame so that about train_samples vertices are recorded (2.5 per lane
@@ -313,8 +315,8 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
     }
     TraversalCounters* ctr = (t->instrumentation & 2U) ? t->traversalCounters.as<TraversalCounters>() : nullptr;
     const int walk = t->scene.simpleMaterials && !(t->instrumentation & 16U) ? kShadeSimple
-                     : !t->scene.hasSubsurface || (t->instrumentation & 4U) ? kShadeWalkInline
-                                                                             : kShadeWalkDefer;
+                     : deferWalks                                             ? kShadeWalkDefer
+                                                                              : kShadeWalkInline;
     const bool share = !(t->instrumentation & 8U);  // lanes of a pixel sample trace an equal ray once
     TraversalCounters* shadowCtr = ctr ? ctr + 1 : nullptr;
     ChunkDesc c{};
@@ -336,6 +338,9 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
             c.p0 = p0;
             c.p1 = p1;
             const std::uint32_t paths = (s1 - s0) * (p1 - p0) * 3U;
+            if (cache)
+                Check(LaunchClearRecords(cp.records, cp.recordSlots * 3U * kRecordVertices, t->maxBlocks, st),
+                      "clear training records");
             t->Timed(HDCX_KERNEL_RAYGEN, [&] { return LaunchRaygen(pb, c, paths, cp, t->maxBlocks, st); });
             t->stats.rays[HDCX_KERNEL_RAYGEN] += paths;
             for (std::uint32_t b = 0; b < bounces; ++b) {
