@@ -7,6 +7,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 namespace hdcx {
 
 constexpr int kBlock = 128;
@@ -89,6 +93,43 @@ inline unsigned GridFor(std::uint32_t work, unsigned maxBlocks)
 {
     const unsigned blocks = (work + kBlock - 1U) / kBlock;
     return blocks == 0 ? 1U : (blocks < maxBlocks ? blocks : maxBlocks);
+}
+
+// Blocks of `kernel` (blockSize threads, no dynamic LDS) the current device
+// keeps resident at once: its occupancy limit per CU times the CU count,
+// queried once per kernel and device.
+inline unsigned ResidentBlocks(const void* kernel, int blockSize)
+{
+    static std::mutex mutex;
+    static std::map<std::pair<const void*, int>, unsigned> cache;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return 0U;
+    const std::lock_guard<std::mutex> lock(mutex);
+    const auto key = std::make_pair(kernel, device);
+    if (const auto found = cache.find(key); found != cache.end()) return found->second;
+    int perCu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, kernel, blockSize, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0U;
+    const unsigned resident = static_cast<unsigned>(perCu > 0 ? perCu : 1) * static_cast<unsigned>(cus > 0 ? cus : 1);
+    cache.emplace(key, resident);
+    return resident;
+}
+
+// Grid of a grid-stride stage kernel: blocks for `work` items, at most
+// maxBlocks and at most what the device keeps resident. Every wave of such
+// a loop does about the same work, so blocks beyond the resident ones run
+// as a partial second round: at 6 resident waves per SIMD and 8 waves of
+// blocks, a third of the GPU would idle for a whole per-wave loop at the
+// end of each launch.
+template <class Kernel>
+unsigned StageGrid(Kernel kernel, std::uint32_t work, unsigned maxBlocks, int blockSize = kBlock)
+{
+    const std::uint32_t blocks = (work + static_cast<std::uint32_t>(blockSize) - 1U) / static_cast<std::uint32_t>(blockSize);
+    unsigned grid = blocks == 0 ? 1U : (blocks < maxBlocks ? blocks : maxBlocks);
+    const unsigned resident = ResidentBlocks(reinterpret_cast<const void*>(kernel), blockSize);
+    if (resident > 0U && grid > resident) grid = resident;
+    return grid;
 }
 
 } // namespace hdcx

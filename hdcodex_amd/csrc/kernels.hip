@@ -173,8 +173,11 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // path and to the lanes that share its ray: the same values each lane
 // would compute for itself.
 // ---------------------------------------------------------------------------
+// One-level extend is held to 64 VGPRs: 8 waves per SIMD (a few values
+// spill to scratch) hide more of the node-fetch latency than 6 waves
+// without spills (C1: 25.4 -> 23.9 ms per frame). Shadow likewise at 6.
 template <bool kCount, class Scenes, bool k2L>
-__global__ __launch_bounds__(kBlock) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 : 8))) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
                                                       TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(kPairBlock) void shadowPairKernel(Scenes scenes, Pa
 // :891-912).
 // ---------------------------------------------------------------------------
 template <bool kCount, class Scenes, bool k2L>
-__global__ __launch_bounds__(kBlock) void shadowKernel(Scenes scenes, PathBuffers pb, bool shared, TraversalCounters* ctr)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 : 6))) void shadowKernel(Scenes scenes, PathBuffers pb, bool shared, TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
     TraversalStack st;
@@ -779,18 +782,20 @@ hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t i
                          unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock),
-                           0, st, s, pb, in, sharing, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes, k2L>),
+                           dim3(StageGrid(extendKernel<true, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb,
+                           in, sharing, ctr);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)),
-                           dim3(kBlock), 0, st, s, pb, in, sharing, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes, k2L>),
+                           dim3(StageGrid(extendKernel<false, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
+                           pb, in, sharing, ctr);
     return hipGetLastError();
 }
 template <class Scenes>
 hipError_t LaunchPairT(const Scenes& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
                        unsigned maxBlocks, hipStream_t st)
 {
-    const unsigned blocks = std::max(1U, std::min((maxWork + kPairBlock - 1U) / kPairBlock, maxBlocks));
+    const unsigned blocks = StageGrid(pairKernel<Scenes>, maxWork, maxBlocks, kPairBlock);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(pairKernel<Scenes>), dim3(blocks), dim3(kPairBlock), 0, st, s, pb, out);
     return hipGetLastError();
 }
@@ -799,11 +804,13 @@ hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, bool shared, st
                          TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock),
-                           0, st, s, pb, shared, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes, k2L>),
+                           dim3(StageGrid(shadowKernel<true, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb,
+                           shared, ctr);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)),
-                           dim3(kBlock), 0, st, s, pb, shared, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes, k2L>),
+                           dim3(StageGrid(shadowKernel<false, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
+                           pb, shared, ctr);
     return hipGetLastError();
 }
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
@@ -820,7 +827,7 @@ hipError_t LaunchPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t
 hipError_t LaunchShadowPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                             hipStream_t st)
 {
-    const unsigned blocks = std::max(1U, std::min((maxWork + kPairBlock - 1U) / kPairBlock, maxBlocks));
+    const unsigned blocks = StageGrid(shadowPairKernel<SingleScene>, maxWork, maxBlocks, kPairBlock);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowPairKernel<SingleScene>), dim3(blocks), dim3(kPairBlock), 0, st,
                        SingleScene{s}, pb);
     return hipGetLastError();
@@ -847,7 +854,7 @@ hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std:
 hipError_t LaunchShadowPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork,
                                  unsigned maxBlocks, hipStream_t st)
 {
-    const unsigned blocks = std::max(1U, std::min((maxWork + kPairBlock - 1U) / kPairBlock, maxBlocks));
+    const unsigned blocks = StageGrid(shadowPairKernel<SceneTable>, maxWork, maxBlocks, kPairBlock);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowPairKernel<SceneTable>), dim3(blocks), dim3(kPairBlock), 0, st,
                        SceneTable{table, pb.pathScene}, pb);
     return hipGetLastError();

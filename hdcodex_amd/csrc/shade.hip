@@ -800,7 +800,7 @@ template <class Scenes>
 hipError_t LaunchMissT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork, unsigned maxBlocks,
                        hipStream_t st)
 {
-    const unsigned blocks = std::max(1U, std::min((maxWork + kPairBlock - 1U) / kPairBlock, maxBlocks));
+    const unsigned blocks = StageGrid(missKernel<Scenes>, maxWork, maxBlocks, kPairBlock);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(missKernel<Scenes>), dim3(blocks), dim3(kPairBlock), 0, st, s, pb, in);
     return hipGetLastError();
 }
@@ -808,25 +808,29 @@ template <class Scenes, bool k2L>
 hipError_t LaunchShadeT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
                         const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
-    const dim3 grid(GridFor(maxWork, maxBlocks));
     if (walk == kShadeSimple)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeSimple>), grid, dim3(kBlock), 0, st, s, pb, in,
-                           bounce, last, cp);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeSimple>),
+                           dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeSimple>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
+                           pb, in, bounce, last, cp);
     else if (walk == kShadeWalkDefer)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkDefer>), grid, dim3(kBlock), 0, st, s, pb, in,
-                           bounce, last, cp);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkDefer>),
+                           dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeWalkDefer>, maxWork, maxBlocks)), dim3(kBlock), 0, st,
+                           s, pb, in, bounce, last, cp);
     else if (walk == kShadeWalkResume)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkResume>), grid, dim3(kBlock), 0, st, s, pb, in,
-                           bounce, last, cp);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkResume>),
+                           dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeWalkResume>, maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, in, bounce, last, cp);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkInline>), grid, dim3(kBlock), 0, st, s, pb, in,
-                           bounce, last, cp);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkInline>),
+                           dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeWalkInline>, maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, in, bounce, last, cp);
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
 hipError_t LaunchWalkT(const Scenes& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(walkKernel<Scenes, k2L>), dim3(GridFor(maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(walkKernel<Scenes, k2L>), dim3(StageGrid(walkKernel<Scenes, k2L>, maxWork, maxBlocks)),
+                       dim3(kBlock), 0, st, s,
                        pb);
     return hipGetLastError();
 }
