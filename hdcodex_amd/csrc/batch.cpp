@@ -49,7 +49,8 @@ struct GraphKey {
     const void* pool = nullptr;
     const void* optional[3] = {};  // walk slots, path spread, scene index (allocated on demand)
     const void* tables[7] = {};
-    std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0, twoLevel = 0, walk = 0;
+    std::uint32_t capacity = 0, bounces = 0, maxBlocks = 0, maxPaths = 0, maxPixels = 0, walk = 0;
+    std::uint32_t kernels = 0;  // bit 0: two-level; bit 1: every scene opaque
     std::uint32_t scenes = 0;  // also keeps the struct free of implicit padding (keys compare with memcmp)
     CacheParams cache{};
     const void* rc[6] = {};    // per-scene weights, sorted queries, bucket arrays
@@ -388,9 +389,10 @@ struct hdcx_batch : PathPool {
         bw.wavePixels = wavePixels.as<std::uint32_t>();
         bw.cursor = cursor.as<std::uint32_t>();
         const DeviceScene* table = sceneTable.as<DeviceScene>();
-        bool twoLevel = false, subsurface = false, simple = true;
+        bool twoLevel = false, subsurface = false, simple = true, opaque = true;
         for (const DeviceScene& d : sceneHost) {
             twoLevel = twoLevel || d.instanceCount > 0;
+            opaque = opaque && d.opaque;
             subsurface = subsurface || d.hasSubsurface;
             simple = simple && d.simpleMaterials;
         }
@@ -406,8 +408,8 @@ struct hdcx_batch : PathPool {
             Check(LaunchRaygenBatch(pb, bw, cp, maxPaths, maxBlocks, st), "batch raygen");
             for (std::uint32_t b = 0; b < bounces; ++b) {
                 const std::uint32_t in = b & 1U;
-                Check(LaunchExtendBatch(table, twoLevel, pb, in, b == 0 ? kShareCamera : kShareLanes, maxPaths, maxBlocks,
-                                        nullptr, st),
+                Check(LaunchExtendBatch(table, twoLevel, opaque, pb, in, b == 0 ? kShareCamera : kShareLanes, maxPaths,
+                                        maxBlocks, nullptr, st),
                       "batch extend");
                 const int last = b + 1 == bounces ? 1 : 0;
                 Check(LaunchMissBatch(table, pb, in, maxPaths, maxBlocks, st), "batch miss");
@@ -428,7 +430,7 @@ struct hdcx_batch : PathPool {
                     Check(LaunchConsumeQueries(cp, st), "batch rc queries");
                 }
                 Check(LaunchShadowPairBatch(table, pb, maxPaths, maxBlocks, st), "batch shadow pair");
-                Check(LaunchShadowBatch(table, twoLevel, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
+                Check(LaunchShadowBatch(table, twoLevel, opaque, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
             }
             if (cache)
                 Check(LaunchGatherTraining(pb, cp, cp.recordSlots * kRecordVertices, maxBlocks, st), "batch rc samples");
@@ -449,7 +451,7 @@ struct hdcx_batch : PathPool {
             key.maxBlocks = maxBlocks;
             key.maxPaths = maxPaths;
             key.maxPixels = maxPixels;
-            key.twoLevel = twoLevel ? 1U : 0U;
+            key.kernels = (twoLevel ? 1U : 0U) | (opaque ? 2U : 0U);
             key.walk = static_cast<std::uint32_t>(walk);
             key.scenes = static_cast<std::uint32_t>(scenes.size());
             key.cache = cp;

@@ -126,10 +126,15 @@ __device__ __forceinline__ void testNode(const Bvh4Node* __restrict__ node, cons
     }
 }
 
+// The spill entries are a private array of their own, declared by the
+// kernel next to the stack (HDCX_TRAVERSAL_STACK): an array indexed at run
+// time inside this struct would keep the whole struct, the stack pointer
+// included, in scratch memory, with a scratch load and store on every push
+// and pop.
 struct TraversalStack {
     std::uint32_t* lds;      // this lane's column; stride = block size
     std::uint32_t stride;
-    std::uint32_t spill[kSpillStack];
+    std::uint32_t* spill;    // kSpillStack private entries
     int sp;
     // The host rejects BVHs whose worst-case occupancy exceeds kMaxStack;
     // the bound check only keeps a corrupt tree from writing out of range.
@@ -146,6 +151,16 @@ struct TraversalStack {
         return sp < kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
     }
 };
+
+// A kernel's traversal stack over its LDS block `mem` (kLdsStack x kBlock
+// words): `name` and its private spill array.
+#define HDCX_TRAVERSAL_STACK(name, mem)               \
+    std::uint32_t name##Spill[kSpillStack];           \
+    TraversalStack name;                              \
+    name.lds = (mem) + threadIdx.x;                   \
+    name.stride = kBlock;                             \
+    name.spill = name##Spill;                         \
+    name.sp = 0
 
 #define HDCX_SWAP_IF(a, b)                                        \
     if (tn[b] < tn[a]) {                                          \

@@ -52,12 +52,13 @@ hipError_t LaunchShadowPair(const DeviceScene& s, const PathBuffers& pb, std::ui
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st);
 // Batch renderer: the same stages over a table of scenes (scene of each
-// path in pb.pathScene; twoLevel when any scene has instances), the waves' raygen / film reading the device wave
+// path in pb.pathScene; twoLevel when any scene has instances, opaque when
+// every scene is opaque), the waves' raygen / film reading the device wave
 // cursor, and the cursor step. maxWork sizes the grid (as a wave's paths or
 // pixels can only be known on the device).
-hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
-                             int sharing, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
-                             hipStream_t st);
+hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, bool opaque, const PathBuffers& pb,
+                             std::uint32_t in, int sharing, std::uint32_t maxWork, unsigned maxBlocks,
+                             TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchMissBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
@@ -70,8 +71,8 @@ hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBu
 hipError_t LaunchShadowPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork,
                                  unsigned maxBlocks, hipStream_t st);
 // The batch always shares shadow rays (LaunchShadowPairBatch first).
-hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
-                             unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
+hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, bool opaque, const PathBuffers& pb,
+                             std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, const CacheParams& cp, std::uint32_t maxWork,
                              unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchFilmBatch(const PathBuffers& pb, const BatchWaves& bw, std::uint32_t maxWork, unsigned maxBlocks,

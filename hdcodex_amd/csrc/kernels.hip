@@ -173,11 +173,16 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // path and to the lanes that share its ray: the same values each lane
 // would compute for itself.
 // ---------------------------------------------------------------------------
-// One-level extend is held to 64 VGPRs: 8 waves per SIMD (a few values
-// spill to scratch) hide more of the node-fetch latency than 6 waves
-// without spills (C1: 25.4 -> 23.9 ms per frame). Shadow likewise at 6.
-template <bool kCount, class Scenes, bool k2L>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 : 8))) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
+// kOpaque: every scene of the launch is opaque (the host knows), so the
+// cut-out candidate loop is compiled out and its registers with it.
+// Occupancy, measured on the 1080p heightfield and 10,000-instance frames:
+// two-level extend runs best held to 7 waves per SIMD when opaque (6
+// otherwise; 4 or 5 waves without spills are 3-8% slower). One-level extend
+// runs best as the generic kernel at the compiler's choice, 78 VGPRs and 6
+// waves (18.5 ms per frame against 19.0 for the opaque kernel at 7 waves
+// and 19.2 at 8), so the host launches no one-level opaque kernel.
+template <bool kCount, class Scenes, bool k2L, bool kOpaque>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? (kOpaque ? 7 : 6) : 1))) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
                                                       TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
@@ -198,9 +203,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 
         pb.totals[3] += rays;
     }
     const std::uint32_t* queue = leaders ? pb.extendQueue[in] : pb.queue[in];
-    TraversalStack st;
-    st.lds = stackMem + threadIdx.x;
-    st.stride = kBlock;
+    HDCX_TRAVERSAL_STACK(st, stackMem);
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rays; i += gridDim.x * blockDim.x) {
         const std::uint32_t pid = camera ? 3U * i : queue[i];
         const std::uint32_t followers = camera ? 3U : leaders ? pb.follow[pid] : 0U;
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 
         HitRec h;
         bool found;
         std::uint32_t rng = 0U;
-        if (s.opaque) {
+        if (kOpaque || s.opaque) {
             found = traceClosestS<kCount, k2L>(s, r, 0.001f, 10000.0f, 0.0f, 0xffffffffU, h, st, ctr);
         } else {
             rng = pb.rng[pid];
@@ -221,11 +224,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 
             if (ctr && found) atomicAdd(&ctr->hits, 1ULL + static_cast<unsigned long long>(__popc(followers & 3U)));
         }
         pb.hit[pid] = hit;
-        if (!s.opaque) pb.rng[pid] = rng;
+        if (!kOpaque && !s.opaque) pb.rng[pid] = rng;
         for (std::uint32_t k = 0; k < 2U; ++k) {
             if (!(followers & (1U << k))) continue;
             pb.hit[pid + 1U + k] = hit;
-            if (!s.opaque) pb.rng[pid + 1U + k] = rng;
+            if (!kOpaque && !s.opaque) pb.rng[pid + 1U + k] = rng;
         }
     }
 }
@@ -354,15 +357,15 @@ __global__ __launch_bounds__(kPairBlock) void shadowPairKernel(Scenes scenes, Pa
 // ---------------------------------------------------------------------------
 // shadow: visibility of each queued path's shadow rays, added to its
 // radiance in slot order (shadowVisibility :855-889, meshLightVisible
-// :891-912).
+// :891-912). kOpaque as in extendKernel. Held to 7 waves per SIMD for
+// one-level opaque scenes (67 VGPRs, no spills; 9.0 ms per heightfield
+// frame against 9.6 at 8 waves), 6 otherwise.
 // ---------------------------------------------------------------------------
-template <bool kCount, class Scenes, bool k2L>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 : 6))) void shadowKernel(Scenes scenes, PathBuffers pb, bool shared, TraversalCounters* ctr)
+template <bool kCount, class Scenes, bool k2L, bool kOpaque>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kOpaque && !k2L) ? 7 : 6))) void shadowKernel(Scenes scenes, PathBuffers pb, bool shared, TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
-    TraversalStack st;
-    st.lds = stackMem + threadIdx.x;
-    st.stride = kBlock;
+    HDCX_TRAVERSAL_STACK(st, stackMem);
     __shared__ unsigned int tracedInBlock;
     if (threadIdx.x == 0) tracedInBlock = 0U;
     __syncthreads();
@@ -377,6 +380,62 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const std::uint32_t pid = queue[i];
         const DeviceScene& s = scenes.Get(pid);
+        // Trace every slot first, keeping only the outcomes: the lanes'
+        // wavelengths, radiance and the light controls are read afterwards,
+        // so that they are not held in registers across the traversals.
+        // (Per-slot values in named registers: an array indexed by the
+        // rolled slot loop would live in scratch memory.)
+        static_assert(kShadowSlots == 3, "three occluder distances");
+        std::uint32_t occludedSlots = 0U;
+        float occluder0 = 0.0f, occluder1 = 0.0f, occluder2 = 0.0f;
+        int slots = 0;
+        for (int k = 0; k < kShadowSlots; ++k) {
+            const std::size_t idx = static_cast<std::size_t>(k) * pb.capacity + pid;
+            const std::uint32_t light = pb.shadowLight[idx];
+            if (light == 0xffffffffU) break;
+            slots = k + 1;
+            const float4 o = pb.shadowO[idx], d = pb.shadowD[idx];
+            float tmax = o.w;
+            const bool mesh = light == kShadowMeshLight;
+            bool falloff = false;
+            if (!mesh) {
+                const F4 shadow = s.lights[light].shadow;
+                if (shadow.x < 0.5f) continue;
+                if (shadow.y >= 0.0f) tmax = fminf(tmax, shadow.y);
+                falloff = shadow.y >= 0.0f && shadow.z > 0.0f;
+            }
+            if (tmax <= 0.001f) continue;
+            ++traced;
+            const RayPre r = prepareRay(o.x, o.y, o.z, d.x, d.y, d.z);
+            bool occluded;
+            float hitDistance = 0.0f;
+            const bool opaque = kOpaque || s.opaque;
+            if (opaque && !falloff) {
+                occluded = traceAnyS<kCount, k2L>(s, r, 0.001f, tmax, st, ctr);
+            } else {
+                // Closest confirmed occluder: opacity >= 0.5 in a
+                // non-opaque scene; its distance drives the falloff.
+                float loT = 0.001f;
+                std::uint32_t loPrim = 0xffffffffU;
+                occluded = false;
+                for (;;) {
+                    HitRec h;
+                    if (!traceClosestS<kCount, k2L>(s, r, 0.001f, tmax, loT, loPrim, h, st, ctr)) break;
+                    if (opaque || materialOpacity<k2L>(s, h.prim, h.u, h.v) >= 0.5f) {
+                        occluded = true;
+                        hitDistance = h.t;
+                        break;
+                    }
+                    loT = h.t;
+                    loPrim = h.prim;
+                }
+            }
+            if (occluded) occludedSlots |= 1U << k;
+            if (k == 0) occluder0 = hitDistance;
+            else if (k == 1) occluder1 = hitDistance;
+            else occluder2 = hitDistance;
+        }
+        if (slots == 0) continue;
         const std::uint32_t followers = shared ? pb.shadowFollow[pid] : 0U;
         std::uint32_t member[3] = {pid, pid, pid};
         int members = 1;
@@ -387,71 +446,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? 1 
             wls[m] = pb.wavelength[member[m]];
             rad[m] = pb.radiance[member[m]];
         }
-        for (int k = 0; k < kShadowSlots; ++k) {
+        for (int k = 0; k < slots; ++k) {
             const std::size_t idx = static_cast<std::size_t>(k) * pb.capacity + pid;
             const std::uint32_t light = pb.shadowLight[idx];
-            if (light == 0xffffffffU) break;
-            const float4 o = pb.shadowO[idx], d = pb.shadowD[idx];
-            float tmax = o.w;
             const bool mesh = light == kShadowMeshLight;
-            bool trace = true;
-            int outcome = 0;  // 1: occluded
-            float occluderDistance = 0.0f;
-            bool falloffSet = false;
             float4 shadow = make_float4(1.0f, -1.0f, -1.0f, 1.0f);
             float4 shadowColor = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            bool trace = true;
             if (!mesh) {
                 const GpuLightRecord& L = s.lights[light];
                 shadow = make_float4(L.shadow.x, L.shadow.y, L.shadow.z, L.shadow.w);
                 shadowColor = make_float4(L.shadowColor.x, L.shadowColor.y, L.shadowColor.z, 0.0f);
                 if (shadow.x < 0.5f) trace = false;
-                if (shadow.y >= 0.0f) tmax = fminf(tmax, shadow.y);
             }
-            if (tmax <= 0.001f) trace = false;
-            if (trace) {
-                ++traced;
-                const RayPre r = prepareRay(o.x, o.y, o.z, d.x, d.y, d.z);
-                const bool falloff = !mesh && shadow.y >= 0.0f && shadow.z > 0.0f;
-                falloffSet = falloff;
-                bool occluded;
-                float hitDistance = 0.0f;
-                if (s.opaque && !falloff) {
-                    occluded = traceAnyS<kCount, k2L>(s, r, 0.001f, tmax, st, ctr);
-                } else {
-                    // Closest confirmed occluder: opacity >= 0.5 in a
-                    // non-opaque scene; its distance drives the falloff.
-                    float loT = 0.001f;
-                    std::uint32_t loPrim = 0xffffffffU;
-                    occluded = false;
-                    for (;;) {
-                        HitRec h;
-                        if (!traceClosestS<kCount, k2L>(s, r, 0.001f, tmax, loT, loPrim, h, st, ctr)) break;
-                        if (s.opaque || materialOpacity<k2L>(s, h.prim, h.u, h.v) >= 0.5f) {
-                            occluded = true;
-                            hitDistance = h.t;
-                            break;
-                        }
-                        loT = h.t;
-                        loPrim = h.prim;
-                    }
-                }
-                outcome = occluded ? 1 : 0;
-                occluderDistance = hitDistance;
-            }
+            const bool occluded = trace && (occludedSlots >> k & 1U) != 0U;
+            const bool falloffSet = !mesh && shadow.y >= 0.0f && shadow.z > 0.0f;
             // Visibility of each lane of the group with its own wavelength, and
             // its own contribution, in slot order.
             for (int m = 0; m < members; ++m) {
                 const float wl = wls[m];
-                const float contribution = m == 0 ? d.w : pb.shadowD[static_cast<std::size_t>(k) * pb.capacity + member[m]].w;
+                const float contribution = pb.shadowD[static_cast<std::size_t>(k) * pb.capacity + member[m]].w;
                 float visibility = 1.0f;
-                if (trace && outcome == 1) {
+                if (occluded) {
                     if (mesh) {
                         visibility = 0.0f;
                     } else {
                         float strength = 1.0f;
                         if (falloffSet) {
                             const float f = fminf(shadow.z, fmaxf(shadow.y, 0.0f));
-                            strength = clampf((shadow.y - occluderDistance) / fmaxf(f, 1e-6f), 0.0f, 1.0f);
+                            const float occluder = k == 0 ? occluder0 : k == 1 ? occluder1 : occluder2;
+                            strength = clampf((shadow.y - occluder) / fmaxf(f, 1e-6f), 0.0f, 1.0f);
                             strength = powf(strength, fmaxf(shadow.w, 0.0f));
                         }
                         const V3 vis = v3(mixf(1.0f, shadowColor.x, strength), mixf(1.0f, shadowColor.y, strength),
@@ -722,9 +746,7 @@ template <bool k2L>
 __global__ __launch_bounds__(kBlock) void queryClosestKernel(DeviceScene s, const float4* rays, std::uint32_t n, float4* hits)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
-    TraversalStack st;
-    st.lds = stackMem + threadIdx.x;
-    st.stride = kBlock;
+    HDCX_TRAVERSAL_STACK(st, stackMem);
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 a = rays[2 * i], b = rays[2 * i + 1];
         const RayPre r = prepareRay(a.x, a.y, a.z, b.x, b.y, b.z);
@@ -739,9 +761,7 @@ template <bool k2L>
 __global__ __launch_bounds__(kBlock) void queryAnyKernel(DeviceScene s, const float4* rays, std::uint32_t n, unsigned char* out)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
-    TraversalStack st;
-    st.lds = stackMem + threadIdx.x;
-    st.stride = kBlock;
+    HDCX_TRAVERSAL_STACK(st, stackMem);
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4 a = rays[2 * i], b = rays[2 * i + 1];
         const RayPre r = prepareRay(a.x, a.y, a.z, b.x, b.y, b.z);
@@ -777,18 +797,30 @@ hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, st
                        calibration);
     return hipGetLastError();
 }
-template <class Scenes, bool k2L>
-hipError_t LaunchExtendT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
-                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+template <class Scenes, bool k2L, bool kOpaque>
+void LaunchExtendK(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
+                   unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes, k2L>),
-                           dim3(StageGrid(extendKernel<true, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb,
-                           in, sharing, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<true, Scenes, k2L, kOpaque>),
+                           dim3(StageGrid(extendKernel<true, Scenes, k2L, kOpaque>, maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, in, sharing, ctr);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes, k2L>),
-                           dim3(StageGrid(extendKernel<false, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
-                           pb, in, sharing, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(extendKernel<false, Scenes, k2L, kOpaque>),
+                           dim3(StageGrid(extendKernel<false, Scenes, k2L, kOpaque>, maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, in, sharing, ctr);
+}
+template <class Scenes, bool k2L>
+hipError_t LaunchExtendT(const Scenes& s, bool opaque, const PathBuffers& pb, std::uint32_t in, int sharing,
+                         std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+{
+    if constexpr (k2L) {
+        if (opaque) {
+            LaunchExtendK<Scenes, k2L, true>(s, pb, in, sharing, maxWork, maxBlocks, ctr, st);
+            return hipGetLastError();
+        }
+    }
+    LaunchExtendK<Scenes, k2L, false>(s, pb, in, sharing, maxWork, maxBlocks, ctr, st);
     return hipGetLastError();
 }
 template <class Scenes>
@@ -799,25 +831,34 @@ hipError_t LaunchPairT(const Scenes& s, const PathBuffers& pb, std::uint32_t out
     hipLaunchKernelGGL(HIP_KERNEL_NAME(pairKernel<Scenes>), dim3(blocks), dim3(kPairBlock), 0, st, s, pb, out);
     return hipGetLastError();
 }
-template <class Scenes, bool k2L>
-hipError_t LaunchShadowT(const Scenes& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
-                         TraversalCounters* ctr, hipStream_t st)
+template <class Scenes, bool k2L, bool kOpaque>
+void LaunchShadowK(const Scenes& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
+                   TraversalCounters* ctr, hipStream_t st)
 {
     if (ctr)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes, k2L>),
-                           dim3(StageGrid(shadowKernel<true, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s, pb,
-                           shared, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<true, Scenes, k2L, kOpaque>),
+                           dim3(StageGrid(shadowKernel<true, Scenes, k2L, kOpaque>, maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, shared, ctr);
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes, k2L>),
-                           dim3(StageGrid(shadowKernel<false, Scenes, k2L>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
-                           pb, shared, ctr);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowKernel<false, Scenes, k2L, kOpaque>),
+                           dim3(StageGrid(shadowKernel<false, Scenes, k2L, kOpaque>, maxWork, maxBlocks)), dim3(kBlock), 0,
+                           st, s, pb, shared, ctr);
+}
+template <class Scenes, bool k2L>
+hipError_t LaunchShadowT(const Scenes& s, bool opaque, const PathBuffers& pb, bool shared, std::uint32_t maxWork,
+                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+{
+    if (opaque) LaunchShadowK<Scenes, k2L, true>(s, pb, shared, maxWork, maxBlocks, ctr, st);
+    else LaunchShadowK<Scenes, k2L, false>(s, pb, shared, maxWork, maxBlocks, ctr, st);
     return hipGetLastError();
 }
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
-    return s.instanceCount ? LaunchExtendT<SingleScene, true>(SingleScene{s}, pb, in, sharing, maxWork, maxBlocks, ctr, st)
-                           : LaunchExtendT<SingleScene, false>(SingleScene{s}, pb, in, sharing, maxWork, maxBlocks, ctr, st);
+    const bool opaque = s.opaque != 0U;
+    return s.instanceCount
+               ? LaunchExtendT<SingleScene, true>(SingleScene{s}, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st)
+               : LaunchExtendT<SingleScene, false>(SingleScene{s}, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
                       unsigned maxBlocks, hipStream_t st)
@@ -835,16 +876,18 @@ hipError_t LaunchShadowPair(const DeviceScene& s, const PathBuffers& pb, std::ui
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st)
 {
-    return s.instanceCount ? LaunchShadowT<SingleScene, true>(SingleScene{s}, pb, shared, maxWork, maxBlocks, ctr, st)
-                           : LaunchShadowT<SingleScene, false>(SingleScene{s}, pb, shared, maxWork, maxBlocks, ctr, st);
+    const bool opaque = s.opaque != 0U;
+    return s.instanceCount
+               ? LaunchShadowT<SingleScene, true>(SingleScene{s}, opaque, pb, shared, maxWork, maxBlocks, ctr, st)
+               : LaunchShadowT<SingleScene, false>(SingleScene{s}, opaque, pb, shared, maxWork, maxBlocks, ctr, st);
 }
-hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in,
-                             int sharing, std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr,
-                             hipStream_t st)
+hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, bool opaque, const PathBuffers& pb,
+                             std::uint32_t in, int sharing, std::uint32_t maxWork, unsigned maxBlocks,
+                             TraversalCounters* ctr, hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchExtendT<SceneTable, true>(t, pb, in, sharing, maxWork, maxBlocks, ctr, st)
-                    : LaunchExtendT<SceneTable, false>(t, pb, in, sharing, maxWork, maxBlocks, ctr, st);
+    return twoLevel ? LaunchExtendT<SceneTable, true>(t, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st)
+                    : LaunchExtendT<SceneTable, false>(t, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st)
@@ -859,12 +902,12 @@ hipError_t LaunchShadowPairBatch(const DeviceScene* table, const PathBuffers& pb
                        SceneTable{table, pb.pathScene}, pb);
     return hipGetLastError();
 }
-hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
-                             unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
+hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, bool opaque, const PathBuffers& pb,
+                             std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchShadowT<SceneTable, true>(t, pb, true, maxWork, maxBlocks, ctr, st)
-                    : LaunchShadowT<SceneTable, false>(t, pb, true, maxWork, maxBlocks, ctr, st);
+    return twoLevel ? LaunchShadowT<SceneTable, true>(t, opaque, pb, true, maxWork, maxBlocks, ctr, st)
+                    : LaunchShadowT<SceneTable, false>(t, opaque, pb, true, maxWork, maxBlocks, ctr, st);
 }
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, const CacheParams& cp, std::uint32_t maxWork,
                              unsigned maxBlocks, hipStream_t st)

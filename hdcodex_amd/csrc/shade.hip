@@ -142,9 +142,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                                                      int lastBounce, CacheParams cp)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
-    TraversalStack st;
-    st.lds = stackMem + threadIdx.x;
-    st.stride = kBlock;
+    HDCX_TRAVERSAL_STACK(st, stackMem);
     // Paths whose ray hit a triangle (missKernel queued them) or whose
     // deferred walk is to be resumed.
     const std::uint32_t count =
@@ -776,9 +774,7 @@ template <class Scenes, bool k2L>
 __global__ __launch_bounds__(kBlock) void walkKernel(Scenes scenes, PathBuffers pb)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
-    TraversalStack st;
-    st.lds = stackMem + threadIdx.x;
-    st.stride = kBlock;
+    HDCX_TRAVERSAL_STACK(st, stackMem);
     const std::uint32_t count = pb.counters[3 * kCounterStride];
     for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
         const DeviceScene& s = scenes.Get(pb.walkQueue[i]);
