@@ -24,7 +24,13 @@ struct RayPre {
     float ix, iy, iz;  // reciprocal direction for the slab test
     int kx, ky, kz;
     float sx, sy, sz;
+    // Byte offset of the entry plane within a node's (lo, hi) plane pair of
+    // each axis: 0 (lo) when the reciprocal is positive, 16 (hi) otherwise.
+    std::uint32_t nearX, nearY, nearZ;
 };
+
+// Entry-plane offsets of a reciprocal direction (see RayPre).
+__device__ __forceinline__ std::uint32_t nearPlaneOffset(float inv) { return inv < 0.0f ? 16U : 0U; }
 
 struct HitRec {
     float t;
@@ -48,6 +54,7 @@ __device__ __forceinline__ RayPre prepareRay(float ox, float oy, float oz, float
     r.ox = ox; r.oy = oy; r.oz = oz;
     r.dx = dx; r.dy = dy; r.dz = dz;
     r.ix = safeRcp(dx); r.iy = safeRcp(dy); r.iz = safeRcp(dz);
+    r.nearX = nearPlaneOffset(r.ix); r.nearY = nearPlaneOffset(r.iy); r.nearZ = nearPlaneOffset(r.iz);
     const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
     int kz = 0;
     float m = ax;
@@ -101,25 +108,45 @@ __device__ __forceinline__ bool intersectTriangle(const RayPre& r, const float4&
 // Slab test of the four children of a node; returns near distances
 // (+inf when missed). tfar is inflated by 4 ulp so that the two roundings
 // of each plane distance never cull a box the ray touches.
-__device__ __forceinline__ void testNode(const Bvh4Node* __restrict__ node, const RayPre& r, float tmin, float tmax,
-                                         float tn[4], std::uint32_t ref[4])
+//
+// The node stores each axis as a lo plane and a hi plane of the four
+// children (16 bytes each). The ray's direction signs say which of the two
+// it enters through, so the entry and exit planes are loaded directly
+// (RayPre::near*) and their distances need no min/max: for a positive
+// reciprocal (lo - o) * i <= (hi - o) * i, for a negative one the reverse,
+// both exactly as rounded. The distances are those of the min/max form bit
+// for bit; an inverted box (lo > hi) gives tnear > tfar and is rejected.
+//
+// Node addresses are a 32-bit byte offset (index x 128) from the array base
+// (node arrays stay below 4 GiB: the host caps them), which the compiler
+// keeps in one register per load next to the scalar base.
+__device__ __forceinline__ const float4& nodePlanes(const Bvh4Node* __restrict__ nodes, std::uint32_t byteOffset)
 {
-    const float4* n4 = reinterpret_cast<const float4*>(node);
-    const float4 lox = n4[0], hix = n4[1], loy = n4[2], hiy = n4[3], loz = n4[4], hiz = n4[5];
-    const uint4 ch = reinterpret_cast<const uint4*>(node)[6];
-    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nodes) + byteOffset);
+}
+__device__ __forceinline__ void testNode(const Bvh4Node* __restrict__ nodes, std::uint32_t index, const RayPre& r, float tmin,
+                                         float tmax, float tn[4], std::uint32_t ref[4])
+{
+    const std::uint32_t base = index << 7;
+    const float4 nx = nodePlanes(nodes, base + r.nearX);
+    const float4 fx = nodePlanes(nodes, base + (16U - r.nearX));
+    const float4 ny = nodePlanes(nodes, base + 32U + r.nearY);
+    const float4 fy = nodePlanes(nodes, base + (48U - r.nearY));
+    const float4 nz = nodePlanes(nodes, base + 64U + r.nearZ);
+    const float4 fz = nodePlanes(nodes, base + (80U - r.nearZ));
+    const uint4 ch = *reinterpret_cast<const uint4*>(&nodePlanes(nodes, base + 96U));
+    const float nxs[4] = {nx.x, nx.y, nx.z, nx.w}, fxs[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nys[4] = {ny.x, ny.y, ny.z, ny.w}, fys[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nzs[4] = {nz.x, nz.y, nz.z, nz.w}, fzs[4] = {fz.x, fz.y, fz.z, fz.w};
     const std::uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float tx0 = (lx[i] - r.ox) * r.ix, tx1 = (hx[i] - r.ox) * r.ix;
-        const float ty0 = (ly[i] - r.oy) * r.iy, ty1 = (hy[i] - r.oy) * r.iy;
-        const float tz0 = (lz[i] - r.oz) * r.iz, tz1 = (hz[i] - r.oz) * r.iz;
-        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-        // Empty slots must be rejected explicitly: min/max of the two plane
-        // distances would turn an inverted box into an infinite one.
+        const float tx0 = (nxs[i] - r.ox) * r.ix, tx1 = (fxs[i] - r.ox) * r.ix;
+        const float ty0 = (nys[i] - r.oy) * r.iy, ty1 = (fys[i] - r.oy) * r.iy;
+        const float tz0 = (nzs[i] - r.oz) * r.iz, tz1 = (fzs[i] - r.oz) * r.iz;
+        const float tnear = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, tmin));
+        const float tfar = fminf(fminf(tx1, ty1), fminf(tz1, tmax));
+        // Empty slots are rejected by reference: their planes are not a box.
         const bool hit = c[i] != kEmptyChild && tnear <= tfar * 1.00000048f;
         tn[i] = hit ? tnear : INFINITY;
         ref[i] = c[i];
@@ -149,6 +176,56 @@ struct TraversalStack {
     {
         --sp;
         return sp < kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+    }
+    // Pushes the valid ones of a node's far children r3, r2, r1 (a sorted
+    // node: validity is a prefix, v3 implies v2 implies v1). With room for
+    // all three in LDS every entry is written at the top and the top
+    // advances past the valid ones, without a branch per child; the write
+    // of an invalid child lands on the free slot above the top (at most
+    // index sp + 2 <= kLdsStack - 1).
+    __device__ __forceinline__ void pushFar3(std::uint32_t r3, bool v3, std::uint32_t r2, bool v2, std::uint32_t r1,
+                                             bool v1)
+    {
+        if (sp + 3 <= kLdsStack) {
+            lds[sp * stride] = r3;
+            sp += v3 ? 1 : 0;
+            lds[sp * stride] = r2;
+            sp += v2 ? 1 : 0;
+            lds[sp * stride] = r1;
+            sp += v1 ? 1 : 0;
+        } else {
+            if (v3) push(r3);
+            if (v2) push(r2);
+            if (v1) push(r1);
+        }
+    }
+    // Any-hit order: continues with the first valid child (next, false when
+    // none) and pushes the other valid ones in child order, as the loop
+    // "first valid -> next, later valid -> push" does. The fast path writes
+    // every child at the top and advances past the pushed ones; before the
+    // last write at most two were pushed, so it lands at index <= sp + 2.
+    __device__ __forceinline__ bool pushAfterFirst(const float tn[4], const std::uint32_t ref[4], std::uint32_t& next)
+    {
+        bool have = false;
+        if (sp + 3 <= kLdsStack) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bool valid = tn[i] != INFINITY;
+                if (valid && !have) next = ref[i];
+                lds[sp * stride] = ref[i];
+                sp += (valid && have) ? 1 : 0;
+                have = have || valid;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (tn[i] != INFINITY) {
+                    if (!have) { next = ref[i]; have = true; }
+                    else push(ref[i]);
+                }
+            }
+        }
+        return have;
     }
 };
 
@@ -210,12 +287,10 @@ __device__ bool traceClosest(const DeviceScene& s, const RayPre& r, float tmin, 
             if (kCount) ++nodes;
             float tn[4];
             std::uint32_t ref[4];
-            testNode(s.nodes + cur, r, tmin, best.t, tn, ref);
+            testNode(s.nodes, cur, r, tmin, best.t, tn, ref);
             HDCX_SWAP_IF(0, 1) HDCX_SWAP_IF(2, 3) HDCX_SWAP_IF(0, 2) HDCX_SWAP_IF(1, 3) HDCX_SWAP_IF(1, 2)
             if (tn[0] != INFINITY) {
-                if (tn[3] != INFINITY) st.push(ref[3]);
-                if (tn[2] != INFINITY) st.push(ref[2]);
-                if (tn[1] != INFINITY) st.push(ref[1]);
+                st.pushFar3(ref[3], tn[3] != INFINITY, ref[2], tn[2] != INFINITY, ref[1], tn[1] != INFINITY);
                 cur = ref[0];
                 continue;
             }
@@ -258,17 +333,9 @@ __device__ bool traceAny(const DeviceScene& s, const RayPre& r, float tmin, floa
             if (kCount) ++nodes;
             float tn[4];
             std::uint32_t ref[4];
-            testNode(s.nodes + cur, r, tmin, tmax, tn, ref);
+            testNode(s.nodes, cur, r, tmin, tmax, tn, ref);
             std::uint32_t next = 0xffffffffU;
-            bool have = false;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (tn[i] != INFINITY) {
-                    if (!have) { next = ref[i]; have = true; }
-                    else st.push(ref[i]);
-                }
-            }
-            if (have) { cur = next; continue; }
+            if (st.pushAfterFirst(tn, ref, next)) { cur = next; continue; }
         }
         if (st.sp == 0) break;
         cur = st.pop();

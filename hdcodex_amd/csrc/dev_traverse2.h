@@ -19,9 +19,13 @@ struct BoxRay {
     float ox, oy, oz;
     float ix, iy, iz;
     float pad;  // widening of every box (object space)
+    std::uint32_t nearX, nearY, nearZ;  // entry-plane offsets, as RayPre's
 };
 
-__device__ __forceinline__ BoxRay worldBoxRay(const RayPre& r) { return BoxRay{r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, 0.0f}; }
+__device__ __forceinline__ BoxRay worldBoxRay(const RayPre& r)
+{
+    return BoxRay{r.ox, r.oy, r.oz, r.ix, r.iy, r.iz, 0.0f, r.nearX, r.nearY, r.nearZ};
+}
 
 // Object-space ray of instance i and the box widening for hits up to tmax.
 // The rounding of the transformed ray grows with the ray parameter t, and a
@@ -45,26 +49,36 @@ __device__ __forceinline__ BoxRay objectBoxRay(const DeviceScene& s, std::uint32
                               fmaxf(fabsf(lo.z - r.oz), fabsf(hi.z - r.oz)));
     const float td = fminf(fminf(tmax, 1e30f) * dInf, reach * 1.00001f);
     const float pad = 16.0f * 5.9604645e-8f * (p.x * (oInf + 2.0f * p.z + td + p.y * p.w) + p.w) + 1e-6f * p.w;
-    return BoxRay{o[0], o[1], o[2], safeRcp(d[0]), safeRcp(d[1]), safeRcp(d[2]), pad};
+    const float ix = safeRcp(d[0]), iy = safeRcp(d[1]), iz = safeRcp(d[2]);
+    return BoxRay{o[0], o[1], o[2], ix, iy, iz, pad, nearPlaneOffset(ix), nearPlaneOffset(iy), nearPlaneOffset(iz)};
 }
 
-__device__ __forceinline__ void testNodeBox(const Bvh4Node* __restrict__ node, const BoxRay& r, float tmin, float tmax,
-                                            float tn[4], std::uint32_t ref[4])
+__device__ __forceinline__ void testNodeBox(const Bvh4Node* __restrict__ nodes, std::uint32_t index, const BoxRay& r,
+                                            float tmin, float tmax, float tn[4], std::uint32_t ref[4])
 {
-    const float4* n4 = reinterpret_cast<const float4*>(node);
-    const float4 lox = n4[0], hix = n4[1], loy = n4[2], hiy = n4[3], loz = n4[4], hiz = n4[5];
-    const uint4 ch = reinterpret_cast<const uint4*>(node)[6];
-    const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-    const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-    const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    // Entry and exit planes loaded by direction sign as in testNode; the
+    // widening moves an entry plane against the ray (lo - pad or hi + pad)
+    // and an exit plane with it.
+    const std::uint32_t base = index << 7;
+    const float4 nx = nodePlanes(nodes, base + r.nearX);
+    const float4 fx = nodePlanes(nodes, base + (16U - r.nearX));
+    const float4 ny = nodePlanes(nodes, base + 32U + r.nearY);
+    const float4 fy = nodePlanes(nodes, base + (48U - r.nearY));
+    const float4 nz = nodePlanes(nodes, base + 64U + r.nearZ);
+    const float4 fz = nodePlanes(nodes, base + (80U - r.nearZ));
+    const uint4 ch = *reinterpret_cast<const uint4*>(&nodePlanes(nodes, base + 96U));
+    const float nxs[4] = {nx.x, nx.y, nx.z, nx.w}, fxs[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nys[4] = {ny.x, ny.y, ny.z, ny.w}, fys[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nzs[4] = {nz.x, nz.y, nz.z, nz.w}, fzs[4] = {fz.x, fz.y, fz.z, fz.w};
     const std::uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+    const float px = r.nearX ? r.pad : -r.pad, py = r.nearY ? r.pad : -r.pad, pz = r.nearZ ? r.pad : -r.pad;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float tx0 = (lx[i] - r.pad - r.ox) * r.ix, tx1 = (hx[i] + r.pad - r.ox) * r.ix;
-        const float ty0 = (ly[i] - r.pad - r.oy) * r.iy, ty1 = (hy[i] + r.pad - r.oy) * r.iy;
-        const float tz0 = (lz[i] - r.pad - r.oz) * r.iz, tz1 = (hz[i] + r.pad - r.oz) * r.iz;
-        const float tnear = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-        const float tfar = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+        const float tx0 = (nxs[i] + px - r.ox) * r.ix, tx1 = (fxs[i] - px - r.ox) * r.ix;
+        const float ty0 = (nys[i] + py - r.oy) * r.iy, ty1 = (fys[i] - py - r.oy) * r.iy;
+        const float tz0 = (nzs[i] + pz - r.oz) * r.iz, tz1 = (fzs[i] - pz - r.oz) * r.iz;
+        const float tnear = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, tmin));
+        const float tfar = fminf(fminf(tx1, ty1), fminf(tz1, tmax));
         const bool hit = c[i] != kEmptyChild && tnear <= tfar * 1.00000048f;
         tn[i] = hit ? tnear : INFINITY;
         ref[i] = c[i];
@@ -165,12 +179,10 @@ __device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin
             if (kCount) ++nodes;
             float tn[4];
             std::uint32_t ref[4];
-            testNodeBox(s.nodes + cur, box, tmin, best.t, tn, ref);
+            testNodeBox(s.nodes, cur, box, tmin, best.t, tn, ref);
             HDCX_SWAP_IF2(0, 1) HDCX_SWAP_IF2(2, 3) HDCX_SWAP_IF2(0, 2) HDCX_SWAP_IF2(1, 3) HDCX_SWAP_IF2(1, 2)
             if (tn[0] != INFINITY) {
-                if (tn[3] != INFINITY) st.push(ref[3]);
-                if (tn[2] != INFINITY) st.push(ref[2]);
-                if (tn[1] != INFINITY) st.push(ref[1]);
+                st.pushFar3(ref[3], tn[3] != INFINITY, ref[2], tn[2] != INFINITY, ref[1], tn[1] != INFINITY);
                 cur = ref[0];
                 continue;
             }
@@ -233,17 +245,9 @@ __device__ bool traceAny2L(const DeviceScene& s, const RayPre& r, float tmin, fl
             if (kCount) ++nodes;
             float tn[4];
             std::uint32_t ref[4];
-            testNodeBox(s.nodes + cur, box, tmin, tmax, tn, ref);
+            testNodeBox(s.nodes, cur, box, tmin, tmax, tn, ref);
             std::uint32_t next = 0xffffffffU;
-            bool have = false;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (tn[i] != INFINITY) {
-                    if (!have) { next = ref[i]; have = true; }
-                    else st.push(ref[i]);
-                }
-            }
-            if (have) { cur = next; continue; }
+            if (st.pushAfterFirst(tn, ref, next)) { cur = next; continue; }
         }
         if (st.sp == 0) break;
         cur = st.pop();

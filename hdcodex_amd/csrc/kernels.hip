@@ -178,11 +178,11 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // Occupancy, measured on the 1080p heightfield and 10,000-instance frames:
 // two-level extend runs best held to 7 waves per SIMD when opaque (6
 // otherwise; 4 or 5 waves without spills are 3-8% slower). One-level extend
-// runs best as the generic kernel at the compiler's choice, 78 VGPRs and 6
-// waves (18.5 ms per frame against 19.0 for the opaque kernel at 7 waves
-// and 19.2 at 8), so the host launches no one-level opaque kernel.
+// runs best as the generic kernel held to 6 waves (80 VGPRs; the opaque
+// kernel at 7 or 8 waves was 3-4% slower), so the host launches no
+// one-level opaque kernel.
 template <bool kCount, class Scenes, bool k2L, bool kOpaque>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? (kOpaque ? 7 : 6) : 1))) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? (kOpaque ? 7 : 6) : 6))) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
                                                       TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];

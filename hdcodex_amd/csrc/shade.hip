@@ -221,7 +221,10 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                     const float2 t0 = s.texcoords[tr.tri * 3], t1 = s.texcoords[tr.tri * 3 + 1], t2 = s.texcoords[tr.tri * 3 + 2];
                     const float su = t0.x * w0 + t1.x * bu + t2.x * bv, sv = t0.y * w0 + t1.y * bu + t2.y * bv;
                     const V3 hp = ro + rd * distance;
-                    const GpuMaterialRecord& m = s.materials[triangleMaterial(s, tr)];
+                    // The record by value: the fields this vertex reads are
+                    // loaded together here, instead of one dependent load
+                    // after every texture-slot test below.
+                    const GpuMaterialRecord m = s.materials[triangleMaterial(s, tr)];
 
                     // Material fetch (:1296-1437).
                     V3 baseRgb = v3(m.baseColorMetalness);
