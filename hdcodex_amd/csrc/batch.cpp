@@ -413,15 +413,17 @@ struct hdcx_batch : PathPool {
                       "batch extend");
                 const int last = b + 1 == bounces ? 1 : 0;
                 Check(LaunchMissBatch(table, pb, in, maxPaths, maxBlocks, st), "batch miss");
-                Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, walk, maxPaths, maxBlocks, st),
+                // Shade also pairs the lanes of each pixel sample with equal
+                // next rays and equal shadow rays.
+                Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, walk, 1, maxPaths, maxBlocks,
+                                       st),
                       "batch shade");
                 if (walk == kShadeWalkDefer && !last) {
                     Check(LaunchWalkBatch(table, twoLevel, pb, maxPaths, maxBlocks, st), "batch walk");
-                    Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, kShadeWalkResume, maxPaths,
-                                           maxBlocks, st),
+                    Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, kShadeWalkResume, 1,
+                                           maxPaths, maxBlocks, st),
                           "batch shade (resume)");
                 }
-                if (!last) Check(LaunchPairBatch(table, pb, in ^ 1U, maxPaths, maxBlocks, st), "batch pair");
                 if (cache && (b == cp.terminateBounce || (b > cp.terminateBounce && cp.spreadThreshold > 0.0F))) {
                     // Queries of this bounce, answered by their scenes' networks.
                     Check(LaunchRcInferScenes(rcWe.as<__bf16>(), cp.queries, cp.queryCount, maxPaths, pb.pathScene, buckets,
@@ -429,7 +431,6 @@ struct hdcx_batch : PathPool {
                           "batch rc infer");
                     Check(LaunchConsumeQueries(cp, st), "batch rc queries");
                 }
-                Check(LaunchShadowPairBatch(table, pb, maxPaths, maxBlocks, st), "batch shadow pair");
                 Check(LaunchShadowBatch(table, twoLevel, opaque, pb, maxPaths, maxBlocks, nullptr, st), "batch shadow");
             }
             if (cache)

@@ -75,6 +75,40 @@ __device__ __forceinline__ void pairAppend(bool want, std::uint32_t pid, std::ui
     __syncthreads();
 }
 
+// Leader / follower decision of one path among the lanes of its pixel
+// sample, from kWords words that must be equal bit for bit (the lane
+// pairing at the end of shade). Returns the follower mask of a path that
+// traces (bit j: pid + 1 + j) in `followers`; true when the path traces.
+// (Skipping the comparisons of words or distances that no lane of the wave
+// needs, behind ballots, measured slower.)
+template <int kWords>
+__device__ __forceinline__ bool pairLanes(bool live, std::uint32_t pid, const std::uint32_t (&w)[kWords],
+                                          std::uint32_t& followers)
+{
+    const std::uint32_t lane = laneId(), sampleLane = pid % 3U;
+    std::uint32_t back = 0U;  // 0: traced here; 1 or 2: path pid - back traces it
+    std::uint32_t dist = 0U;  // threads back to that path
+    for (std::uint32_t k = 1U; k <= 2U; ++k) {
+        const std::uint32_t qp = __shfl_up(pid, k);
+        const int ql = __shfl_up(static_cast<int>(live), k);
+        bool same = live && lane >= k && ql != 0 && qp < pid && pid - qp <= sampleLane;
+        for (int j = 0; j < kWords; ++j) same = (__shfl_up(w[j], k) == w[j]) && same;
+        if (same && pid - qp > back) {
+            back = pid - qp;
+            dist = k;
+        }
+    }
+    const std::uint32_t leaderBack1 = __shfl_up(back, 1U), leaderBack2 = __shfl_up(back, 2U);
+    if (back != 0U && (dist == 1U ? leaderBack1 : leaderBack2) != 0U) back = 0U;
+    followers = 0U;
+    for (std::uint32_t k = 1U; k <= 2U; ++k) {
+        const std::uint32_t qp = __shfl_down(pid, k);
+        const std::uint32_t qb = __shfl_down(back, k);
+        if (lane + k < 64U && qb != 0U && qp - qb == pid) followers |= 1U << (qp - pid - 1U);
+    }
+    return live && back == 0U;
+}
+
 template <bool k2L>
 __device__ __forceinline__ float materialOpacity(const DeviceScene& s, std::uint32_t prim, float bu, float bv)
 {

@@ -20,16 +20,12 @@ hipError_t LaunchGatherTraining(const PathBuffers& pb, const CacheParams& cp, st
 std::size_t GatherScratchWords(std::uint32_t records);
 // Extend sharing: kShareNone traces every queued path; kShareCamera (the
 // launch after raygen) traces one camera ray per pixel sample for its
-// three lanes; kShareLanes traces the paths LaunchPair queued as tracing
-// and writes their hits to the later lanes of their sample with the same
-// ray.
+// three lanes; kShareLanes traces the paths the previous shade queued as
+// tracing (its lane pairing) and writes their hits to the later lanes of
+// their sample with the same ray.
 constexpr int kShareNone = 0, kShareCamera = 1, kShareLanes = 2;
 hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int sharing, std::uint32_t maxWork,
                         unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
-// Lane sharing for the next extend (run with kShareLanes), over the paths
-// that survived this bounce: queue[out], after shade and its walk resume.
-hipError_t LaunchPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
-                      unsigned maxBlocks, hipStream_t st);
 // walk: kShadeWalkInline runs subsurface walks inside shade;
 // kShadeWalkDefer queues them (then LaunchWalk, then shade again with
 // kShadeWalkResume over the walk queue: same results, the walks' traversal
@@ -41,14 +37,14 @@ constexpr int kShadeWalkInline = 0, kShadeWalkDefer = 1, kShadeWalkResume = 2, k
 // queues the others for shade; run between extend and shade.
 hipError_t LaunchMiss(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                       unsigned maxBlocks, hipStream_t st);
+// share: shade also pairs the lanes of a pixel sample with equal next rays
+// (the queue the next extend reads with kShareLanes) and with equal shadow
+// rays (LaunchShadow with shared = true traces each group once).
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                       const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st);
+                       const CacheParams& cp, int walk, int share, std::uint32_t maxWork, unsigned maxBlocks,
+                       hipStream_t st);
 hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                       hipStream_t st);
-// Groups the lanes of a pixel sample with equal shadow rays (shadowQueue);
-// LaunchShadow with shared = true then traces each group once.
-hipError_t LaunchShadowPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
-                            hipStream_t st);
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st);
 // Batch renderer: the same stages over a table of scenes (scene of each
@@ -59,18 +55,14 @@ hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, bool shared
 hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, bool opaque, const PathBuffers& pb,
                              std::uint32_t in, int sharing, std::uint32_t maxWork, unsigned maxBlocks,
                              TraversalCounters* ctr, hipStream_t st);
-hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
-                           unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchMissBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st);
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
-                            int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,
+                            int last, const CacheParams& cp, int walk, int share, std::uint32_t maxWork, unsigned maxBlocks,
                             hipStream_t st);
 hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st);
-hipError_t LaunchShadowPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork,
-                                 unsigned maxBlocks, hipStream_t st);
-// The batch always shares shadow rays (LaunchShadowPairBatch first).
+// The batch always shares rays (its shade runs with share = 1).
 hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, bool opaque, const PathBuffers& pb,
                              std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchRaygenBatch(const PathBuffers& pb, const BatchWaves& bw, const CacheParams& cp, std::uint32_t maxWork,

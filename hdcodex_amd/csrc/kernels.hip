@@ -166,7 +166,7 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // three per pixel sample, and the three lanes of a sample start with the
 // same camera ray and the same random state (the seed and the jitter do
 // not depend on the lane; only the wavelength does), so the ray is traced
-// once, for lane 0. kShareLanes: pairKernel queued the paths that trace
+// once, for lane 0. kShareLanes: the previous shade queued the paths that trace
 // (pb.extendQueue) with a mask of the later lanes of their sample whose
 // next ray it found equal (pb.follow). The hit (and, in a scene with
 // cut-outs, the random state after the opacity draws) is written to the
@@ -234,127 +234,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? (k
 }
 
 // ---------------------------------------------------------------------------
-// pair: lane sharing for the next extend (kShareLanes), over the paths that
-// survived this bounce (queue[out]) in queue order. With the same random
-// numbers, the lanes of a pixel sample keep the same rays until a lobe or
-// light choice differs. A path whose next ray (and, in a scene with
-// cut-outs, random state) equals bit for bit that of an earlier lane of its
-// sample, held by one of the two preceding threads of the wave, is not
-// traced: that lane's extend writes the hit to both. Lanes of a sample have
-// consecutive path ids (lane = pid % 3) and keep their order in the queues,
-// but another path can sit between two of them (the queues are filled wave
-// by wave in atomic order), so the earliest equal lane may be out of a
-// path's sight while the lane it sees follows that earlier one: the
-// earliest visible equal lane is taken, and kept only if it traces itself,
-// so every path that does not trace has a leader that does. The tracing
-// paths go to extendQueue[out], one atomic per block of 16 waves, with the
-// mask of their followers in follow[].
-// ---------------------------------------------------------------------------
-// Leader / follower decision of one path among the lanes of its pixel
-// sample, from kWords words that must be equal bit for bit (see pairKernel).
-// Returns the follower mask of a path that traces (bit j: pid + 1 + j) in
-// `followers`; true when the path traces.
-template <int kWords>
-__device__ __forceinline__ bool pairLanes(bool live, std::uint32_t pid, const std::uint32_t (&w)[kWords],
-                                          std::uint32_t& followers)
-{
-    const std::uint32_t lane = laneId(), sampleLane = pid % 3U;
-    std::uint32_t back = 0U;  // 0: traced here; 1 or 2: path pid - back traces it
-    std::uint32_t dist = 0U;  // threads back to that path
-    for (std::uint32_t k = 1U; k <= 2U; ++k) {
-        const std::uint32_t qp = __shfl_up(pid, k);
-        const int ql = __shfl_up(static_cast<int>(live), k);
-        bool same = live && lane >= k && ql != 0 && qp < pid && pid - qp <= sampleLane;
-        for (int j = 0; j < kWords; ++j) same = (__shfl_up(w[j], k) == w[j]) && same;
-        if (same && pid - qp > back) {
-            back = pid - qp;
-            dist = k;
-        }
-    }
-    const std::uint32_t leaderBack1 = __shfl_up(back, 1U), leaderBack2 = __shfl_up(back, 2U);
-    if (back != 0U && (dist == 1U ? leaderBack1 : leaderBack2) != 0U) back = 0U;
-    followers = 0U;
-    for (std::uint32_t k = 1U; k <= 2U; ++k) {
-        const std::uint32_t qp = __shfl_down(pid, k);
-        const std::uint32_t qb = __shfl_down(back, k);
-        if (lane + k < 64U && qb != 0U && qp - qb == pid) followers |= 1U << (qp - pid - 1U);
-    }
-    return live && back == 0U;
-}
-
-template <class Scenes>
-__global__ __launch_bounds__(kPairBlock) void pairKernel(Scenes scenes, PathBuffers pb, std::uint32_t out)
-{
-    __shared__ std::uint32_t waveCount[kPairBlock / 64U], waveBase[kPairBlock / 64U];
-    const std::uint32_t count = pb.counters[out * kCounterStride];
-    const std::uint32_t* queue = pb.queue[out];
-    const std::uint32_t stride = gridDim.x * blockDim.x;
-    const std::uint32_t padded = (count + stride - 1U) / stride * stride;  // every thread reaches the barriers
-    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < padded; i += stride) {
-        const bool live = i < count;
-        std::uint32_t pid = 0U;
-        std::uint32_t w[7] = {0U, 0U, 0U, 0U, 0U, 0U, 0U};  // ray origin, direction, random state (cut-outs only)
-        if (live) {
-            pid = queue[i];
-            const float4 o = pb.rayO[pid], d = pb.rayD[pid];
-            w[0] = __float_as_uint(o.x);
-            w[1] = __float_as_uint(o.y);
-            w[2] = __float_as_uint(o.z);
-            w[3] = __float_as_uint(d.x);
-            w[4] = __float_as_uint(d.y);
-            w[5] = __float_as_uint(d.z);
-            if (!scenes.Get(pid).opaque) w[6] = pb.rng[pid];
-        }
-        std::uint32_t followers;
-        const bool traces = pairLanes<7>(live, pid, w, followers);
-        if (traces) pb.follow[pid] = followers;
-        pairAppend(traces, pid, pb.extendQueue[out], pb.counters + (4U + out) * kCounterStride, waveCount, waveBase);
-    }
-}
-
-// shadowPair: the same grouping over the paths with shadow rays
-// (shadowQueue), comparing every slot's light, origin, extent and
-// direction. The shadow kernel then traces each group's rays once and
-// applies the visibility to every lane of the group with the lane's own
-// wavelength and contribution (the visibility test itself draws no random
-// numbers). Leaders go to shadowLeaders with their masks in shadowFollow.
-template <class Scenes>
-__global__ __launch_bounds__(kPairBlock) void shadowPairKernel(Scenes scenes, PathBuffers pb)
-{
-    __shared__ std::uint32_t waveCount[kPairBlock / 64U], waveBase[kPairBlock / 64U];
-    const std::uint32_t count = pb.counters[2 * kCounterStride];
-    const std::uint32_t stride = gridDim.x * blockDim.x;
-    const std::uint32_t padded = (count + stride - 1U) / stride * stride;  // every thread reaches the barriers
-    for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < padded; i += stride) {
-        const bool live = i < count;
-        std::uint32_t pid = 0U;
-        std::uint32_t w[8 * kShadowSlots];
-        for (int j = 0; j < 8 * kShadowSlots; ++j) w[j] = 0U;
-        if (live) {
-            pid = pb.shadowQueue[i];
-            for (int k = 0; k < kShadowSlots; ++k) {
-                const std::size_t idx = static_cast<std::size_t>(k) * pb.capacity + pid;
-                const std::uint32_t light = pb.shadowLight[idx];
-                w[8 * k] = light;
-                if (light == 0xffffffffU) break;
-                const float4 o = pb.shadowO[idx], d = pb.shadowD[idx];
-                w[8 * k + 1] = __float_as_uint(o.x);
-                w[8 * k + 2] = __float_as_uint(o.y);
-                w[8 * k + 3] = __float_as_uint(o.z);
-                w[8 * k + 4] = __float_as_uint(o.w);
-                w[8 * k + 5] = __float_as_uint(d.x);
-                w[8 * k + 6] = __float_as_uint(d.y);
-                w[8 * k + 7] = __float_as_uint(d.z);
-            }
-        }
-        std::uint32_t followers;
-        const bool traces = pairLanes<8 * kShadowSlots>(live, pid, w, followers);
-        if (traces) pb.shadowFollow[pid] = followers;
-        pairAppend(traces, pid, pb.shadowLeaders, pb.counters + 6U * kCounterStride, waveCount, waveBase);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // shadow: visibility of each queued path's shadow rays, added to its
 // radiance in slot order (shadowVisibility :855-889, meshLightVisible
 // :891-912). kOpaque as in extendKernel. Held to 7 waves per SIMD for
@@ -372,7 +251,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kOpaque
     unsigned int traced = 0U;
     const std::uint32_t queued = pb.counters[2 * kCounterStride];
     if (blockIdx.x == 0 && threadIdx.x == 0) pb.totals[1] += queued;
-    // shared: shadowPairKernel grouped the lanes of a pixel sample with
+    // shared: shade grouped the lanes of a pixel sample with
     // equal shadow rays; each group's rays are traced once and the
     // visibility applied to every lane of the group.
     const std::uint32_t count = shared ? pb.counters[6 * kCounterStride] : queued;
@@ -823,14 +702,6 @@ hipError_t LaunchExtendT(const Scenes& s, bool opaque, const PathBuffers& pb, st
     LaunchExtendK<Scenes, k2L, false>(s, pb, in, sharing, maxWork, maxBlocks, ctr, st);
     return hipGetLastError();
 }
-template <class Scenes>
-hipError_t LaunchPairT(const Scenes& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
-                       unsigned maxBlocks, hipStream_t st)
-{
-    const unsigned blocks = StageGrid(pairKernel<Scenes>, maxWork, maxBlocks, kPairBlock);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(pairKernel<Scenes>), dim3(blocks), dim3(kPairBlock), 0, st, s, pb, out);
-    return hipGetLastError();
-}
 template <class Scenes, bool k2L, bool kOpaque>
 void LaunchShadowK(const Scenes& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                    TraversalCounters* ctr, hipStream_t st)
@@ -860,19 +731,6 @@ hipError_t LaunchExtend(const DeviceScene& s, const PathBuffers& pb, std::uint32
                ? LaunchExtendT<SingleScene, true>(SingleScene{s}, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st)
                : LaunchExtendT<SingleScene, false>(SingleScene{s}, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st);
 }
-hipError_t LaunchPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
-                      unsigned maxBlocks, hipStream_t st)
-{
-    return LaunchPairT<SingleScene>(SingleScene{s}, pb, out, maxWork, maxBlocks, st);
-}
-hipError_t LaunchShadowPair(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
-                            hipStream_t st)
-{
-    const unsigned blocks = StageGrid(shadowPairKernel<SingleScene>, maxWork, maxBlocks, kPairBlock);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowPairKernel<SingleScene>), dim3(blocks), dim3(kPairBlock), 0, st,
-                       SingleScene{s}, pb);
-    return hipGetLastError();
-}
 hipError_t LaunchShadow(const DeviceScene& s, const PathBuffers& pb, bool shared, std::uint32_t maxWork, unsigned maxBlocks,
                         TraversalCounters* ctr, hipStream_t st)
 {
@@ -888,19 +746,6 @@ hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, bool opaqu
     const SceneTable t{table, pb.pathScene};
     return twoLevel ? LaunchExtendT<SceneTable, true>(t, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st)
                     : LaunchExtendT<SceneTable, false>(t, opaque, pb, in, sharing, maxWork, maxBlocks, ctr, st);
-}
-hipError_t LaunchPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t out, std::uint32_t maxWork,
-                           unsigned maxBlocks, hipStream_t st)
-{
-    return LaunchPairT<SceneTable>(SceneTable{table, pb.pathScene}, pb, out, maxWork, maxBlocks, st);
-}
-hipError_t LaunchShadowPairBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork,
-                                 unsigned maxBlocks, hipStream_t st)
-{
-    const unsigned blocks = StageGrid(shadowPairKernel<SceneTable>, maxWork, maxBlocks, kPairBlock);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(shadowPairKernel<SceneTable>), dim3(blocks), dim3(kPairBlock), 0, st,
-                       SceneTable{table, pb.pathScene}, pb);
-    return hipGetLastError();
 }
 hipError_t LaunchShadowBatch(const DeviceScene* table, bool twoLevel, bool opaque, const PathBuffers& pb,
                              std::uint32_t maxWork, unsigned maxBlocks, TraversalCounters* ctr, hipStream_t st)

@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kPairBlock) void missKernel(Scenes scenes, PathBuff
 // ---------------------------------------------------------------------------
 template <class Scenes, bool k2L, int kWalk>
 __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int bounce,
-                                                     int lastBounce, CacheParams cp)
+                                                     int lastBounce, CacheParams cp, int share)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
     HDCX_TRAVERSAL_STACK(st, stackMem);
@@ -163,6 +163,9 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
         RcQuery query{};
         bool deferred = false;
         float4 walkIn0{}, walkIn1{}, walkIn2{};
+        // The continuing ray (and, with cut-outs, random state) compared by
+        // the lane pairing below.
+        std::uint32_t pairWords[7] = {0U, 0U, 0U, 0U, 0U, 0U, 0U};
         if (i < count) {
             pid = queue[i];
             const DeviceScene& s = scenes.Get(pid);
@@ -741,6 +744,13 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                         pb.flags[pid] = flags;
                         pb.mis[pid] = mis;
                         pb.medium[pid] = med;
+                        pairWords[0] = __float_as_uint(ro.x);
+                        pairWords[1] = __float_as_uint(ro.y);
+                        pairWords[2] = __float_as_uint(ro.z);
+                        pairWords[3] = __float_as_uint(rd.x);
+                        pairWords[4] = __float_as_uint(rd.y);
+                        pairWords[5] = __float_as_uint(rd.z);
+                        if (!s.opaque) pairWords[6] = rng;
                     }
                     pb.rng[pid] = rng;
                     pb.throughput[pid] = throughput;
@@ -750,9 +760,50 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
             }
         }
         waveAppend(keep, pid, outQueue, outCounter);
+        // Lane sharing (share != 0): paths whose next ray equals, bit for
+        // bit, that of an earlier lane of their pixel sample held by one of
+        // the two preceding lanes of this wave do not trace it; the tracing
+        // paths go to the next extend's queue with the mask of their
+        // followers (pairLanes). The lanes of a sample have consecutive path
+        // ids and sit next to each other in the hit queue whenever they hit.
+        if (share && !lastBounce) {
+            std::uint32_t followers;
+            const bool traces = pairLanes<7>(keep, pid, pairWords, followers);
+            if (traces) pb.follow[pid] = followers;
+            waveAppend(traces, pid, pb.extendQueue[in ^ 1U], pb.counters + (4U + (in ^ 1U)) * kCounterStride);
+        }
         if constexpr (kWalk != kShadeWalkResume) {
             // A resumed path queued its shadow rays when it was deferred.
             waveAppend(slot > 0, pid, pb.shadowQueue, pb.counters + 2 * kCounterStride);
+            if (share) {
+                // The same grouping of the shadow rays just queued: every
+                // slot's light, origin, extent and direction must be equal.
+                // The shadow kernel traces each group once and applies the
+                // visibility to every lane of it with the lane's own
+                // wavelength and contribution.
+                std::uint32_t w[8 * kShadowSlots];
+                for (int j = 0; j < 8 * kShadowSlots; ++j) w[j] = 0U;
+                if (slot > 0) {
+                    for (int k = 0; k < kShadowSlots; ++k) {
+                        const std::size_t idx = static_cast<std::size_t>(k) * pb.capacity + pid;
+                        const std::uint32_t light = k < slot ? pb.shadowLight[idx] : 0xffffffffU;
+                        w[8 * k] = light;
+                        if (light == 0xffffffffU) break;
+                        const float4 o = pb.shadowO[idx], d = pb.shadowD[idx];
+                        w[8 * k + 1] = __float_as_uint(o.x);
+                        w[8 * k + 2] = __float_as_uint(o.y);
+                        w[8 * k + 3] = __float_as_uint(o.z);
+                        w[8 * k + 4] = __float_as_uint(o.w);
+                        w[8 * k + 5] = __float_as_uint(d.x);
+                        w[8 * k + 6] = __float_as_uint(d.y);
+                        w[8 * k + 7] = __float_as_uint(d.z);
+                    }
+                }
+                std::uint32_t followers;
+                const bool traces = pairLanes<8 * kShadowSlots>(slot > 0, pid, w, followers);
+                if (traces) pb.shadowFollow[pid] = followers;
+                waveAppend(traces, pid, pb.shadowLeaders, pb.counters + 6U * kCounterStride);
+            }
             if (cp.enabled) {
                 const std::uint32_t qslot = waveSlot(wantQuery, cp.queryCount);
                 if (wantQuery && qslot < cp.queryCapacity) cp.queries[qslot] = query;
@@ -805,24 +856,24 @@ hipError_t LaunchMissT(const Scenes& s, const PathBuffers& pb, std::uint32_t in,
 }
 template <class Scenes, bool k2L>
 hipError_t LaunchShadeT(const Scenes& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                        const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+                        const CacheParams& cp, int walk, int share, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
     if (walk == kShadeSimple)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeSimple>),
                            dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeSimple>, maxWork, maxBlocks)), dim3(kBlock), 0, st, s,
-                           pb, in, bounce, last, cp);
+                           pb, in, bounce, last, cp, share);
     else if (walk == kShadeWalkDefer)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkDefer>),
                            dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeWalkDefer>, maxWork, maxBlocks)), dim3(kBlock), 0, st,
-                           s, pb, in, bounce, last, cp);
+                           s, pb, in, bounce, last, cp, share);
     else if (walk == kShadeWalkResume)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkResume>),
                            dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeWalkResume>, maxWork, maxBlocks)), dim3(kBlock), 0,
-                           st, s, pb, in, bounce, last, cp);
+                           st, s, pb, in, bounce, last, cp, share);
     else
         hipLaunchKernelGGL(HIP_KERNEL_NAME(shadeKernel<Scenes, k2L, kShadeWalkInline>),
                            dim3(StageGrid(shadeKernel<Scenes, k2L, kShadeWalkInline>, maxWork, maxBlocks)), dim3(kBlock), 0,
-                           st, s, pb, in, bounce, last, cp);
+                           st, s, pb, in, bounce, last, cp, share);
     return hipGetLastError();
 }
 template <class Scenes, bool k2L>
@@ -839,11 +890,11 @@ hipError_t LaunchMiss(const DeviceScene& s, const PathBuffers& pb, std::uint32_t
     return LaunchMissT<SingleScene>(SingleScene{s}, pb, in, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShade(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, int bounce, int last,
-                       const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
+                       const CacheParams& cp, int walk, int share, std::uint32_t maxWork, unsigned maxBlocks, hipStream_t st)
 {
     return s.instanceCount
-        ? LaunchShadeT<SingleScene, true>(SingleScene{s}, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st)
-        : LaunchShadeT<SingleScene, false>(SingleScene{s}, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st);
+        ? LaunchShadeT<SingleScene, true>(SingleScene{s}, pb, in, bounce, last, cp, walk, share, maxWork, maxBlocks, st)
+        : LaunchShadeT<SingleScene, false>(SingleScene{s}, pb, in, bounce, last, cp, walk, share, maxWork, maxBlocks, st);
 }
 hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                       hipStream_t st)
@@ -857,12 +908,12 @@ hipError_t LaunchMissBatch(const DeviceScene* table, const PathBuffers& pb, std:
     return LaunchMissT<SceneTable>(SceneTable{table, pb.pathScene}, pb, in, maxWork, maxBlocks, st);
 }
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
-                            int last, const CacheParams& cp, int walk, std::uint32_t maxWork, unsigned maxBlocks,
+                            int last, const CacheParams& cp, int walk, int share, std::uint32_t maxWork, unsigned maxBlocks,
                             hipStream_t st)
 {
     const SceneTable t{table, pb.pathScene};
-    return twoLevel ? LaunchShadeT<SceneTable, true>(t, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st)
-                    : LaunchShadeT<SceneTable, false>(t, pb, in, bounce, last, cp, walk, maxWork, maxBlocks, st);
+    return twoLevel ? LaunchShadeT<SceneTable, true>(t, pb, in, bounce, last, cp, walk, share, maxWork, maxBlocks, st)
+                    : LaunchShadeT<SceneTable, false>(t, pb, in, bounce, last, cp, walk, share, maxWork, maxBlocks, st);
 }
 hipError_t LaunchWalkBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st)

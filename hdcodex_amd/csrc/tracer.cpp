@@ -350,17 +350,19 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                          [&] { return LaunchExtend(t->scene, pb, in, sharing, paths, t->maxBlocks, ctr, st); });
                 const int last = b + 1 == bounces ? 1 : 0;
                 t->Timed(HDCX_KERNEL_MISS, [&] { return LaunchMiss(t->scene, pb, in, paths, t->maxBlocks, st); });
+                // With sharing on, shade also pairs the lanes of each pixel
+                // sample with equal next rays and equal shadow rays.
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
-                    return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, walk, paths, t->maxBlocks, st);
+                    return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, walk, share ? 1 : 0, paths,
+                                       t->maxBlocks, st);
                 });
                 if (walk == kShadeWalkDefer && !last) {
                     t->Timed(HDCX_KERNEL_WALK, [&] { return LaunchWalk(t->scene, pb, paths, t->maxBlocks, st); });
                     t->Timed(HDCX_KERNEL_SHADE, [&] {
-                        return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, kShadeWalkResume, paths, t->maxBlocks, st);
+                        return LaunchShade(t->scene, pb, in, static_cast<int>(b), last, cp, kShadeWalkResume, share ? 1 : 0,
+                                           paths, t->maxBlocks, st);
                     });
                 }
-                if (share && !last)
-                    t->Timed(HDCX_KERNEL_PAIR, [&] { return LaunchPair(t->scene, pb, in ^ 1U, paths, t->maxBlocks, st); });
                 // Queries come from the terminate vertex, or from any later
                 // one when the spread rule is on.
                 if (cache && (b == cp.terminateBounce || (b > cp.terminateBounce && cp.spreadThreshold > 0.0F))) {
@@ -371,7 +373,6 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                     });
                     Check(LaunchConsumeQueries(cp, st), "rc queries");
                 }
-                if (share) t->Timed(HDCX_KERNEL_PAIR, [&] { return LaunchShadowPair(t->scene, pb, paths, t->maxBlocks, st); });
                 t->Timed(HDCX_KERNEL_SHADOW,
                          [&] { return LaunchShadow(t->scene, pb, share, paths, t->maxBlocks, shadowCtr, st); });
             }

@@ -500,7 +500,7 @@ int hdcx_batch_get_rc_stats(hdcx_batch* batch, hdcx_batch_rc_stats* out_stats);
 #define HDCX_KERNEL_FILM 4
 #define HDCX_KERNEL_CACHE 5 /* radiance-cache inference */
 #define HDCX_KERNEL_WALK 6  /* subsurface random walks deferred by shade */
-#define HDCX_KERNEL_PAIR 7  /* finds the lanes of a pixel sample with equal next rays */
+#define HDCX_KERNEL_PAIR 7  /* no launches: the lane pairing runs inside shade (slot kept for the layout) */
 #define HDCX_KERNEL_MISS 8  /* environment at misses; queues the hits for shade */
 #define HDCX_KERNEL_COUNT 9
 
