@@ -83,16 +83,18 @@ MISS_VERTEX_READ = 20     # queue entry 4, hit 16
 MISS_READ = 40            # on a miss: flags 4, MIS pdfs 8, throughput 4, radiance 4, direction 16, wavelength 4
 MISS_WRITE = 4            # on a miss: radiance (environment added)
 MISS_HIT_WRITE = 4        # on a hit: hit-queue entry
-# Bytes the shade kernel moves per surface hit (single-level scene, no
-# radiance cache), from its loads and stores:
-SHADE_VERTEX_READ = 96    # hit-queue entry 4, hit 16, ray origin 16 + direction 16, rng 4, wavelength 4,
-                          # throughput 4, radiance 4, flags 4, MIS pdfs 8, medium 16
-SHADE_VERTEX_WRITE = 12   # the three shadow-slot light ids cleared
+# Bytes the shade kernel moves per surface hit (single-level scene, the
+# simple-material variant that the heightfield runs, no radiance cache),
+# from its loads and stores:
+SHADE_VERTEX_READ = 80    # hit-queue entry 4, hit 16, ray origin 16 + direction 16, rng 4, wavelength 4,
+                          # throughput 4, radiance 4, flags 4, MIS pdfs 8 (no medium in this variant)
 SHADE_HIT_READ = 136      # three vertex indices 12, positions 3 x 16, corner normals 3 x 16, UVs 3 x 8,
                           # triangle material id 4
 SHADE_HIT_WRITE = 12      # rng, throughput, radiance
-SHADE_KEPT_WRITE = 64     # next ray origin + direction 32, flags 4, MIS pdfs 8, medium 16, queue entry 4
-SHADE_SHADOW_WRITE = 40   # shadow slot: origin + tmax 16, direction + contribution 16, light 4; queue entry 4
+SHADE_KEPT_WRITE = 56     # next ray origin + direction 32, flags 4, MIS pdfs 8, survivor-queue entry 4;
+                          # lane pairing: extend-queue entry 4 + follower mask 4 (counted for every kept path)
+SHADE_SHADOW_WRITE = 88   # shadow slot: origin + tmax 16, direction + contribution 16, light 4, list end 4,
+                          # shadow-queue entry 4; lane pairing: the slot read back 36, leader entry + mask 8
 
 
 def miss_bytes(timed: dict) -> dict:
@@ -117,7 +119,7 @@ def shade_bytes(timed: dict) -> dict:
     hits = timed["shade"]["items"]
     kept = max(timed["miss"]["items"] - timed["raygen"]["items"], 0)  # every later vertex was kept by the one before
     queued = timed["shadow_queued_paths"]
-    total = (hits * (SHADE_VERTEX_READ + SHADE_VERTEX_WRITE + SHADE_HIT_READ + SHADE_HIT_WRITE)
+    total = (hits * (SHADE_VERTEX_READ + SHADE_HIT_READ + SHADE_HIT_WRITE)
              + kept * SHADE_KEPT_WRITE + queued * SHADE_SHADOW_WRITE)
     return {"bytes": total, "bytes_per_hit": total / max(hits, 1), "hits": hits, "shadow_queued_paths": queued}
 

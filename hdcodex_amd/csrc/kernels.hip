@@ -64,10 +64,10 @@ __device__ __forceinline__ void startPath(const PathBuffers& pb, const CachePara
     pb.wavelength[pid] = wl;
     pb.throughput[pid] = 1.0f;
     pb.radiance[pid] = 0.0f;
+    // kPrevDelta: the MIS pdfs are read only after a vertex has set them;
+    // the medium only with kInMedium, set together with it; shadow slots
+    // only after shade has queued them. None of the three is written here.
     pb.flags[pid] = kEnvOnMiss | kPrevDelta;
-    pb.mis[pid] = make_float2(0.0f, 0.0f);
-    pb.medium[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (int k = 0; k < kShadowSlots; ++k) pb.shadowLight[k * pb.capacity + pid] = 0xffffffffU;
     pb.queue[0][pid] = pid;
 }
 

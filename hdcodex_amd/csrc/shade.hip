@@ -179,10 +179,11 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
             float radiance = pb.radiance[pid];
             std::uint32_t flags = pb.flags[pid];
             float2 mis = pb.mis[pid];
-            float4 med = pb.medium[pid];
+            // The medium matters only under kInMedium, which only the full
+            // variant's transmission lobe sets (with the medium).
+            float4 med = kFull ? pb.medium[pid] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float2 spreadOut = make_float2(0.0f, 0.0f);
             bool spreadSet = false;
-            for (int k = 0; k < kShadowSlots; ++k) pb.shadowLight[k * pb.capacity + pid] = 0xffffffffU;
 
             {
                 const float distance = hit4.x, bu = hit4.z, bv = hit4.w;
@@ -737,13 +738,18 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                         }
                     }
                 }
+                // The shadow slots end at the first light of 0xffffffff: a
+                // path that queued one or two ends its list here (a path
+                // with none is not in the shadow queue, and no pairing or
+                // shadow pass reads its slots).
+                if (slot > 0 && slot < kShadowSlots) pb.shadowLight[static_cast<std::size_t>(slot) * pb.capacity + pid] = 0xffffffffU;
                 if (!deferred) {
                     if (keep) {
                         pb.rayO[pid] = make_float4(ro.x, ro.y, ro.z, 0.0f);
                         pb.rayD[pid] = make_float4(rd.x, rd.y, rd.z, 0.0f);
                         pb.flags[pid] = flags;
                         pb.mis[pid] = mis;
-                        pb.medium[pid] = med;
+                        if (kFull) pb.medium[pid] = med;
                         pairWords[0] = __float_as_uint(ro.x);
                         pairWords[1] = __float_as_uint(ro.y);
                         pairWords[2] = __float_as_uint(ro.z);
