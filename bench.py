@@ -151,11 +151,12 @@ def cpu_share() -> dict:
             "affinity": affinity, "cgroup_quota": quota, "omp_num_threads": omp}
 
 
-def cpu_baseline(scene, cam, share: dict, target_s: float = 15.0) -> dict:
+def cpu_baseline(scene, cam, share: dict, target_s: float = 25.0) -> dict:
     """FP32 CPU restatement (the -O3 build for this host's x86-64 level, SAH
     BVH) timed on the host's CPU share over the heightfield frame: a pilot on
-    every 60th row picks the row stride so that the timed sample takes about
-    target_s seconds (every row when that fits), all columns, all 16 spp."""
+    every 60th row picks the row stride so that the timed sample takes at
+    most about target_s seconds (every row when that fits: about 14 s on a
+    16-thread share), all columns, all 16 spp."""
     import oracle
 
     threads = share["threads"]
