@@ -41,6 +41,10 @@ public:
                DeviceBuffer& slots, RefitPlan& plan, DeviceBvhStats& stats);
 
 private:
+    // Stable radix sort of (Morton code, triangle id) pairs; the result is in
+    // (sa, sb), a and b are overwritten.
+    void SortPairs30(std::uint32_t* a, std::uint32_t* b, std::uint32_t* sa, std::uint32_t* sb, std::uint32_t n,
+                     hipStream_t st);
     DeviceBuffer morton_, mortonSorted_, ids_, idsSorted_, triBox_, child_, parent_, leafParent_, range_, arrivals_,
         nodeBox_, nodeCost_, leafFlag_, source_, nodeStack_, counters_, frontier_, frontierBox_, temp_;
 };

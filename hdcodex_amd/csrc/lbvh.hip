@@ -7,7 +7,8 @@
 // (src/gpu/vulkan_path_tracer.cpp:3032-3125).
 //
 //   1. Morton code (10 bits per axis) of each triangle's box centre, sorted
-//      with its triangle id by rocPRIM's radix sort (30 key bits, stable).
+//      with its triangle id by a stable LSD radix sort (30 key bits, three
+//      passes of 10 bits; below).
 //   2. Binary radix tree, one thread per inner node.
 //   3. Boxes bottom-up: the second child to arrive at a node unions both
 //      children (per-node counter, agent-scope acquire/release).
@@ -21,8 +22,6 @@
 #include "lbvh.h"
 #include "lbvh_core.h"
 #include "lbvh_host.hpp"
-
-#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 
@@ -48,6 +47,106 @@ struct DeviceAtomics {
 
 #define HDCX_GRID_STRIDE(i, begin, end) \
     for (std::uint32_t i = (begin) + blockIdx.x * blockDim.x + threadIdx.x; i < (end); i += gridDim.x * blockDim.x)
+
+// ---- Stable LSD radix sort of (key, value) pairs, 10-bit digits ----
+//
+// One wave per tile of kSortTile consecutive pairs. A pass over digit
+// (key >> shift) & 1023:
+//   histogram: each tile counts its digits (LDS) into hist[tile * 1024 + d];
+//   scan (one block of 1024 threads, thread d): hist[t][d] becomes the count
+//     of digit d in tiles before t, and start[d] the count of all digits
+//     below d (an exclusive scan of the digit totals);
+//   scatter: each tile walks its pairs in order, 64 per round; the lanes of
+//     a round with equal digits find each other with ten ballots (one per
+//     digit bit), take consecutive places in lane order after the digit's
+//     running count in LDS, and write to start[d] + hist[t][d] + place.
+// Pairs of equal digit keep their order within a round, across rounds and
+// across tiles, so the sort is stable (the serial run of the builder uses
+// std::stable_sort on the same keys).
+constexpr std::uint32_t kSortDigits = 1024;
+constexpr std::uint32_t kSortTile = 64U * 64U;
+
+__global__ __launch_bounds__(64) void sortHistogramKernel(const std::uint32_t* __restrict__ keys, std::uint32_t n,
+                                                          std::uint32_t shift, std::uint32_t* __restrict__ hist)
+{
+    __shared__ std::uint32_t count[kSortDigits];
+    for (std::uint32_t d = threadIdx.x; d < kSortDigits; d += 64U) count[d] = 0U;
+    __syncthreads();
+    const std::uint32_t base = blockIdx.x * kSortTile;
+    const std::uint32_t end = min(n, base + kSortTile);
+    for (std::uint32_t i = base + threadIdx.x; i < end; i += 64U) atomicAdd(&count[(keys[i] >> shift) & (kSortDigits - 1U)], 1U);
+    __syncthreads();
+    for (std::uint32_t d = threadIdx.x; d < kSortDigits; d += 64U) hist[static_cast<std::size_t>(blockIdx.x) * kSortDigits + d] = count[d];
+}
+
+__global__ __launch_bounds__(kSortDigits) void sortScanKernel(std::uint32_t* __restrict__ hist, std::uint32_t tiles,
+                                                              std::uint32_t* __restrict__ start)
+{
+    __shared__ std::uint32_t total[kSortDigits];
+    const std::uint32_t d = threadIdx.x;
+    std::uint32_t running = 0U;
+    for (std::uint32_t t = 0; t < tiles; ++t) {
+        const std::size_t k = static_cast<std::size_t>(t) * kSortDigits + d;
+        const std::uint32_t c = hist[k];
+        hist[k] = running;
+        running += c;
+    }
+    total[d] = running;
+    __syncthreads();
+    // Exclusive scan of the 1024 totals (Hillis-Steele on an inclusive copy).
+    for (std::uint32_t off = 1U; off < kSortDigits; off <<= 1) {
+        const std::uint32_t v = d >= off ? total[d - off] : 0U;
+        __syncthreads();
+        total[d] += v;
+        __syncthreads();
+    }
+    start[d] = total[d] - running;
+}
+
+__global__ __launch_bounds__(64) void sortScatterKernel(const std::uint32_t* __restrict__ keys,
+                                                        const std::uint32_t* __restrict__ values, std::uint32_t n,
+                                                        std::uint32_t shift, const std::uint32_t* __restrict__ hist,
+                                                        const std::uint32_t* __restrict__ start,
+                                                        std::uint32_t* __restrict__ keysOut,
+                                                        std::uint32_t* __restrict__ valuesOut)
+{
+    __shared__ std::uint32_t next[kSortDigits];  // output index of the next pair of each digit
+    const std::size_t row = static_cast<std::size_t>(blockIdx.x) * kSortDigits;
+    for (std::uint32_t d = threadIdx.x; d < kSortDigits; d += 64U) next[d] = start[d] + hist[row + d];
+    __syncthreads();
+    const std::uint32_t lane = threadIdx.x;
+    const unsigned long long below = (1ULL << lane) - 1ULL;
+    const std::uint32_t base = blockIdx.x * kSortTile;
+    const std::uint32_t end = min(n, base + kSortTile);
+    for (std::uint32_t r = base; r < end; r += 64U) {
+        const std::uint32_t i = r + lane;
+        const bool valid = i < end;
+        const std::uint32_t key = valid ? keys[i] : 0U;
+        const std::uint32_t value = valid ? values[i] : 0U;
+        const std::uint32_t digit = (key >> shift) & (kSortDigits - 1U);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (std::uint32_t b = 0; b < 10U; ++b) {
+            const bool set = (digit >> b) & 1U;
+            const unsigned long long m = __ballot(set);
+            peers &= set ? m : ~m;
+        }
+        // Lanes with this lane's digit (itself included, when valid); the
+        // lowest of them reserves places for the whole group.
+        const std::uint32_t place = static_cast<std::uint32_t>(__popcll(peers & below));
+        const int first = __ffsll(static_cast<long long>(peers)) - 1;
+        std::uint32_t at = 0U;
+        if (valid && place == 0U) {
+            at = next[digit];
+            next[digit] = at + static_cast<std::uint32_t>(__popcll(peers));
+        }
+        at = __shfl(at, valid ? first : 0);
+        if (valid) {
+            keysOut[at + place] = key;
+            valuesOut[at + place] = value;
+        }
+    }
+}
 
 __global__ void prepareKernel(const float* __restrict__ pos, const std::uint32_t* __restrict__ idx, std::uint32_t n,
                               float lox, float loy, float loz, float sx, float sy, float sz,
@@ -132,6 +231,31 @@ __global__ void refitLevelKernel(const std::uint32_t* __restrict__ order, std::u
 
 } // namespace
 
+// Sorts n (key, value) pairs by the low 30 key bits, stably: three passes
+// of 10 bits, (a, b) -> (sa, sb) -> (a, b) -> (sa, sb).
+void DeviceBvhBuilder::SortPairs30(std::uint32_t* a, std::uint32_t* b, std::uint32_t* sa, std::uint32_t* sb,
+                                   std::uint32_t n, hipStream_t st)
+{
+    const std::uint32_t tiles = (n + kSortTile - 1U) / kSortTile;
+    temp_.Reserve(sizeof(std::uint32_t) * (static_cast<std::size_t>(tiles) * kSortDigits + kSortDigits));
+    std::uint32_t* hist = temp_.as<std::uint32_t>();
+    std::uint32_t* start = hist + static_cast<std::size_t>(tiles) * kSortDigits;
+    std::uint32_t* inK = a;
+    std::uint32_t* inV = b;
+    std::uint32_t* outK = sa;
+    std::uint32_t* outV = sb;
+    for (std::uint32_t shift = 0; shift < 30U; shift += 10U) {
+        hipLaunchKernelGGL(sortHistogramKernel, dim3(tiles), dim3(64), 0, st, inK, n, shift, hist);
+        Check(hipGetLastError(), "sort histogram");
+        hipLaunchKernelGGL(sortScanKernel, dim3(1), dim3(kSortDigits), 0, st, hist, tiles, start);
+        Check(hipGetLastError(), "sort scan");
+        hipLaunchKernelGGL(sortScatterKernel, dim3(tiles), dim3(64), 0, st, inK, inV, n, shift, hist, start, outK, outV);
+        Check(hipGetLastError(), "sort scatter");
+        std::swap(inK, outK);
+        std::swap(inV, outV);
+    }
+}
+
 void DeviceBvhBuilder::Build(const float4* positions, const std::uint32_t* indices, std::uint32_t n, const float lo[3],
                              const float hi[3], std::uint32_t targetClusters, hipStream_t st, DeviceBuffer& nodes,
                              DeviceBuffer& slots, RefitPlan& plan, DeviceBvhStats& stats)
@@ -177,11 +301,7 @@ void DeviceBvhBuilder::Build(const float4* positions, const std::uint32_t* indic
     lbvh::MortonScale(lo, hi, scale);
     const unsigned g = Grid(n);
     HDCX_LAUNCH(prepareKernel, g, pos, indices, n, lo[0], lo[1], lo[2], scale[0], scale[1], scale[2], keys, ids, triBox);
-    std::size_t tempBytes = 0;
-    Check(rocprim::radix_sort_pairs(nullptr, tempBytes, keys, sortedKeys, ids, sortedIds, n, 0, 30, st), "radix sort");
-    temp_.Reserve(std::max<std::size_t>(tempBytes, 16));
-    Check(rocprim::radix_sort_pairs(temp_.as<void>(), tempBytes, keys, sortedKeys, ids, sortedIds, n, 0, 30, st),
-          "radix sort");
+    SortPairs30(keys, ids, sortedKeys, sortedIds, n, st);
     const std::uint32_t none = lbvh::kNone;
     Check(hipMemcpyAsync(parent_.as<void>(), &none, 4, hipMemcpyHostToDevice, st), "lbvh root");
     HDCX_LAUNCH(radixTreeKernel, g, sortedKeys, n, child2, parent_.as<std::uint32_t>(),
