@@ -13,14 +13,24 @@
 #include "rcache.h"
 
 #include <cmath>
+#include <map>
+#include <mutex>
 
 namespace hdcx {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kInferBlock = 256;
+// Two FP32 values rounded to BF16 as one 32-bit pair (one v_cvt_pk_bf16_f32).
+__device__ __forceinline__ std::uint32_t bf16Pair(float a, float b)
+{
+    return __builtin_bit_cast(std::uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
+}
+
+constexpr int kInferBlock = 512;  // 8 waves share one LDS copy of the weight tiles
 constexpr int kWeightTiles = 52;  // 1 KB MFMA-ready A tiles: W0 16, W1..W4 8 each, W5 4
 constexpr int kP0 = 136;          // LDS row pitch of the encoded inputs (bf16)
 constexpr int kPH = 72;           // LDS row pitch of hidden activations / deltas (bf16)
@@ -56,6 +66,7 @@ __device__ __forceinline__ void encodeHalf(const RcQuery& q, bf16x8 x[8])
     for (int s = 0; s < 8; ++s) {
         int prevAxis = -1, prevBand = -1;  // band carried in (sn, cs), per block
         float sn = 0.0f, cs = 1.0f;
+        std::uint32_t pairs[4];
 #pragma unroll
         for (int e = 0; e < 8; e += 2) {
             const int f = 16 * s + 8 * kH + e;
@@ -70,7 +81,15 @@ __device__ __forceinline__ void encodeHalf(const RcQuery& q, bf16x8 x[8])
                     sn = s2;
                     cs = c2;
                 } else {
-                    sincospif(ldexpf(pos ? p[a] : d[a - 3], k), &sn, &cs);
+                    // sin and cos of pi y, y = 2^k v (exact), at the hardware
+                    // rate: v_sin/v_cos take revolutions, and y / 2 reduced to
+                    // [0, 1) by its integer part is exact. Their absolute
+                    // error (a few 2^-21) is far below the BF16 rounding of
+                    // the features.
+                    const float t = 0.5f * ldexpf(pos ? p[a] : d[a - 3], k);
+                    const float r = t - floorf(t);
+                    sn = __builtin_amdgcn_sinf(r);
+                    cs = __builtin_amdgcn_cosf(r);
                 }
                 prevAxis = a;
                 prevBand = k;
@@ -85,9 +104,9 @@ __device__ __forceinline__ void encodeHalf(const RcQuery& q, bf16x8 x[8])
             } else {
                 v0 = v1 = 0.0f;
             }
-            x[s][e] = static_cast<__bf16>(v0);
-            x[s][e + 1] = static_cast<__bf16>(v1);
+            pairs[e >> 1] = bf16Pair(v0, v1);
         }
+        x[s] = __builtin_bit_cast(bf16x8, uint4{pairs[0], pairs[1], pairs[2], pairs[3]});
     }
 }
 
@@ -97,13 +116,23 @@ __device__ __forceinline__ void encode(const RcQuery& q, int h, bf16x8 x[8])
     else encodeHalf<1>(q, x);
 }
 
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+
+// BF16 ReLU of a layer's 16 outputs: rounded to BF16 first (v_cvt_pk_bf16_f32,
+// two values per instruction), then negative values cleared with a signed
+// 16-bit max against 0 on the packed bits (a negative BF16, -0 included, has
+// the sign bit set, so it compares below 0). Rounding commutes with
+// max(x, 0), so this is max(x, 0) rounded to BF16, in a third of the VALU
+// work of an FP32 max (with its NaN canonicalisation) per value.
 __device__ __forceinline__ void reluPack(const f32x16& a, bf16x8& lo, bf16x8& hi)
 {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        lo[e] = static_cast<__bf16>(fmaxf(a[e], 0.0f));
-        hi[e] = static_cast<__bf16>(fmaxf(a[e + 8], 0.0f));
-    }
+    lo = __builtin_bit_cast(bf16x8, uint4{bf16Pair(a[0], a[1]), bf16Pair(a[2], a[3]), bf16Pair(a[4], a[5]),
+                                          bf16Pair(a[6], a[7])});
+    hi = __builtin_bit_cast(bf16x8, uint4{bf16Pair(a[8], a[9]), bf16Pair(a[10], a[11]), bf16Pair(a[12], a[13]),
+                                          bf16Pair(a[14], a[15])});
+    const i16x8 zero = {};
+    lo = __builtin_bit_cast(bf16x8, __builtin_elementwise_max(__builtin_bit_cast(i16x8, lo), zero));
+    hi = __builtin_bit_cast(bf16x8, __builtin_elementwise_max(__builtin_bit_cast(i16x8, hi), zero));
 }
 
 __device__ __forceinline__ const __bf16* layerBase(const __bf16* w, int layer)
@@ -122,17 +151,20 @@ __device__ __forceinline__ bf16x8 weightTile(const __bf16* w, int layer, int t, 
     return asBf16x8(*reinterpret_cast<const uint4*>(src));
 }
 
-// Full forward pass of 32 queries; y holds outputs 0..2 in registers 0..2
-// of lanes 0..31. Hidden activations optionally stored for backward.
+// Full forward pass of 32 queries with the block's weight tiles in LDS; y
+// holds outputs 0..2 in registers 0..2 of lanes 0..31. Hidden activations
+// optionally stored for backward.
+// A scheduling barrier between layers keeps each layer's weight-tile reads
+// next to its MFMAs: hoisted all together they held 208 VGPRs and spilled.
 template <bool kStore>
-__device__ f32x16 forward(const __bf16* w, const uint4* ldsTiles, const bf16x8 x[8], int lane, __bf16* hs)
+__device__ __forceinline__ f32x16 forward(const uint4* ldsTiles, const bf16x8 x[8], int lane, __bf16* hs)
 {
     const int h = lane >> 5, q = lane & 31;
     f32x16 a0 = {}, a1 = {};
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        const bf16x8 w0 = ldsTiles ? asBf16x8(ldsTiles[s * 64 + lane]) : weightTile(w, 0, 0, s, lane);
-        const bf16x8 w1 = ldsTiles ? asBf16x8(ldsTiles[(8 + s) * 64 + lane]) : weightTile(w, 0, 1, s, lane);
+        const bf16x8 w0 = asBf16x8(ldsTiles[s * 64 + lane]);
+        const bf16x8 w1 = asBf16x8(ldsTiles[(8 + s) * 64 + lane]);
         a0 = mfma(w0, x[s], a0);
         a1 = mfma(w1, x[s], a1);
     }
@@ -140,6 +172,7 @@ __device__ f32x16 forward(const __bf16* w, const uint4* ldsTiles, const bf16x8 x
     reluPack(a0, hc[0], hc[1]);
     reluPack(a1, hc[2], hc[3]);
     for (int layer = 1; layer <= 4; ++layer) {
+        __builtin_amdgcn_sched_barrier(0);
         if (kStore) {
             __bf16* row = hs + (layer - 1) * 32 * kPH + q * kPH;
 #pragma unroll
@@ -153,8 +186,8 @@ __device__ f32x16 forward(const __bf16* w, const uint4* ldsTiles, const bf16x8 x
         a1 = f32x16{};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const bf16x8 w0 = ldsTiles ? asBf16x8(ldsTiles[(base + s) * 64 + lane]) : weightTile(w, layer, 0, s, lane);
-            const bf16x8 w1 = ldsTiles ? asBf16x8(ldsTiles[(base + 4 + s) * 64 + lane]) : weightTile(w, layer, 1, s, lane);
+            const bf16x8 w0 = asBf16x8(ldsTiles[(base + s) * 64 + lane]);
+            const bf16x8 w1 = asBf16x8(ldsTiles[(base + 4 + s) * 64 + lane]);
             a0 = mfma(w0, hc[s], a0);
             a1 = mfma(w1, hc[s], a1);
         }
@@ -169,10 +202,11 @@ __device__ f32x16 forward(const __bf16* w, const uint4* ldsTiles, const bf16x8 x
             *reinterpret_cast<bf16x8*>(row + 32 * t + 16 + 8 * h) = hc[2 * t + 1];
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
     f32x16 y = {};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const bf16x8 w5 = ldsTiles ? asBf16x8(ldsTiles[(48 + s) * 64 + lane]) : weightTile(w, 5, 0, s, lane);
+        const bf16x8 w5 = asBf16x8(ldsTiles[(48 + s) * 64 + lane]);
         y = mfma(w5, hc[s], y);
     }
     return y;
@@ -202,7 +236,7 @@ __device__ __forceinline__ void inferTile(const __bf16* w, const uint4* tiles, c
     if (valid) q = queries[qi];
     bf16x8 x[8];
     encode(q, lane >> 5, x);
-    const f32x16 y = forward<false>(w, tiles, x, lane, nullptr);
+    const f32x16 y = forward<false>(tiles, x, lane, nullptr);
     if (valid && lane < 32) {
         if (out) {
             out[qi * 3 + 0] = y[0];
@@ -718,13 +752,40 @@ void InitRcWeights(std::uint64_t seed, float* w)
     }
 }
 
+// Blocks of rcInferKernel<kOut> the current device keeps resident (its
+// occupancy per CU times the CU count; 0 when the query fails), per device.
+template <bool kOut>
+unsigned ResidentInferBlocks()
+{
+    static std::mutex mutex;
+    static std::map<int, unsigned> cache;
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return 0U;
+    const std::lock_guard<std::mutex> lock(mutex);
+    if (const auto found = cache.find(device); found != cache.end()) return found->second;
+    int perCu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, reinterpret_cast<const void*>(rcInferKernel<kOut>),
+                                                     kInferBlock, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0U;
+    const unsigned resident = static_cast<unsigned>(perCu > 0 ? perCu : 1) * static_cast<unsigned>(cus > 0 ? cus : 1);
+    cache.emplace(device, resident);
+    return resident;
+}
+
 hipError_t LaunchRcInfer(const RcNetwork& net, const RcQuery* queries, const std::uint32_t* count,
                          std::uint32_t maxCount, float* out, float* radiance, const float* throughput,
                          unsigned maxBlocks, hipStream_t st, const float* calibration)
 {
+    // One wave per 32 queries, grid-stride over the tiles, with at most the
+    // blocks the device keeps resident: every wave has about the same work,
+    // so a partial second round of blocks would idle most of the GPU.
     const unsigned tiles = (maxCount + 31U) / 32U;
-    unsigned blocks = (tiles + 3U) / 4U;
+    constexpr unsigned kWaves = kInferBlock / 64;
+    unsigned blocks = (tiles + kWaves - 1U) / kWaves;
     blocks = blocks == 0 ? 1U : (blocks < maxBlocks ? blocks : maxBlocks);
+    const unsigned resident = out ? ResidentInferBlocks<true>() : ResidentInferBlocks<false>();
+    if (resident > 0U && blocks > resident) blocks = resident;
     if (out)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(rcInferKernel<true>), dim3(blocks), dim3(kInferBlock), 0, st, net, queries,
                            count, maxCount, out, radiance, throughput, calibration);
