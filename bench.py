@@ -88,8 +88,8 @@ MISS_HIT_WRITE = 4        # on a hit: hit-queue entry
 # from its loads and stores:
 SHADE_VERTEX_READ = 80    # hit-queue entry 4, hit 16, ray origin 16 + direction 16, rng 4, wavelength 4,
                           # throughput 4, radiance 4, flags 4, MIS pdfs 8 (no medium in this variant)
-SHADE_HIT_READ = 136      # three vertex indices 12, positions 3 x 16, corner normals 3 x 16, UVs 3 x 8,
-                          # triangle material id 4
+SHADE_HIT_READ = 112      # the triangle's shading record: world positions 3 x 12, corner normals 3 x 12,
+                          # UVs 3 x 8, material id 4, padded to 7 x 16 (one 128-B line per triangle)
 SHADE_HIT_WRITE = 12      # rng, throughput, radiance
 SHADE_KEPT_WRITE = 56     # next ray origin + direction 32, flags 4, MIS pdfs 8, survivor-queue entry 4;
                           # lane pairing: extend-queue entry 4 + follower mask 4 (counted for every kept path)

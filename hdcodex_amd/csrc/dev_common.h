@@ -42,6 +42,9 @@ struct DeviceScene {
     const float2* texcoords;         // 3 per triangle
     const float4* normals;           // 3 per triangle
     const float4* emissive;          // 1 per triangle
+    // Per triangle, what shading reads of it in one 128-byte line (8 float4,
+    // see ShadingRecord): corner positions, corner normals, UVs, material.
+    const float4* shading;
     const GpuMaterialRecord* materials;
     const GpuLightRecord* lights;
     const TexDesc* textures;

@@ -328,6 +328,52 @@ __device__ __forceinline__ float4 cornerNormal(const DeviceScene& s, TriRef t, i
     return make_float4(w[0], w[1], w[2], 0.0f);
 }
 
+// What shading reads of a triangle, from its shading record (one 128-byte
+// line, DeviceScene::shading): world-space corners and corner normals
+// (through the instance transform, as triangleWorld and cornerNormal), UVs
+// and material (an instance's material overrides the triangle's). The same
+// values as the scene arrays, read from one line instead of five.
+struct TriShading {
+    V3 p0, p1, p2;
+    V3 n0, n1, n2;
+    float2 uv0, uv1, uv2;
+    std::uint32_t material;
+};
+__device__ __forceinline__ TriShading loadTriShading(const DeviceScene& s, TriRef t)
+{
+    const float4* rec = s.shading + static_cast<std::size_t>(t.tri) * 8U;
+    const float4 r0 = rec[0], r1 = rec[1], r2 = rec[2], r3 = rec[3], r4 = rec[4], r5 = rec[5], r6 = rec[6];
+    TriShading o;
+    o.p0 = v3(r0.x, r0.y, r0.z);
+    o.p1 = v3(r0.w, r1.x, r1.y);
+    o.p2 = v3(r1.z, r1.w, r2.x);
+    o.n0 = v3(r2.y, r2.z, r2.w);
+    o.n1 = v3(r3.x, r3.y, r3.z);
+    o.n2 = v3(r3.w, r4.x, r4.y);
+    o.uv0 = make_float2(r4.z, r4.w);
+    o.uv1 = make_float2(r5.x, r5.y);
+    o.uv2 = make_float2(r5.z, r5.w);
+    o.material = __float_as_uint(r6.x);
+    if (t.inst >= 0) {
+        const InstanceRecord& in = s.instances[t.inst];
+        float w[3];
+        TransformPoint(in.toWorld, o.p0.x, o.p0.y, o.p0.z, w);
+        o.p0 = v3(w[0], w[1], w[2]);
+        TransformPoint(in.toWorld, o.p1.x, o.p1.y, o.p1.z, w);
+        o.p1 = v3(w[0], w[1], w[2]);
+        TransformPoint(in.toWorld, o.p2.x, o.p2.y, o.p2.z, w);
+        o.p2 = v3(w[0], w[1], w[2]);
+        TransformNormal(in.normal, o.n0.x, o.n0.y, o.n0.z, w);
+        o.n0 = v3(w[0], w[1], w[2]);
+        TransformNormal(in.normal, o.n1.x, o.n1.y, o.n1.z, w);
+        o.n1 = v3(w[0], w[1], w[2]);
+        TransformNormal(in.normal, o.n2.x, o.n2.y, o.n2.z, w);
+        o.n2 = v3(w[0], w[1], w[2]);
+        if (in.material != 0xffffffffU) o.material = in.material;
+    }
+    return o;
+}
+
 __device__ __forceinline__ std::uint32_t triangleMaterial(const DeviceScene& s, TriRef t)
 {
     if (t.inst >= 0) {

@@ -397,6 +397,11 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     t->triMaterial.Upload(v.triangle_materials, static_cast<std::size_t>(v.triangle_count) * 4, st);
     t->texcoords.Upload(v.texcoords, static_cast<std::size_t>(v.triangle_count) * 24, st);
     t->normals.Upload(v.normals, static_cast<std::size_t>(v.triangle_count) * 48, st);
+    t->shading.Allocate(static_cast<std::size_t>(std::max<std::uint32_t>(v.triangle_count, 1)) * 128);
+    Check(LaunchShadingRecords(t->positions.as<float4>(), t->indices.as<std::uint32_t>(), t->normals.as<float4>(),
+                               t->texcoords.as<float2>(), t->triMaterial.as<std::uint32_t>(), v.triangle_count,
+                               t->shading.as<float4>(), st),
+          "shading records");
     const bool deviceEmissive = v.emissive == nullptr;
     if (!deviceEmissive) t->emissive.Upload(v.emissive, static_cast<std::size_t>(v.triangle_count) * 16, st);
     t->emissiveEntries = v.triangle_count;
@@ -586,6 +591,7 @@ void UploadScene(SceneStore* t, const hdcx_scene_view* view, hipStream_t st)
     s.texcoords = t->texcoords.as<float2>();
     s.normals = t->normals.as<float4>();
     s.emissive = t->emissive.as<float4>();
+    s.shading = t->shading.as<float4>();
     s.materials = t->materials.as<GpuMaterialRecord>();
     s.lights = t->lights.as<GpuLightRecord>();
     s.textures = t->textures.as<TexDesc>();

@@ -19,7 +19,7 @@ namespace hdcx {
 struct SceneStore {
     DeviceScene scene{};
     bool geometry = false;
-    DeviceBuffer nodes, tris, positions, indices, triMaterial, texcoords, normals, emissive, materials, lights,
+    DeviceBuffer nodes, tris, positions, indices, triMaterial, texcoords, normals, emissive, shading, materials, lights,
         textures, texels, importance, importancePool;
     // Light tables built on the device: per-material emission power, FP64
     // scan workspace, one success flag per dome light.

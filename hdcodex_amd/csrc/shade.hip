@@ -205,16 +205,15 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                 }
                 if (!mediumScattered) {
                     const TriRef tr = resolvePrim<k2L>(s, prim);
-                    V3 p0, p1, p2;
-                    triangleWorld(s, tr, p0, p1, p2);
+                    const TriShading ts = loadTriShading(s, tr);
+                    const V3 p0 = ts.p0, p1 = ts.p1, p2 = ts.p2;
                     const V3 gn = normalize(cross(p1 - p0, p2 - p0));
                     const bool front = dot(gn, rd) < 0.0f;
                     const V3 ogn = front ? gn : -gn;
                     const float w0 = 1.0f - bu - bv;
                     V3 normal;
                     {
-                        const float4 n0 = cornerNormal(s, tr, 0), n1 = cornerNormal(s, tr, 1), n2 = cornerNormal(s, tr, 2);
-                        V3 an = v3(n0) * w0 + v3(n1) * bu + v3(n2) * bv;
+                        V3 an = ts.n0 * w0 + ts.n1 * bu + ts.n2 * bv;
                         if (dot(an, an) < 1e-12f) an = gn;
                         else {
                             an = normalize(an);
@@ -222,13 +221,13 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers
                         }
                         normal = front ? an : -an;
                     }
-                    const float2 t0 = s.texcoords[tr.tri * 3], t1 = s.texcoords[tr.tri * 3 + 1], t2 = s.texcoords[tr.tri * 3 + 2];
+                    const float2 t0 = ts.uv0, t1 = ts.uv1, t2 = ts.uv2;
                     const float su = t0.x * w0 + t1.x * bu + t2.x * bv, sv = t0.y * w0 + t1.y * bu + t2.y * bv;
                     const V3 hp = ro + rd * distance;
                     // The record by value: the fields this vertex reads are
                     // loaded together here, instead of one dependent load
                     // after every texture-slot test below.
-                    const GpuMaterialRecord m = s.materials[triangleMaterial(s, tr)];
+                    const GpuMaterialRecord m = s.materials[ts.material];
 
                     // Material fetch (:1296-1437).
                     V3 baseRgb = v3(m.baseColorMetalness);

@@ -34,6 +34,15 @@ hipError_t LaunchEmissiveTable(const float4* positions, const std::uint32_t* ind
 hipError_t LaunchMaterialPower(const GpuMaterialRecord* materials, std::uint32_t count, const TexDesc* textures,
                                std::uint32_t textureCount, const unsigned char* texels, float* power, hipStream_t st);
 
+// The shading record of every triangle (8 float4 = one 128-byte line per
+// triangle): r0..r2 the corner positions p0.xyz p1.xyz p2.xyz, r2.w..r4.y the
+// corner normals n0.xyz n1.xyz n2.xyz, r4.z..r5.w the corner UVs, r6.x the
+// material id bits; the rest zero. The values are the scene arrays' own,
+// copied (positions through the vertex indices).
+hipError_t LaunchShadingRecords(const float4* positions, const std::uint32_t* indices, const float4* normals,
+                                const float2* texcoords, const std::uint32_t* triMaterial, std::uint32_t triangles,
+                                float4* out, hipStream_t st);
+
 // Doubles of scratch one dome's importance tables need (width x height texels).
 std::size_t ImportanceScratchDoubles(std::uint32_t width, std::uint32_t height);
 

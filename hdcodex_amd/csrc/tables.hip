@@ -5,6 +5,7 @@
 // thread.
 #include "tables.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace hdcx {
@@ -394,6 +395,42 @@ hipError_t LaunchImportanceTables(const TexDesc& texture, const unsigned char* t
     hipLaunchKernelGGL(importanceGuideKernel, dim3(GridFor(guides)), dim3(kThreads), 0, st, pool + d.marginal,
                        pool + d.conditional, d.width, d.height, d.rowShift, d.colShift,
                        reinterpret_cast<std::uint32_t*>(pool + d.rowGuide), reinterpret_cast<std::uint32_t*>(pool + d.colGuide));
+    return hipGetLastError();
+}
+
+// ---- Shading records (one 128-byte line per triangle) ----
+__global__ __launch_bounds__(256) void shadingRecordsKernel(const float4* __restrict__ positions,
+                                                            const std::uint32_t* __restrict__ indices,
+                                                            const float4* __restrict__ normals,
+                                                            const float2* __restrict__ texcoords,
+                                                            const std::uint32_t* __restrict__ triMaterial,
+                                                            std::uint32_t triangles, float4* __restrict__ out)
+{
+    for (std::uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < triangles; t += gridDim.x * blockDim.x) {
+        const std::size_t c = static_cast<std::size_t>(t) * 3U;
+        const float4 p0 = positions[indices[c]], p1 = positions[indices[c + 1]], p2 = positions[indices[c + 2]];
+        const float4 n0 = normals[c], n1 = normals[c + 1], n2 = normals[c + 2];
+        const float2 u0 = texcoords[c], u1 = texcoords[c + 1], u2 = texcoords[c + 2];
+        float4* r = out + static_cast<std::size_t>(t) * 8U;
+        r[0] = make_float4(p0.x, p0.y, p0.z, p1.x);
+        r[1] = make_float4(p1.y, p1.z, p2.x, p2.y);
+        r[2] = make_float4(p2.z, n0.x, n0.y, n0.z);
+        r[3] = make_float4(n1.x, n1.y, n1.z, n2.x);
+        r[4] = make_float4(n2.y, n2.z, u0.x, u0.y);
+        r[5] = make_float4(u1.x, u1.y, u2.x, u2.y);
+        r[6] = make_float4(__uint_as_float(triMaterial[t]), 0.0f, 0.0f, 0.0f);
+        r[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+hipError_t LaunchShadingRecords(const float4* positions, const std::uint32_t* indices, const float4* normals,
+                                const float2* texcoords, const std::uint32_t* triMaterial, std::uint32_t triangles,
+                                float4* out, hipStream_t st)
+{
+    if (triangles == 0) return hipSuccess;
+    const unsigned blocks = std::min<unsigned>((triangles + 255U) / 256U, 16384U);
+    hipLaunchKernelGGL(shadingRecordsKernel, dim3(blocks), dim3(256), 0, st, positions, indices, normals, texcoords,
+                       triMaterial, triangles, out);
     return hipGetLastError();
 }
 
