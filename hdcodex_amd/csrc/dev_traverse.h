@@ -27,6 +27,10 @@ struct RayPre {
     // Byte offset of the entry plane within a node's (lo, hi) plane pair of
     // each axis: 0 (lo) when the reciprocal is positive, 16 (hi) otherwise.
     std::uint32_t nearX, nearY, nearZ;
+    // Slab distances are fma(plane, i, c) per axis: c = -o * i, lowered for
+    // entry planes (cn*) and raised for exit planes (cf*) by 2^-22 |c|,
+    // four times the rounding error of c (see testNode).
+    float cnx, cny, cnz, cfx, cfy, cfz;
 };
 
 // Entry-plane offsets of a reciprocal direction (see RayPre).
@@ -55,6 +59,12 @@ __device__ __forceinline__ RayPre prepareRay(float ox, float oy, float oz, float
     r.dx = dx; r.dy = dy; r.dz = dz;
     r.ix = safeRcp(dx); r.iy = safeRcp(dy); r.iz = safeRcp(dz);
     r.nearX = nearPlaneOffset(r.ix); r.nearY = nearPlaneOffset(r.iy); r.nearZ = nearPlaneOffset(r.iz);
+    {
+        const float cx = -ox * r.ix, cy = -oy * r.iy, cz = -oz * r.iz;
+        const float ex = fabsf(cx) * 0x1p-22f, ey = fabsf(cy) * 0x1p-22f, ez = fabsf(cz) * 0x1p-22f;
+        r.cnx = cx - ex; r.cny = cy - ey; r.cnz = cz - ez;
+        r.cfx = cx + ex; r.cfy = cy + ey; r.cfz = cz + ez;
+    }
     const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
     int kz = 0;
     float m = ax;
@@ -71,10 +81,13 @@ __device__ __forceinline__ RayPre prepareRay(float ox, float oy, float oz, float
     return r;
 }
 
-// Watertight ray/triangle test; returns true with t in [tmin, tmax].
+// Watertight ray/triangle test; returns true with t in [tmin, tmax]. The
+// barycentrics are u = V / det and v = W / det; the closest-hit loops keep
+// V, W and det of the best candidate and divide once at the end (the same
+// quotients, without two divisions per accepted candidate).
 __device__ __forceinline__ bool intersectTriangle(const RayPre& r, const float4& p0, const float4& p1,
-                                                  const float4& p2, float tmin, float tmax, float& t, float& u,
-                                                  float& v)
+                                                  const float4& p2, float tmin, float tmax, float& t, float& vNum,
+                                                  float& wNum, float& det)
 {
 #pragma clang fp contract(off)
     const float ax0 = p0.x - r.ox, ay0 = p0.y - r.oy, az0 = p0.z - r.oz;
@@ -94,28 +107,38 @@ __device__ __forceinline__ bool intersectTriangle(const RayPre& r, const float4&
         W = static_cast<float>(bxd * ayd - byd * axd);
     }
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
-    const float det = U + V + W;
+    det = U + V + W;
     if (det == 0.0f) return false;
     const float az = r.sz * aK, bz = r.sz * bK, cz = r.sz * cK;
     const float T = U * az + V * bz + W * cz;
     t = T / det;
     if (!(t >= tmin && t <= tmax)) return false;
-    u = V / det;
-    v = W / det;
+    vNum = V;
+    wNum = W;
     return true;
 }
 
 // Slab test of the four children of a node; returns near distances
-// (+inf when missed). tfar is inflated by 4 ulp so that the two roundings
-// of each plane distance never cull a box the ray touches.
+// (+inf when missed).
 //
 // The node stores each axis as a lo plane and a hi plane of the four
 // children (16 bytes each). The ray's direction signs say which of the two
 // it enters through, so the entry and exit planes are loaded directly
 // (RayPre::near*) and their distances need no min/max: for a positive
-// reciprocal (lo - o) * i <= (hi - o) * i, for a negative one the reverse,
-// both exactly as rounded. The distances are those of the min/max form bit
-// for bit; an inverted box (lo > hi) gives tnear > tfar and is rejected.
+// reciprocal the entry plane is lo, for a negative one hi.
+//
+// Each plane distance is one fma, fma(p, i, c), instead of (p - o) * i.
+// Against the exact (p - o) * i (with the rounded reciprocal) the fma's
+// only errors are c's rounding, at most 2^-24 |c|, and its own final
+// rounding, 2^-24 of the result. The entry constant is c - 2^-22 |c| and the
+// exit constant c + 2^-22 |c| (both still beyond -o * i after their own
+// rounding), so entry distances are never above and exit distances never
+// below the exact ones except by the final rounding, which the 4-ulp
+// inflation of tfar covers. The test therefore never culls a box the exact
+// test keeps; which boxes it keeps only changes the order and number of
+// triangle tests, and the closest hit (ties broken by primitive id) does
+// not depend on either. A NaN distance (an overflowing constant) drops out
+// of the min/max, which keeps the box.
 //
 // Node addresses are a 32-bit byte offset (index x 128) from the array base
 // (node arrays stay below 4 GiB: the host caps them), which the compiler
@@ -141,9 +164,9 @@ __device__ __forceinline__ void testNode(const Bvh4Node* __restrict__ nodes, std
     const std::uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const float tx0 = (nxs[i] - r.ox) * r.ix, tx1 = (fxs[i] - r.ox) * r.ix;
-        const float ty0 = (nys[i] - r.oy) * r.iy, ty1 = (fys[i] - r.oy) * r.iy;
-        const float tz0 = (nzs[i] - r.oz) * r.iz, tz1 = (fzs[i] - r.oz) * r.iz;
+        const float tx0 = fmaf(nxs[i], r.ix, r.cnx), tx1 = fmaf(fxs[i], r.ix, r.cfx);
+        const float ty0 = fmaf(nys[i], r.iy, r.cny), ty1 = fmaf(fys[i], r.iy, r.cfy);
+        const float tz0 = fmaf(nzs[i], r.iz, r.cnz), tz1 = fmaf(fzs[i], r.iz, r.cfz);
         const float tnear = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, tmin));
         const float tfar = fminf(fminf(tx1, ty1), fminf(tz1, tmax));
         // Empty slots are rejected by reference: their planes are not a box.
@@ -245,6 +268,116 @@ struct TraversalStack {
         const std::uint32_t rr = ref[a]; ref[a] = ref[b]; ref[b] = rr; \
     }
 
+// Near-to-far order of a node's four children (5-comparator network).
+__device__ __forceinline__ void sortChildren(float tn[4], std::uint32_t ref[4])
+{
+    HDCX_SWAP_IF(0, 1) HDCX_SWAP_IF(2, 3) HDCX_SWAP_IF(0, 2) HDCX_SWAP_IF(1, 3) HDCX_SWAP_IF(1, 2)
+}
+
+// Closest-hit traversal state of one ray (see closestRound).
+struct ClosestState {
+    std::uint32_t cur;   // next node or leaf to visit (valid while !exhausted)
+    bool exhausted;      // nothing left to visit
+    bool found;
+    float bestT;
+    std::uint32_t bestPrim;
+    float bestV, bestW, bestDet;  // barycentric numerators and denominator of the best hit
+};
+
+__device__ __forceinline__ void startClosest(ClosestState& c, float tmax, TraversalStack& st)
+{
+    c.cur = 0;  // root is an inner node
+    c.exhausted = false;
+    c.found = false;
+    c.bestT = tmax;
+    c.bestPrim = 0xffffffffU;
+    c.bestV = c.bestW = 0.0f;
+    c.bestDet = 1.0f;
+    st.sp = 0;
+}
+
+// Takes the next leaf: `cur` (a leaf) becomes `leaf` and the next stack
+// entry becomes `cur` (or the traversal is exhausted).
+__device__ __forceinline__ void takeLeaf(std::uint32_t& leaf, ClosestState& c, TraversalStack& st)
+{
+    leaf = c.cur;
+    if (st.sp == 0) c.exhausted = true;
+    else c.cur = st.pop();
+}
+
+// One round of closest-hit traversal: inner nodes, then the leaves met.
+//
+// Speculative traversal (Aila and Laine, "Understanding the Efficiency of
+// Ray Traversal on GPUs", 2009): the inner-node loop postpones the first leaf
+// it meets and keeps descending until every lane still in the loop holds a
+// leaf, so node fetches and triangle tests run with more lanes of the wave.
+// Nodes visited past a postponed leaf are culled against the closest hit so
+// far, which only admits more of them; the closest hit (ties broken by
+// primitive id) does not depend on the visiting order. For a lone lane the
+// order is that of the plain loop (each leaf as it is met). Only triangles
+// with (t, prim) > (loT, loPrim) are accepted (loPrim = 0xffffffff: all).
+template <bool kCount>
+__device__ __forceinline__ void closestRound(const DeviceScene& s, const RayPre& r, float tmin, float loT,
+                                             std::uint32_t loPrim, ClosestState& c, TraversalStack& st,
+                                             unsigned long long& nodes, unsigned long long& tris)
+{
+    std::uint32_t leaf = 0;  // postponed leaf; 0 = none (leaf references carry kLeafFlag)
+    while (!c.exhausted && !(c.cur & kLeafFlag)) {
+        if (kCount) ++nodes;
+        float tn[4];
+        std::uint32_t ref[4];
+        testNode(s.nodes, c.cur, r, tmin, c.bestT, tn, ref);
+        sortChildren(tn, ref);
+        if (tn[0] != INFINITY) {
+            st.pushFar3(ref[3], tn[3] != INFINITY, ref[2], tn[2] != INFINITY, ref[1], tn[1] != INFINITY);
+            c.cur = ref[0];
+        } else if (st.sp == 0) {
+            c.exhausted = true;
+        } else {
+            c.cur = st.pop();
+        }
+        if (!c.exhausted && (c.cur & kLeafFlag) && leaf == 0) takeLeaf(leaf, c, st);
+        if (!__any(leaf == 0)) break;
+    }
+    if (leaf == 0 && !c.exhausted && (c.cur & kLeafFlag)) takeLeaf(leaf, c, st);
+    while (leaf != 0) {
+        const std::uint32_t first = (leaf >> 3) & 0x0fffffffU;
+        const std::uint32_t count = (leaf & 7U) + 1U;
+        for (std::uint32_t j = 0; j < count; ++j) {
+            const float4* tp = s.tris + static_cast<std::size_t>(first + j) * 3;
+            const float4 a = tp[0], b = tp[1], cc = tp[2];
+            const std::uint32_t prim = __float_as_uint(a.w);
+            if (kCount) ++tris;
+            float t, vn, wn, det;
+            if (intersectTriangle(r, a, b, cc, tmin, c.bestT, t, vn, wn, det)) {
+                const bool aboveLo = loPrim == 0xffffffffU || t > loT || (t == loT && prim > loPrim);
+                const bool better = !c.found || t < c.bestT || (t == c.bestT && prim < c.bestPrim);
+                if (aboveLo && better) {
+                    c.found = true;
+                    c.bestT = t;
+                    c.bestPrim = prim;
+                    c.bestV = vn;
+                    c.bestW = wn;
+                    c.bestDet = det;
+                }
+            }
+        }
+        leaf = 0;
+        if (!c.exhausted && (c.cur & kLeafFlag)) takeLeaf(leaf, c, st);
+    }
+}
+
+// The hit record of a finished traversal: (tmax, none, 0, 0) for a miss.
+__device__ __forceinline__ HitRec closestResult(const ClosestState& c)
+{
+    HitRec h;
+    h.t = c.bestT;
+    h.prim = c.bestPrim;
+    h.u = c.found ? c.bestV / c.bestDet : 0.0f;
+    h.v = c.found ? c.bestW / c.bestDet : 0.0f;
+    return h;
+}
+
 // Closest hit with a lexicographic lower bound: only triangles with
 // (t, prim) > (loT, loPrim) are accepted (loPrim = 0xffffffff with
 // loT = tmin accepts everything from tmin on). Used directly for opaque
@@ -257,52 +390,16 @@ __device__ bool traceClosest(const DeviceScene& s, const RayPre& r, float tmin, 
     best.prim = 0xffffffffU;
     best.u = best.v = 0.0f;
     if (s.nodeCount == 0) return false;
-    bool found = false;
-    st.sp = 0;
-    std::uint32_t cur = 0;  // root is an inner node
+    ClosestState c;
+    startClosest(c, tmax, st);
     unsigned long long nodes = 0, tris = 0;
-    for (;;) {
-        if (cur & kLeafFlag) {
-            const std::uint32_t first = (cur >> 3) & 0x0fffffffU;
-            const std::uint32_t count = (cur & 7U) + 1U;
-            for (std::uint32_t j = 0; j < count; ++j) {
-                const float4* tp = s.tris + static_cast<std::size_t>(first + j) * 3;
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                const std::uint32_t prim = __float_as_uint(a.w);
-                if (kCount) ++tris;
-                float t, u, v;
-                if (intersectTriangle(r, a, b, c, tmin, best.t, t, u, v)) {
-                    const bool aboveLo = loPrim == 0xffffffffU || t > loT || (t == loT && prim > loPrim);
-                    const bool better = !found || t < best.t || (t == best.t && prim < best.prim);
-                    if (aboveLo && better) {
-                        found = true;
-                        best.t = t;
-                        best.prim = prim;
-                        best.u = u;
-                        best.v = v;
-                    }
-                }
-            }
-        } else {
-            if (kCount) ++nodes;
-            float tn[4];
-            std::uint32_t ref[4];
-            testNode(s.nodes, cur, r, tmin, best.t, tn, ref);
-            HDCX_SWAP_IF(0, 1) HDCX_SWAP_IF(2, 3) HDCX_SWAP_IF(0, 2) HDCX_SWAP_IF(1, 3) HDCX_SWAP_IF(1, 2)
-            if (tn[0] != INFINITY) {
-                st.pushFar3(ref[3], tn[3] != INFINITY, ref[2], tn[2] != INFINITY, ref[1], tn[1] != INFINITY);
-                cur = ref[0];
-                continue;
-            }
-        }
-        if (st.sp == 0) break;
-        cur = st.pop();
-    }
+    while (!c.exhausted) closestRound<kCount>(s, r, tmin, loT, loPrim, c, st, nodes, tris);
+    best = closestResult(c);
     if (kCount && ctr) {
         atomicAdd(&ctr->nodes, nodes);
         atomicAdd(&ctr->triangles, tris);
     }
-    return found;
+    return c.found;
 }
 
 // Any hit in [tmin, tmax] (terminate on first hit).
@@ -322,8 +419,8 @@ __device__ bool traceAny(const DeviceScene& s, const RayPre& r, float tmin, floa
             for (std::uint32_t j = 0; j < count; ++j) {
                 const float4* tp = s.tris + static_cast<std::size_t>(first + j) * 3;
                 if (kCount) ++tris;
-                float t, u, v;
-                if (intersectTriangle(r, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) {
+                float t, vn, wn, det;
+                if (intersectTriangle(r, tp[0], tp[1], tp[2], tmin, tmax, t, vn, wn, det)) {
                     hitAny = true;
                     break;
                 }

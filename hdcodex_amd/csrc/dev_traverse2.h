@@ -122,6 +122,7 @@ __device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin
     best.u = best.v = 0.0f;
     if (s.nodeCount == 0) return false;
     bool found = false;
+    float bestV = 0.0f, bestW = 0.0f, bestDet = 1.0f;
     st.sp = 0;
     std::uint32_t cur = s.tlasRoot;
     bool inBlas = false;
@@ -162,16 +163,17 @@ __device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin
                 const std::uint32_t slotPrim = __float_as_uint(s.tris[static_cast<std::size_t>(lf + j) * 3].w);
                 const std::uint32_t prim = inst >= 0 ? base + (slotPrim - first) : slotPrim;
                 if (kCount) ++tris;
-                float t, u, v;
-                if (intersectTriangle(r, a, b, c, tmin, best.t, t, u, v)) {
+                float t, vn, wn, det;
+                if (intersectTriangle(r, a, b, c, tmin, best.t, t, vn, wn, det)) {
                     const bool aboveLo = loPrim == 0xffffffffU || t > loT || (t == loT && prim > loPrim);
                     const bool better = !found || t < best.t || (t == best.t && prim < best.prim);
                     if (aboveLo && better) {
                         found = true;
                         best.t = t;
                         best.prim = prim;
-                        best.u = u;
-                        best.v = v;
+                        bestV = vn;
+                        bestW = wn;
+                        bestDet = det;
                     }
                 }
             }
@@ -189,6 +191,10 @@ __device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin
         }
         if (st.sp == 0) break;
         cur = st.pop();
+    }
+    if (found) {
+        best.u = bestV / bestDet;
+        best.v = bestW / bestDet;
     }
     if (kCount && ctr) {
         atomicAdd(&ctr->nodes, nodes);
@@ -234,8 +240,8 @@ __device__ bool traceAny2L(const DeviceScene& s, const RayPre& r, float tmin, fl
                 float4 a, b, c;
                 slotWorld(s, lf + j, inst, a, b, c);
                 if (kCount) ++tris;
-                float t, u, v;
-                if (intersectTriangle(r, a, b, c, tmin, tmax, t, u, v)) {
+                float t, vn, wn, det;
+                if (intersectTriangle(r, a, b, c, tmin, tmax, t, vn, wn, det)) {
                     hitAny = true;
                     break;
                 }

@@ -1,10 +1,12 @@
 """BVH4 layout and a CPU emulation of the device traversal.
 
 ``emulate_closest`` follows the traversal kernel step by step (node decode,
-empty-slot rejection, slab test with the 4-ulp inflated far distance,
-near-to-far push order, leaf decode, watertight triangle test in float32)
-and asserts that every node, slot and stack access stays in bounds. Its
-hits must equal the oracle's brute-force closest hits bit for bit.
+empty-slot rejection, slab test with one fma per plane against per-axis
+entry/exit constants and the 4-ulp inflated far distance, near-to-far push
+order, leaf decode, watertight triangle test in float32) and asserts that
+every node, slot and stack access stays in bounds. Its hits must equal the
+oracle's brute-force closest hits bit for bit. The fma is taken in float64
+(the float32 product is exact there) and rounded to float32.
 """
 
 import ctypes
@@ -43,6 +45,20 @@ def _prepare(o, d):
     return kx, ky, kz, sx, sy, sz, inv
 
 
+def _fma(a, b, c):
+    with np.errstate(over="ignore", invalid="ignore"):
+        return F(np.float64(a) * np.float64(b) + np.float64(c))
+
+
+def _fmax(a, b):
+    # fmaxf / fminf: a NaN operand yields the other one.
+    return b if np.isnan(a) else (a if np.isnan(b) else max(a, b))
+
+
+def _fmin(a, b):
+    return b if np.isnan(a) else (a if np.isnan(b) else min(a, b))
+
+
 def _intersect(o, pre, p0, p1, p2, tmin, tmax):
     kx, ky, kz, sx, sy, sz, _ = pre
     a, b, c = (p0[:3] - o).astype(F), (p1[:3] - o).astype(F), (p2[:3] - o).astype(F)
@@ -72,6 +88,9 @@ def emulate_closest(nodes, slots, o, d, tmin=F(0.001), tmax=F(1e4), counter=None
     d = d.astype(F)
     pre = _prepare(o, d)
     inv = pre[6]
+    c = (-o * inv).astype(F)
+    e = (np.abs(c) * F(2.0 ** -22)).astype(F)
+    cn, cf = (c - e).astype(F), (c + e).astype(F)
     best_t, best_p, best_uv = F(tmax), EMPTY, (F(0), F(0))
     found = False
     stack = []
@@ -96,10 +115,13 @@ def emulate_closest(nodes, slots, o, d, tmin=F(0.001), tmax=F(1e4), counter=None
             tn, ref = [], []
             for i in range(4):
                 c = int(n["child"][i])
-                t0 = (n["lox"][i] - o[0]) * inv[0], (n["loy"][i] - o[1]) * inv[1], (n["loz"][i] - o[2]) * inv[2]
-                t1 = (n["hix"][i] - o[0]) * inv[0], (n["hiy"][i] - o[1]) * inv[1], (n["hiz"][i] - o[2]) * inv[2]
-                near = max(max(min(t0[0], t1[0]), min(t0[1], t1[1])), max(min(t0[2], t1[2]), tmin))
-                far = min(min(max(t0[0], t1[0]), max(t0[1], t1[1])), min(max(t0[2], t1[2]), best_t))
+                lo = (n["lox"][i], n["loy"][i], n["loz"][i])
+                hi = (n["hix"][i], n["hiy"][i], n["hiz"][i])
+                # Entry plane by the reciprocal's sign, exit plane the other.
+                tin = [_fma(lo[k] if inv[k] >= 0 else hi[k], inv[k], cn[k]) for k in range(3)]
+                tout = [_fma(hi[k] if inv[k] >= 0 else lo[k], inv[k], cf[k]) for k in range(3)]
+                near = _fmax(_fmax(tin[0], tin[1]), _fmax(tin[2], tmin))
+                far = _fmin(_fmin(tout[0], tout[1]), _fmin(tout[2], best_t))
                 hit = c != EMPTY and near <= F(far * F(1.00000048))
                 tn.append(F(near) if hit else F(np.inf))
                 ref.append(c)
@@ -161,6 +183,60 @@ def test_empty_slot_box_alone_would_pass_the_slab_test():
     o, inv = F(0.3), F(1 / 0.7)
     t0, t1 = (inf - o) * inv, (-inf - o) * inv
     assert max(min(t0, t1), F(0.001)) <= min(max(t0, t1), F(1e4))
+
+
+def test_fma_slab_test_never_culls_a_touched_box():
+    """The kernel's slab test (one fma per plane, entry/exit constants
+    c -/+ 2^-22 |c|, 4-ulp inflated far distance) accepts every box that the
+    exact test with the same rounded reciprocal accepts. Rays are aimed at a
+    box corner, edge or face from origins up to 1e6 away (large |c|, where a
+    plain fma with c = -o * i would lose the plane distance), including
+    axis-parallel directions (the 1e30 reciprocal)."""
+    rng = np.random.default_rng(7)
+    n = 200_000
+    scale = (F(10.0) ** rng.uniform(-2, 6, (n, 1))).astype(F)
+    lo = (rng.uniform(-1, 1, (n, 3)) * scale).astype(F)
+    hi = (lo + rng.uniform(0, 1, (n, 3)) * scale * F(0.1)).astype(F)
+    # Target: a corner, an edge point or a face point of the box, or (half
+    # of the rays) a point just outside it.
+    w = rng.integers(0, 3, (n, 3)).astype(F) / F(2)
+    w[n // 2:] += rng.choice([-1, 1], (n - n // 2, 3)) * (F(10.0) ** rng.uniform(-7, -2, (n - n // 2, 3))).astype(F)
+    q = (lo + (hi - lo) * w).astype(F)
+    d = rng.normal(size=(n, 3)).astype(F)
+    d[rng.random((n, 3)) < 0.05] = F(0)
+    d[np.all(d == 0, axis=1), 0] = F(1)
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(F)
+    o = (q - d * (scale * F(rng.uniform(0.5, 3.0, (n, 1))))).astype(F)
+    tiny = F(1e-30)
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        inv = (F(1) / np.where(np.abs(d) > tiny, d, np.copysign(tiny, d))).astype(F)
+        c = (-o * inv).astype(F)
+        e = (np.abs(c) * F(2.0 ** -22)).astype(F)
+        cn, cf = (c - e).astype(F), (c + e).astype(F)
+        entry = np.where(inv >= 0, lo, hi)
+        exit_ = np.where(inv >= 0, hi, lo)
+        tin = (entry.astype(np.float64) * inv + cn).astype(F)
+        tout = (exit_.astype(np.float64) * inv + cf).astype(F)
+        tmin, tmax = F(0.001), F(1e30)
+        near = np.fmax(np.fmax(tin[:, 0], tin[:, 1]), np.fmax(tin[:, 2], tmin))
+        far = np.fmin(np.fmin(tout[:, 0], tout[:, 1]), np.fmin(tout[:, 2], tmax))
+        kernel = near <= (far * F(1.00000048)).astype(F)
+        # Exact distances (float64: p - o and the product by the float32
+        # reciprocal carry far less error than the kernel's slack).
+        ein = (entry.astype(np.float64) - o) * inv.astype(np.float64)
+        eout = (exit_.astype(np.float64) - o) * inv.astype(np.float64)
+        exact = np.maximum(ein.max(axis=1), float(tmin)) <= np.minimum(eout.min(axis=1), float(tmax))
+    assert exact.sum() > n // 4
+    missed = np.flatnonzero(exact & ~kernel)
+    assert missed.size == 0, (missed[:5], o[missed[:1]], d[missed[:1]], lo[missed[:1]], hi[missed[:1]])
+    # The slack is small: a box that the exact test rejects by more than
+    # 2^-20 max|c| + 1e-6 of its exit distance (in front of the origin) is
+    # rejected by the kernel too.
+    gap = ein.max(axis=1) - eout.min(axis=1)
+    bound = 2.0 ** -20 * np.abs(c.astype(np.float64)).max(axis=1) + 1e-6 * np.abs(eout.min(axis=1))
+    clear = (eout.min(axis=1) > float(tmin)) & (gap > bound) & np.all(np.abs(d) > 1e-3, axis=1)
+    assert clear.sum() > n // 100
+    assert not np.any(kernel & clear)
 
 
 class _Geometry:
