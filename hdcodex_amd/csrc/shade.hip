@@ -16,6 +16,8 @@
 #include "dev_wavefront.h"
 #include "rcache.h"
 
+#include <type_traits>
+
 namespace hdcx {
 
 // Closest opaque hit used by the subsurface walk (:1130-1151).
@@ -137,8 +139,15 @@ __global__ __launch_bounds__(kPairBlock) void missKernel(Scenes scenes, PathBuff
 // it draws the same random numbers (the transmission pick is drawn and
 // ignored), so its images equal the full kernel's.
 // ---------------------------------------------------------------------------
+// Occupancy: the simple variant and the one-scene full variants are held to
+// 3 waves per SIMD (168 VGPRs, a few spilled; at the 188-252 registers they
+// need otherwise the kernel runs 2 waves): C1 shade 16.5 -> 15.2 ms per
+// frame, the 10,000-instance frame's 560 -> 520 ms; 4 waves were slower
+// (16.4 ms). The full variants over a scene table (the batch renderer) keep
+// their registers (3 waves measured 0.5% slower there).
 template <class Scenes, bool k2L, int kWalk>
-__global__ __launch_bounds__(kBlock) void shadeKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int bounce,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(
+    (kWalk == kShadeSimple || std::is_same<Scenes, SingleScene>::value) ? 3 : 1))) void shadeKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int bounce,
                                                      int lastBounce, CacheParams cp, int share)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
