@@ -112,7 +112,12 @@ __device__ __forceinline__ void slotWorld(const DeviceScene& s, std::uint32_t k,
     }
 
 // Closest hit over the instance tree, with the same lexicographic lower
-// bound as traceClosest. Primitive ids are flattened ids.
+// bound as traceClosest. Primitive ids are flattened ids. Speculative as
+// traceClosest: a triangle leaf of the current instance's BVH is postponed
+// while the lane keeps descending; the postponed leaf is tested before the
+// lane leaves the instance (kInstanceExit ends the inner-node loop, and
+// instance-tree items are only met after it), so it is always tested in its
+// own instance's frame.
 template <bool kCount>
 __device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin, float tmax, float loT,
                                std::uint32_t loPrim, HitRec& best, TraversalStack& st, TraversalCounters* ctr)
@@ -125,38 +130,43 @@ __device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin
     float bestV = 0.0f, bestW = 0.0f, bestDet = 1.0f;
     st.sp = 0;
     std::uint32_t cur = s.tlasRoot;
+    bool exhausted = false;
     bool inBlas = false;
     int inst = -1;
     std::uint32_t base = 0, first = 0;
     BoxRay box = worldBoxRay(r);
     unsigned long long nodes = 0, tris = 0;
-    for (;;) {
-        if (cur == kInstanceExit) {
-            inBlas = false;
-            inst = -1;
-            box = worldBoxRay(r);
-        } else if (cur & kLeafFlag) {
-            const std::uint32_t lf = (cur >> 3) & 0x0fffffffU;
-            if (!inBlas) {
-                // Item of the instance tree: enter its BVH.
-                st.push(kInstanceExit);
-                inBlas = true;
-                if (lf >= s.instanceCount) {
-                    inst = -1;
-                    base = 0;
-                    first = 0;
-                    cur = s.drawnRoot;
-                } else {
-                    inst = static_cast<int>(lf);
-                    const PrototypeRecord pr = s.prototypes[s.instances[lf].prototype];
-                    base = s.instanceBase[lf];
-                    first = pr.firstTriangle;
-                    box = objectBoxRay(s, lf, r, best.t);
-                    cur = pr.root;
-                }
-                continue;
+    // Next item into `leaf` when it is a triangle leaf of the current BVH.
+    auto takeTriangleLeaf = [&](std::uint32_t& leaf) {
+        if (!exhausted && inBlas && cur != kInstanceExit && (cur & kLeafFlag)) {
+            leaf = cur;
+            if (st.sp == 0) exhausted = true;
+            else cur = st.pop();
+        }
+    };
+    while (!exhausted) {
+        std::uint32_t leaf = 0;  // postponed triangle leaf; 0 = none
+        while (!exhausted && !(cur & kLeafFlag)) {
+            if (kCount) ++nodes;
+            float tn[4];
+            std::uint32_t ref[4];
+            testNodeBox(s.nodes, cur, box, tmin, best.t, tn, ref);
+            HDCX_SWAP_IF2(0, 1) HDCX_SWAP_IF2(2, 3) HDCX_SWAP_IF2(0, 2) HDCX_SWAP_IF2(1, 3) HDCX_SWAP_IF2(1, 2)
+            if (tn[0] != INFINITY) {
+                st.pushFar3(ref[3], tn[3] != INFINITY, ref[2], tn[2] != INFINITY, ref[1], tn[1] != INFINITY);
+                cur = ref[0];
+            } else if (st.sp == 0) {
+                exhausted = true;
+            } else {
+                cur = st.pop();
             }
-            const std::uint32_t count = (cur & 7U) + 1U;
+            if (leaf == 0) takeTriangleLeaf(leaf);
+            if (!__any(leaf == 0)) break;
+        }
+        if (leaf == 0) takeTriangleLeaf(leaf);
+        while (leaf != 0) {
+            const std::uint32_t lf = (leaf >> 3) & 0x0fffffffU;
+            const std::uint32_t count = (leaf & 7U) + 1U;
             for (std::uint32_t j = 0; j < count; ++j) {
                 float4 a, b, c;
                 slotWorld(s, lf + j, inst, a, b, c);
@@ -177,20 +187,36 @@ __device__ bool traceClosest2L(const DeviceScene& s, const RayPre& r, float tmin
                     }
                 }
             }
+            leaf = 0;
+            takeTriangleLeaf(leaf);
+        }
+        if (exhausted || !(cur & kLeafFlag)) continue;
+        if (cur == kInstanceExit) {
+            // Back to the instance tree.
+            inBlas = false;
+            inst = -1;
+            box = worldBoxRay(r);
+            if (st.sp == 0) exhausted = true;
+            else cur = st.pop();
         } else {
-            if (kCount) ++nodes;
-            float tn[4];
-            std::uint32_t ref[4];
-            testNodeBox(s.nodes, cur, box, tmin, best.t, tn, ref);
-            HDCX_SWAP_IF2(0, 1) HDCX_SWAP_IF2(2, 3) HDCX_SWAP_IF2(0, 2) HDCX_SWAP_IF2(1, 3) HDCX_SWAP_IF2(1, 2)
-            if (tn[0] != INFINITY) {
-                st.pushFar3(ref[3], tn[3] != INFINITY, ref[2], tn[2] != INFINITY, ref[1], tn[1] != INFINITY);
-                cur = ref[0];
-                continue;
+            // Item of the instance tree: enter its BVH.
+            const std::uint32_t lf = (cur >> 3) & 0x0fffffffU;
+            st.push(kInstanceExit);
+            inBlas = true;
+            if (lf >= s.instanceCount) {
+                inst = -1;
+                base = 0;
+                first = 0;
+                cur = s.drawnRoot;
+            } else {
+                inst = static_cast<int>(lf);
+                const PrototypeRecord pr = s.prototypes[s.instances[lf].prototype];
+                base = s.instanceBase[lf];
+                first = pr.firstTriangle;
+                box = objectBoxRay(s, lf, r, best.t);
+                cur = pr.root;
             }
         }
-        if (st.sp == 0) break;
-        cur = st.pop();
     }
     if (found) {
         best.u = bestV / bestDet;
