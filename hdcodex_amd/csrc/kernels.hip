@@ -247,12 +247,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? (k
 // speculative traversal round (closestRound). On the 1080p heightfield
 // frame: refill alone left extend at 15.8 ms per frame, speculation alone
 // gave 14.7 ms, both 13.3 ms (22 instead of 17 of 64 lanes active per VALU
-// instruction, 22% fewer memory instructions).
+// instruction, 22% fewer memory instructions). Held to 5 waves per SIMD (no
+// spills): 13.2 ms, against 13.4 at 6, 13.9 at 7 and 16.0 at 8.
 // ---------------------------------------------------------------------------
 constexpr int kRefillLanes = 16;
 
 template <bool kCount, class Scenes>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void extendRefillKernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void extendRefillKernel(
     Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing, TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
