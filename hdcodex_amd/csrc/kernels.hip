@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     std::uint64_t taken = 0;
     bool active = false;
     std::uint32_t pid = 0, followers = 0;
-    const DeviceScene* sc = &scenes.Get(0);
+    const DeviceScene* sc = nullptr;  // the active ray's scene (set at refill)
     ClosestState c;
     c.exhausted = true;
     RayPre r = prepareRay(0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f);

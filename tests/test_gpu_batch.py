@@ -83,6 +83,20 @@ def test_batch_equals_single_scene_renders(scenes, tuning):
         assert np.array_equal(film.view(np.uint32), ref.view(np.uint32))
 
 
+@pytest.mark.parametrize("tuning", [None, (3 * 8 * 300, 97, 64, True)])
+def test_opaque_batch_equals_single_scene_renders(scenes, tuning):
+    """A batch whose scenes are all opaque (Cornell box, the materials
+    scene, the heightfield) runs extend as the lane-refill kernel over the
+    scene table, each lane holding its own scene: films equal the
+    single-scene renders bit for bit."""
+    opaque = [scenes[0], scenes[1], scenes[3]]
+    assert all(scene.arrays()["opaque"] for scene, _, _, _ in opaque)
+    films, _ = _batch(opaque, tuning)
+    for (scene, w, h, spp), film in zip(opaque, films):
+        ref = _single(scene, w, h, spp)
+        assert np.array_equal(film.view(np.uint32), ref.view(np.uint32))
+
+
 def test_batch_progressive_mean(scenes):
     films, _ = _batch(scenes[:3], sample_index=5)
     for (scene, w, h, spp), film in zip(scenes[:3], films):
