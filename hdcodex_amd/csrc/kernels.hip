@@ -347,12 +347,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
 // ---------------------------------------------------------------------------
 // shadow: visibility of each queued path's shadow rays, added to its
 // radiance in slot order (shadowVisibility :855-889, meshLightVisible
-// :891-912). kOpaque as in extendKernel. Held to 7 waves per SIMD for
-// one-level opaque scenes (67 VGPRs, no spills; 9.0 ms per heightfield
-// frame against 9.6 at 8 waves), 6 otherwise.
+// :891-912). kOpaque as in extendKernel. Held to 6 waves per SIMD: on the
+// 1080p heightfield frame 7.75 ms against 7.90 at 7 waves and 8.8 at 8
+// (5 waves: the same as 6).
 // ---------------------------------------------------------------------------
 template <bool kCount, class Scenes, bool k2L, bool kOpaque>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu((kOpaque && !k2L) ? 7 : 6))) void shadowKernel(Scenes scenes, PathBuffers pb, bool shared, TraversalCounters* ctr)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void shadowKernel(Scenes scenes, PathBuffers pb, bool shared, TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
     HDCX_TRAVERSAL_STACK(st, stackMem);
