@@ -75,6 +75,35 @@ __device__ __forceinline__ void pairAppend(bool want, std::uint32_t pid, std::ui
     __syncthreads();
 }
 
+// pairAppend for threads that each hold three consecutive path ids
+// (pid0 .. pid0 + 2), appended in thread order.
+__device__ __forceinline__ void pairAppend3(bool want, std::uint32_t pid0, std::uint32_t* queue, std::uint32_t* counter,
+                                            std::uint32_t* waveCount, std::uint32_t* waveBase)
+{
+    constexpr unsigned kWaves = kPairBlock / 64U;
+    const std::uint32_t lane = laneId(), wave = threadIdx.x >> 6;
+    const unsigned long long mask = __ballot(want);
+    if (lane == 0U) waveCount[wave] = 3U * static_cast<std::uint32_t>(__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0U) {
+        std::uint32_t total = 0U;
+        for (unsigned k = 0; k < kWaves; ++k) total += waveCount[k];
+        std::uint32_t at = total ? atomicAdd(counter, total) : 0U;
+        for (unsigned k = 0; k < kWaves; ++k) {
+            waveBase[k] = at;
+            at += waveCount[k];
+        }
+    }
+    __syncthreads();
+    if (want) {
+        const std::uint32_t at = waveBase[wave] + 3U * static_cast<std::uint32_t>(__popcll(mask & ((1ULL << lane) - 1ULL)));
+        queue[at] = pid0;
+        queue[at + 1U] = pid0 + 1U;
+        queue[at + 2U] = pid0 + 2U;
+    }
+    __syncthreads();
+}
+
 // Leader / follower decision of one path among the lanes of its pixel
 // sample, from kWords words that must be equal bit for bit (the lane
 // pairing at the end of shade). Returns the follower mask of a path that

@@ -37,6 +37,11 @@ constexpr int kShadeWalkInline = 0, kShadeWalkDefer = 1, kShadeWalkResume = 2, k
 // queues the others for shade; run between extend and shade.
 hipError_t LaunchMiss(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                       unsigned maxBlocks, hipStream_t st);
+// The first bounce's miss (queue 0 = the launch's lane paths in order, three
+// per pixel sample with one camera ray): one thread per pixel sample, the
+// same results as LaunchMiss. maxWork: the launch's lane paths.
+hipError_t LaunchMissCamera(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                            hipStream_t st);
 // share: shade also pairs the lanes of a pixel sample with equal next rays
 // (the queue the next extend reads with kShareLanes) and with equal shadow
 // rays (LaunchShadow with shared = true traces each group once).

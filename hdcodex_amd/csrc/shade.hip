@@ -116,6 +116,49 @@ __global__ __launch_bounds__(kPairBlock) void missKernel(Scenes scenes, PathBuff
     }
 }
 
+// First-bounce miss: queue[0] is every lane path of the launch in order,
+// three per pixel sample, and the three lanes of a sample hold the same
+// camera ray and the same hit (their ray and random state do not depend on
+// the lane). One thread handles a sample: the environment's RGB (the dome
+// lookup) once, its spectrum at each lane's wavelength, the same sums as
+// missKernel per lane; the hit lanes go to the hit queue in path order.
+template <class Scenes>
+__global__ __launch_bounds__(kPairBlock) void missCameraKernel(Scenes scenes, PathBuffers pb)
+{
+    __shared__ std::uint32_t waveCount[kPairBlock / 64U], waveBase[kPairBlock / 64U];
+    const std::uint32_t samples = pb.counters[0] / 3U;
+    const std::uint32_t stride = gridDim.x * blockDim.x;
+    const std::uint32_t padded = (samples + stride - 1U) / stride * stride;
+    for (std::uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < padded; j += stride) {
+        bool hit = false;
+        const std::uint32_t pid0 = 3U * j;
+        if (j < samples) {
+            hit = __float_as_uint(pb.hit[pid0].y) != 0xffffffffU;
+            if (!hit) {
+                const DeviceScene& s = scenes.Get(pid0);
+                bool have = false;
+                V3 rgb = v3(0.0f);
+                for (std::uint32_t lane = 0; lane < 3U; ++lane) {
+                    const std::uint32_t pid = pid0 + lane;
+                    const std::uint32_t flags = pb.flags[pid];
+                    if (!(flags & kEnvOnMiss)) continue;
+                    if (!have) {
+                        rgb = environmentRgb(s, v3(pb.rayD[pid0]));
+                        have = true;
+                    }
+                    float w = 1.0f;
+                    if (!(flags & kPrevDelta)) {
+                        const float2 mis = pb.mis[pid];
+                        w = powerHeuristic(mis.x, mis.y);
+                    }
+                    pb.radiance[pid] += pb.throughput[pid] * w * illuminantSpectrum(rgb, pb.wavelength[pid]);
+                }
+            }
+        }
+        pairAppend3(hit, pid0, pb.extendQueue[0], pb.counters + kHitCounter * kCounterStride, waveCount, waveBase);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // shade: one path vertex (:1245-1879). Radiance that needs a visibility
 // test is queued as a shadow ray carrying its unoccluded contribution; the
@@ -896,6 +939,14 @@ hipError_t LaunchWalkT(const Scenes& s, const PathBuffers& pb, std::uint32_t max
     hipLaunchKernelGGL(HIP_KERNEL_NAME(walkKernel<Scenes, k2L>), dim3(StageGrid(walkKernel<Scenes, k2L>, maxWork, maxBlocks)),
                        dim3(kBlock), 0, st, s,
                        pb);
+    return hipGetLastError();
+}
+hipError_t LaunchMissCamera(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                            hipStream_t st)
+{
+    const unsigned blocks = StageGrid(missCameraKernel<SingleScene>, maxWork / 3U, maxBlocks, kPairBlock);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(missCameraKernel<SingleScene>), dim3(blocks), dim3(kPairBlock), 0, st,
+                       SingleScene{s}, pb);
     return hipGetLastError();
 }
 hipError_t LaunchMiss(const DeviceScene& s, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,

@@ -349,7 +349,10 @@ void Render(hdcx_tracer* t, const hdcx_camera& cam, std::uint32_t w, std::uint32
                 t->Timed(HDCX_KERNEL_EXTEND,
                          [&] { return LaunchExtend(t->scene, pb, in, sharing, paths, t->maxBlocks, ctr, st); });
                 const int last = b + 1 == bounces ? 1 : 0;
-                t->Timed(HDCX_KERNEL_MISS, [&] { return LaunchMiss(t->scene, pb, in, paths, t->maxBlocks, st); });
+                t->Timed(HDCX_KERNEL_MISS, [&] {
+                    return b == 0 ? LaunchMissCamera(t->scene, pb, paths, t->maxBlocks, st)
+                                  : LaunchMiss(t->scene, pb, in, paths, t->maxBlocks, st);
+                });
                 // With sharing on, shade also pairs the lanes of each pixel
                 // sample with equal next rays and equal shadow rays.
                 t->Timed(HDCX_KERNEL_SHADE, [&] {
