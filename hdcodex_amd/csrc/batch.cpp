@@ -412,7 +412,9 @@ struct hdcx_batch : PathPool {
                                         maxBlocks, nullptr, st),
                       "batch extend");
                 const int last = b + 1 == bounces ? 1 : 0;
-                Check(LaunchMissBatch(table, pb, in, maxPaths, maxBlocks, st), "batch miss");
+                Check(b == 0 ? LaunchMissCameraBatch(table, pb, maxPaths, maxBlocks, st)
+                             : LaunchMissBatch(table, pb, in, maxPaths, maxBlocks, st),
+                      "batch miss");
                 // Shade also pairs the lanes of each pixel sample with equal
                 // next rays and equal shadow rays.
                 Check(LaunchShadeBatch(table, twoLevel, pb, in, static_cast<int>(b), last, cp, walk, 1, maxPaths, maxBlocks,

@@ -62,6 +62,10 @@ hipError_t LaunchExtendBatch(const DeviceScene* table, bool twoLevel, bool opaqu
                              TraversalCounters* ctr, hipStream_t st);
 hipError_t LaunchMissBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st);
+// LaunchMissCamera for a batch wave (its lane paths are three per pixel
+// sample, each sample's lanes consecutive and in one scene).
+hipError_t LaunchMissCameraBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                                 hipStream_t st);
 hipError_t LaunchShadeBatch(const DeviceScene* table, bool twoLevel, const PathBuffers& pb, std::uint32_t in, int bounce,
                             int last, const CacheParams& cp, int walk, int share, std::uint32_t maxWork, unsigned maxBlocks,
                             hipStream_t st);

@@ -967,6 +967,14 @@ hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t
     return s.instanceCount ? LaunchWalkT<SingleScene, true>(SingleScene{s}, pb, maxWork, maxBlocks, st)
                            : LaunchWalkT<SingleScene, false>(SingleScene{s}, pb, maxWork, maxBlocks, st);
 }
+hipError_t LaunchMissCameraBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
+                                 hipStream_t st)
+{
+    const unsigned blocks = StageGrid(missCameraKernel<SceneTable>, maxWork / 3U, maxBlocks, kPairBlock);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(missCameraKernel<SceneTable>), dim3(blocks), dim3(kPairBlock), 0, st,
+                       SceneTable{table, pb.pathScene}, pb);
+    return hipGetLastError();
+}
 hipError_t LaunchMissBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t in, std::uint32_t maxWork,
                            unsigned maxBlocks, hipStream_t st)
 {
