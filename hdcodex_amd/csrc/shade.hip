@@ -944,6 +944,7 @@ hipError_t LaunchWalkT(const Scenes& s, const PathBuffers& pb, std::uint32_t max
 hipError_t LaunchMissCamera(const DeviceScene& s, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                             hipStream_t st)
 {
+    if (maxWork % 3U != 0U) return hipErrorInvalidValue;  // not three lanes per pixel sample
     const unsigned blocks = StageGrid(missCameraKernel<SingleScene>, maxWork / 3U, maxBlocks, kPairBlock);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(missCameraKernel<SingleScene>), dim3(blocks), dim3(kPairBlock), 0, st,
                        SingleScene{s}, pb);
@@ -970,6 +971,7 @@ hipError_t LaunchWalk(const DeviceScene& s, const PathBuffers& pb, std::uint32_t
 hipError_t LaunchMissCameraBatch(const DeviceScene* table, const PathBuffers& pb, std::uint32_t maxWork, unsigned maxBlocks,
                                  hipStream_t st)
 {
+    if (maxWork % 3U != 0U) return hipErrorInvalidValue;  // not three lanes per pixel sample
     const unsigned blocks = StageGrid(missCameraKernel<SceneTable>, maxWork / 3U, maxBlocks, kPairBlock);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(missCameraKernel<SceneTable>), dim3(blocks), dim3(kPairBlock), 0, st,
                        SceneTable{table, pb.pathScene}, pb);
