@@ -178,14 +178,13 @@ __device__ bool traceStochastic(const DeviceScene& s, const RayPre& r, std::uint
 // ---------------------------------------------------------------------------
 // kOpaque: every scene of the launch is opaque (the host knows), so the
 // cut-out candidate loop is compiled out and its registers with it.
-// Occupancy, measured on the 1080p heightfield and 10,000-instance frames:
-// two-level extend runs best held to 7 waves per SIMD when opaque (6
-// otherwise; 4 or 5 waves without spills are 3-8% slower). One-level extend
-// runs best as the generic kernel held to 6 waves (80 VGPRs; the opaque
-// kernel at 7 or 8 waves was 3-4% slower), so the host launches no
-// one-level opaque kernel.
+// Occupancy, measured on the 10,000-instance frame: with speculative
+// traversal the two-level kernel runs best held to 6 waves per SIMD
+// (extend 517 ms per frame at 7 waves, 453 at 6, 466 at 5). One-level
+// opaque scenes use extendRefillKernel; the generic one-level kernel is held
+// to 6 waves (80 VGPRs).
 template <bool kCount, class Scenes, bool k2L, bool kOpaque>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(k2L ? (kOpaque ? 7 : 6) : 6))) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void extendKernel(Scenes scenes, PathBuffers pb, std::uint32_t in, int sharing,
                                                       TraversalCounters* ctr)
 {
     __shared__ std::uint32_t stackMem[kLdsStack * kBlock];
