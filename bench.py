@@ -788,6 +788,12 @@ def main() -> int:
                                    "alg_bytes_per_launch": v["alg_bytes_per_launch"],
                                    "avg_launch_ms": v["avg_launch_ms"], "traffic": measured_traffic(k)}
                                for k, v in roof.items() if k != dominant}}
+        for k, v in roofline["others"].items():
+            if v["frac"] > 1.0:
+                # Node and triangle reads are algorithmic bytes whether L2 or
+                # HBM serves them; the counter traffic is what HBM moved.
+                v["note"] = ("algorithmic rate above the HBM peak: L2/MALL serve most of the node and triangle "
+                             "reads; HBM traffic per launch is `traffic`")
 
     result = {
         "metric": METRIC,
